@@ -1,0 +1,212 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check is chosen so that a plausible mistake in the oracle -- a dropped
+term, a wrong sign or index, a transposed operand, a wrong constant -- fails
+one of them.  Literal values come from closed forms, not from the oracle.
+"""
+import numpy as np
+import pytest
+
+from helios_amd import molecules as M
+from oracle import ccsd, dressed, fci, ppp, scf, spin_ccsd
+from tests._molprep import prepare, random_amplitudes
+
+# Closed-form two-site PPP/FCI correlation energies, E = 2|b| - sqrt(((U-g12)/2)^2 + 4b^2)
+# with U = 11.13 eV, b = -2.4 eV and the Ohno gamma with a0*Eh = 14.399645 eV*A.
+ETHYLENE = {1.35: (7.700964636662581, -0.297015879978662),
+            1.40: (7.5538620908960965, -0.322225159657911)}
+
+
+def test_gamma_two_site_hand_value():
+    for r, (g12, _) in ETHYLENE.items():
+        G = ppp.gamma(np.array([[0, 0, 0], [r, 0, 0.0]]), [11.13, 11.13])
+        assert abs(G[0, 1] - g12) < 1e-12
+        assert abs(G[0, 0] - 11.13) < 1e-14
+    # the rounded constant 14.397 of the config text gives a measurably different gamma
+    k = 1.0 / 14.397
+    g_rounded = 11.13 / np.sqrt(1 + (k * 11.13 * 1.35) ** 2)
+    assert abs(g_rounded - ETHYLENE[1.35][0]) > 1e-4
+
+
+def test_gamma_hubbard_and_core():
+    x = np.array([[0, 0, 0], [1.4, 0, 0], [2.8, 0, 0.0]])
+    G = ppp.gamma(x, [11.13] * 3, "hubbard")
+    assert np.allclose(G, np.diag([11.13] * 3))
+    Go = ppp.gamma(x, [11.13] * 3)
+    H = ppp.core(ppp.connect(np.zeros(3), [(0, 1, -2.4), (1, 2, -2.4)], 3), Go, [1, 1, 1])
+    assert abs(H[0, 0] - (-Go[0, 1] - Go[0, 2])) < 1e-14
+    assert abs(H[1, 1] - (-Go[1, 0] - Go[1, 2])) < 1e-14
+    assert H[0, 1] == -2.4 and H[0, 2] == 0.0
+
+
+@pytest.mark.parametrize("mol", [M.octatetraene(), M.batch_member(5)])
+def test_scf_stationary(mol):
+    p = prepare(mol)
+    o = p["nocc"]
+    C, F = p["C"], p["F"]
+    assert p["scf"]["converged"]
+    assert np.abs(C.T @ C - np.eye(p["n"])).max() < 1e-12
+    assert np.abs(F[:o, o:]).max() < 1e-9            # Brillouin: occupied-virtual Fock vanishes
+    assert np.abs(np.diag(F) - p["E"]).max() < 1e-9
+    D = C[:, :o] @ C[:, :o].T
+    Fao = scf.fockian(p["core"], p["G"], D)
+    assert np.abs(Fao @ D - D @ Fao).max() < 1e-9
+
+
+def test_scf_energy_matches_direct_roothaan():
+    mol = M.batch_member(11)
+    p = prepare(mol)
+    o = mol.nocc
+    # independent: Roothaan with Pulay commutator extrapolation, numpy eigh
+    D = p["C"][:, :o] @ p["C"][:, :o].T * 0.0 + np.eye(mol.n) * 0.5
+    for _ in range(200):
+        F = scf.fockian(p["core"], p["G"], D)
+        _, V = np.linalg.eigh(F)
+        Dn = V[:, :o] @ V[:, :o].T
+        D = 0.5 * D + 0.5 * Dn
+    F = scf.fockian(p["core"], p["G"], D)
+    _, V = np.linalg.eigh(F)
+    D = V[:, :o] @ V[:, :o].T
+    F = scf.fockian(p["core"], p["G"], D)
+    e = scf.energy(p["core"], p["G"], D, F)[0]
+    assert abs(e - p["eref"]) < 1e-8
+
+
+def test_huckel_alternant_pairing():
+    mol = M.batch_member(7)
+    h = ppp.connect(np.zeros(mol.n), mol.bonds, mol.n)
+    e = np.linalg.eigvalsh(h)
+    assert np.abs(e + e[::-1]).max() < 1e-10
+
+
+@pytest.mark.parametrize("r", sorted(ETHYLENE))
+def test_two_electron_closed_form_and_fci(r):
+    p = prepare(M.ethylene(r))
+    res = ccsd.solve(p["C"], p["F"], p["G"], 1, p["eref"], eps=1e-12, diis_scale=True)
+    assert res["status"] == 1
+    ecorr = res["energy"][0] - p["eref"]
+    assert abs(ecorr - ETHYLENE[r][1]) < 1e-11
+    assert abs(res["energy"][0] - fci.fci_energy(p["C"], p["G"], p["core"], 1)) < 1e-11
+
+
+@pytest.mark.parametrize("mol", [M.octatetraene(), M.batch_member(21)])
+def test_spatial_equals_spin_orbital_ccsd(mol):
+    if mol.n > 14:
+        mol = M.polyene(10)
+    p = prepare(mol)
+    res = ccsd.solve(p["C"], p["F"], p["G"], mol.nocc, p["eref"], eps=1e-11, diis_scale=True)
+    assert res["status"] == 1
+    e_spin = spin_ccsd.ccsd(p["C"], p["G"], p["F"], mol.nocc, p["eref"], eps=1e-12)[0]
+    assert abs(res["energy"][0] - e_spin) < 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_residual_identities_at_random_amplitudes(seed):
+    """Reference-form d1/d2 == textbook T1-dressed Omega, and Omega == the
+    spin-orbital residual on closed-shell amplitudes (alpha-beta block)."""
+    mol = M.polyene(6) if seed == 0 else M.batch_member(2)
+    if mol.n > 10:
+        mol = M.polyene(10)
+    p = prepare(mol)
+    o, v = mol.nocc, mol.n - mol.nocc
+    t1, t2 = random_amplitudes(o, v, seed)
+    B = ccsd.Blocks(ccsd.rint(p["C"], p["G"]), o)
+    d1 = ccsd.singles(p["F"], B, t1, t2)
+    d2 = ccsd.doubles(p["F"], B, t1, t2)
+    w1, w2 = dressed.omega(p["C"], p["F"], p["G"], o, t1, t2)
+    assert np.abs(d1 - w1).max() < 1e-12 * max(1.0, np.abs(d1).max())
+    assert np.abs(d2 - dressed.contravariant(w2)).max() < 1e-12 * max(1.0, np.abs(d2).max())
+    T1, T2 = spin_ccsd.spatial_to_spin(t1, t2)
+    r1, r2 = spin_ccsd.residual(p["C"], p["G"], p["F"], o, T1, T2)
+    ia = np.arange(o)
+    aa = np.arange(v)
+    ab_block = r2[2 * ia[:, None, None, None], 2 * ia[None, :, None, None] + 1,
+                  2 * aa[None, None, :, None], 2 * aa[None, None, None, :] + 1]
+    assert np.abs(ab_block - w2).max() < 1e-12 * max(1.0, np.abs(w2).max())
+    assert np.abs(r1[2 * ia[:, None], 2 * aa[None, :]] - w1).max() < 1e-12 * max(1.0, np.abs(w1).max())
+
+
+def test_guess_is_contravariant_mp2():
+    p = prepare(M.polyene(6))
+    o = 3
+    B = ccsd.Blocks(ccsd.rint(p["C"], p["G"]), o)
+    t1, t2 = ccsd.guess(p["F"], B, o)
+    tm = ccsd.mp2_amplitudes(p["C"], p["G"], p["E"], o)
+    assert np.abs(t2 - (2 * tm - tm.transpose(0, 1, 3, 2))).max() < 1e-12
+    assert np.abs(t1).max() < 1e-9
+
+
+def test_u_to_zero_limit():
+    # Hubbard gamma: with Ohno gamma a small U makes gamma nearly constant, and a
+    # constant gamma gives (pq|rs) = U delta_pq delta_rs, which correlates nothing
+    mol = M.polyene(6)
+    e = []
+    for U in (0.02, 0.04):
+        mol.U = U
+        p = prepare(mol, "hubbard")
+        res = ccsd.solve(p["C"], p["F"], p["G"], 3, p["eref"], eps=1e-13, diis_scale=True)
+        e.append(res["energy"][0] - p["eref"])
+    assert abs(e[0]) < 1e-4
+    assert abs(e[1] / e[0] - 4.0) < 0.05          # E_corr = O(U^2)
+
+
+def test_size_extensive_hubbard():
+    a = M.ethylene(1.35)
+    b = M.Molecule("pair", np.array([[0, 0, 0], [1.35, 0, 0], [0, 10.0, 0], [1.35, 10.0, 0]]),
+                   [(0, 1, -2.4), (2, 3, -2.4)])
+    ea = prepare(a, "hubbard")
+    eb = prepare(b, "hubbard")
+    ra = ccsd.solve(ea["C"], ea["F"], ea["G"], 1, ea["eref"], eps=1e-12, diis_scale=True)
+    rb = ccsd.solve(eb["C"], eb["F"], eb["G"], 2, eb["eref"], eps=1e-12, diis_scale=True)
+    assert abs((rb["energy"][0] - eb["eref"]) - 2 * (ra["energy"][0] - ea["eref"])) < 1e-10
+
+
+def test_orbital_rotation_invariance():
+    mol = M.polyene(8)
+    p = prepare(mol)
+    o = 4
+    rng = np.random.default_rng(5)
+    Qo, _ = np.linalg.qr(rng.standard_normal((o, o)))
+    Qv, _ = np.linalg.qr(rng.standard_normal((o, o)))
+    Q = np.zeros((8, 8))
+    Q[:o, :o], Q[o:, o:] = Qo, Qv
+    C2 = p["C"] @ Q
+    F2 = Q.T @ p["F"] @ Q
+    r1 = ccsd.solve(p["C"], p["F"], p["G"], o, p["eref"], eps=1e-11, diis_scale=True)
+    r2 = ccsd.solve(C2, F2, p["G"], o, p["eref"], eps=1e-11, diis_scale=True, maxiter=500)
+    assert r2["status"] == 1
+    assert abs(r1["energy"][0] - r2["energy"][0]) < 1e-9
+
+
+def test_iterator_statuses():
+    p = prepare(M.polyene(6))
+    cc = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"])
+    st, it, _, _ = ccsd.iterator(cc.iterate, cc.energy, 1, 1e-10)
+    assert (st, it) == (-1, 1)
+    # no DIIS: the x3 overshoot of the 2-1 update diverges
+    cc = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"], diis=False)
+    st, _, _, _ = ccsd.iterator(cc.iterate, cc.energy, 200, 1e-10, divergence=1e2)
+    assert st == -2
+    cc = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"])
+    st, it, _, _ = ccsd.iterator(cc.iterate, cc.energy, 200, 1e-10, stop_flag=lambda i: i == 3)
+    assert (st, it) == (-4, 3)
+    # the unscaled reference DIIS stalls near 1e-10 on octatetraene and trips the
+    # stagnation test after 200 iterations; the scaled solve converges
+    q = prepare(M.octatetraene())
+    cc = ccsd.CCSD(q["C"], q["F"], q["G"], 4, q["eref"])
+    st, it, _, _ = ccsd.iterator(cc.iterate, cc.energy, 400, 1e-11)
+    assert st == -3 and it == 201
+    cc = ccsd.CCSD(q["C"], q["F"], q["G"], 4, q["eref"], diis_scale=True)
+    st, _, _, _ = ccsd.iterator(cc.iterate, cc.energy, 400, 1e-11)
+    assert st == 1
+
+
+def test_diis_scaling_keeps_coefficients():
+    """Scaled and unscaled extrapolations agree while B is well conditioned."""
+    p = prepare(M.polyene(6))
+    a = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"])
+    b = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"], diis_scale=True)
+    for it in range(1, 6):
+        a.iterate(it, 0.0)
+        b.iterate(it, 0.0)
+    assert np.abs(a.t2 - b.t2).max() < 1e-10
