@@ -1,0 +1,196 @@
+"""ctypes binding of libhelios_ccsd (include/helios_ccsd.h).
+
+This is the binding a Python caller would add; it loads the in-tree library
+and raises if it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhelios_ccsd.so")
+
+HCC_FP64 = 0
+HCC_MIXED = 1
+GUESS_REFERENCE = 0
+GUESS_MP2 = 1
+GUESS_LEARNED = 2
+
+_P = ctypes.POINTER
+_d = ctypes.c_double
+_i = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/helios_ccsd.h
+SIGNATURES = {
+    "hcc_create": (_i, [_P(_vp), _i, _i, _i]),
+    "hcc_set_options": (_i, [_vp, _d, _d, _i, _i]),
+    "hcc_set_molecule": (_i, [_vp, _i, _i, _P(_d), _P(_d), _P(_d), _d]),
+    "hcc_guess": (_i, [_vp, _i]),
+    "hcc_iterate": (_i, [_vp, _i, _d, _P(_d)]),
+    "hcc_energy": (_i, [_vp, _P(_d)]),
+    "hcc_get_amplitudes": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_set_amplitudes": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_residual": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_solve": (_i, [_vp, _i, _i, _d, _P(_d), _P(_i), _P(_i)]),
+    "hcc_batch_load": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _P(_d)]),
+    "hcc_batch_solve": (_i, [_vp, _i, _i, _d]),
+    "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
+    "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_destroy": (None, [_vp]),
+    "hcc_last_error": (ctypes.c_char_p, [_vp]),
+    "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the library (once).  Raises OSError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"libhelios_ccsd.so not built at {path}; run helios_amd.build")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+def _ip(a):
+    return a.ctypes.data_as(_P(_i))
+
+
+def fortran(a):
+    """Column-major float64 copy (the boundary's storage order)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+class HccError(RuntimeError):
+    pass
+
+
+class Context:
+    """Thin owner of an hcc_ctx; every method is one C-ABI call."""
+
+    def __init__(self, device=0, precision=HCC_FP64, diis_steps=20):
+        self.lib = load()
+        self.h = _vp()
+        rc = self.lib.hcc_create(ctypes.byref(self.h), device, precision, diis_steps)
+        if rc != 0:
+            raise HccError(f"hcc_create failed ({rc})")
+        self.o = self.v = None
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.hcc_last_error(self.h)
+            raise HccError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.hcc_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, t1_step=1.0, t2_step=1.0, projection_type=2, diis=True):
+        self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
+                    "hcc_set_options")
+
+    def set_molecule(self, C, F, G, nocc, eref):
+        n = C.shape[0]
+        self._C, self._F, self._G = fortran(C), fortran(F), fortran(G)
+        self._check(self.lib.hcc_set_molecule(self.h, n, nocc, _dp(self._C), _dp(self._F), _dp(self._G),
+                                              float(eref)), "hcc_set_molecule")
+        self.o, self.v = nocc, n - nocc
+
+    def guess(self, kind=GUESS_REFERENCE):
+        self._check(self.lib.hcc_guess(self.h, kind), "hcc_guess")
+
+    def iterate(self, iteration, eps):
+        acc = np.zeros(5)
+        self._check(self.lib.hcc_iterate(self.h, iteration, eps, _dp(acc)), "hcc_iterate")
+        return acc
+
+    def energy(self):
+        e = np.zeros(5)
+        self._check(self.lib.hcc_energy(self.h, _dp(e)), "hcc_energy")
+        return e
+
+    def amplitudes(self):
+        """(t1[i,a], t2[i,j,a,b]) as numpy arrays (converted from Fortran order)."""
+        t1 = np.zeros((self.o, self.v), order="F")
+        t2 = np.zeros((self.o, self.o, self.v, self.v), order="F")
+        self._check(self.lib.hcc_get_amplitudes(self.h, _dp(t1), _dp(t2)), "hcc_get_amplitudes")
+        return np.ascontiguousarray(t1), np.ascontiguousarray(t2)
+
+    def set_amplitudes(self, t1, t2):
+        a, b = fortran(t1), fortran(t2)
+        self._check(self.lib.hcc_set_amplitudes(self.h, _dp(a), _dp(b)), "hcc_set_amplitudes")
+
+    def residual(self):
+        d1 = np.zeros((self.o, self.v), order="F")
+        d2 = np.zeros((self.o, self.o, self.v, self.v), order="F")
+        self._check(self.lib.hcc_residual(self.h, _dp(d1), _dp(d2)), "hcc_residual")
+        return np.ascontiguousarray(d1), np.ascontiguousarray(d2)
+
+    def solve(self, guess=GUESS_REFERENCE, maxiter=2000, eps=1e-10):
+        e = np.zeros(5)
+        it = ctypes.c_int()
+        st = ctypes.c_int()
+        self._check(self.lib.hcc_solve(self.h, guess, maxiter, eps, _dp(e), ctypes.byref(it), ctypes.byref(st)),
+                    "hcc_solve")
+        return e, it.value, st.value
+
+    def batch_load(self, mols):
+        """mols: list of dicts with C, F, G (n x n), nocc, eref."""
+        n = np.array([m["C"].shape[0] for m in mols], dtype=np.int32)
+        no = np.array([m["nocc"] for m in mols], dtype=np.int32)
+        C = np.concatenate([fortran(m["C"]).ravel(order="F") for m in mols])
+        F = np.concatenate([fortran(m["F"]).ravel(order="F") for m in mols])
+        G = np.concatenate([fortran(m["G"]).ravel(order="F") for m in mols])
+        er = np.array([m["eref"] for m in mols], dtype=np.float64)
+        self._check(self.lib.hcc_batch_load(self.h, len(mols), _ip(n), _ip(no), _dp(C), _dp(F), _dp(G), _dp(er)),
+                    "hcc_batch_load")
+        self.nmol = len(mols)
+
+    def batch_solve(self, guess=GUESS_MP2, maxiter=200, eps=1e-7):
+        self._check(self.lib.hcc_batch_solve(self.h, guess, maxiter, eps), "hcc_batch_solve")
+
+    def batch_results(self):
+        e = np.zeros((self.nmol, 5))
+        it = np.zeros(self.nmol, dtype=np.int32)
+        st = np.zeros(self.nmol, dtype=np.int32)
+        self._check(self.lib.hcc_batch_results(self.h, _dp(e), _ip(it), _ip(st)), "hcc_batch_results")
+        return e, it, st
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        its = ctypes.c_double()
+        self._check(self.lib.hcc_last_timing(self.h, ctypes.byref(ms), ctypes.byref(its)), "hcc_last_timing")
+        return ms.value, its.value
+
+
+def selftest_gemm(A, B, C=None, alpha=1.0, beta=0.0, precision=HCC_FP64):
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    m, k = A.shape
+    n = B.shape[1]
+    C = np.zeros((m, n)) if C is None else np.ascontiguousarray(C, dtype=np.float64).copy()
+    rc = lib.hcc_selftest_gemm(m, n, k, _dp(A), _dp(B), _dp(C), alpha, beta, precision)
+    if rc != 0:
+        raise HccError(f"hcc_selftest_gemm failed ({rc})")
+    return C

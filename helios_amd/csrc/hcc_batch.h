@@ -1,0 +1,63 @@
+// Per-molecule descriptors and solver state shared by the host driver and the
+// elementwise kernels of libhelios_ccsd.
+#pragma once
+#include "hcc_internal.h"
+
+namespace hcc {
+
+// Device pointers of one molecule's blocks in the per-buffer arenas.
+struct Mol {
+  int n, o, v, m;          // sites, occupied, virtual, DIIS depth
+  int nd;                  // packed DIIS vector length: ov + ov(ov+1)/2
+  int pad0;
+  double eref;
+  // inputs (row-major)
+  double *C, *G, *F, *Fao;
+  // dressing
+  double *Xf, *Yf, *Mf, *P, *Q, *Ft, *tmp;
+  // amplitudes and residuals
+  double *t1, *t2, *u, *w1, *w2, *x, *Zc, *WC, *WD;
+  // integrals and intermediates
+  double *G4, *Wkl, *Gklcd, *Lldkc, *Lt, *gvvov, *gooov, *Xvv, *Xoo;
+  // DIIS history: m slots of nd (amplitudes st, errors sd); B matrix m x m
+  double *st, *sd, *Bm;
+  // guess features / MP2 amplitudes (o^2 v^2)
+  double *t2mp2;
+};
+
+// Per-molecule iteration state (reference iterator semantics,
+// src/calculation/gIterator.f90:88-210, statuses mod_helios.f90:119-120).
+struct State {
+  int iter;          // iterations done
+  int status;        // 0 running, 1 converged, -1 max-iter, -2 diverged, -3 stagnated, -4 stopped
+  int active;        // 1 while iterating
+  int successful;    // improving iterations (starts at 1)
+  int diis_count;    // vectors in history
+  int diis_next;     // next ring slot
+  int staged;        // 1 when the current iteration staged a DIIS vector
+  int pad;
+  double acc[2];     // max|d1|, max|d2|
+  double cur, prev;  // current / previous max accuracy
+  double energy[5];
+  double coef[64];   // DIIS coefficients per slot
+};
+
+struct Params {
+  double eps;            // convergence threshold on max(acc)
+  double divergence;     // divergence threshold
+  double s1, s2;         // t1/t2 steps
+  int projtype;          // 1 or 2 (the reference's '2-1')
+  int diis;              // DIIS enabled
+  int maxiter;
+  int stagnation_after;  // iterations before the stagnation test
+  double stagnation_pct; // success percentage at or below which the solve stagnates
+  int check_divergence, check_stagnation;
+};
+
+// block -> molecule table for ragged elementwise launches
+struct Ragged {
+  int* prefix;   // device, nm+1 block prefix
+  int nblocks;
+};
+
+}  // namespace hcc

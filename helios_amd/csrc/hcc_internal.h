@@ -1,0 +1,48 @@
+// Internal declarations of libhelios_ccsd (not part of the C ABI).
+//
+// Layout conventions (all device arrays row-major, one molecule after another
+// in per-buffer arenas):
+//   C[mu][p]        MO coefficients, column p = MO p (occupied first)
+//   t1[i][a]        o x v
+//   t2[i][j][a][b]  o*o*v*v, pair-symmetric: t2[i][j][a][b] == t2[j][i][b][a]
+//
+// A GEMM operand is addressed through "composite" indices: a row (or column)
+// index r is split as r = hi * radix + lo and the element offset is
+// hi * s_hi + lo * s_lo.  With one composite index per dimension every
+// permuted tensor contraction of the T1-dressed CCSD residual becomes a plain
+// GEMM with no transposition pass.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hcc {
+
+struct Operand {
+  double* p;        // base of this molecule's block
+  int rr, rc;       // radices of the row and column composite indices (>= 1)
+  int srh, srl;     // row offset = (r / rr) * srh + (r % rr) * srl
+  int sch, scl;     // col offset = (c / rc) * sch + (c % rc) * scl
+};
+
+struct GemmProb {
+  Operand A, B, C;  // A: M x K, B: K x N, C: M x N
+  int M, N, K;
+  double alpha, beta;
+  int tiles_n;      // tiles along N for this problem
+  int tile_begin;   // first global tile of this problem
+  int mol;          // molecule index (for the active mask)
+  int a_kfast;      // 1: A's unit stride runs along K, 0: along M
+  int b_nfast;      // 1: B's unit stride runs along N, 0: along K
+  int pad_;
+};
+
+// Tile shape of the FP64 MFMA GEMM.
+constexpr int kBM = 64, kBN = 64, kBK = 16, kThreads = 256;
+
+// launches one grouped GEMM over all problems of a table (device memory)
+hipError_t launch_gemm_f64(const GemmProb* probs, int nprob, int total_tiles,
+                           const int* active, hipStream_t s);
+hipError_t launch_gemm_bf16(const GemmProb* probs, int nprob, int total_tiles,
+                            const int* active, hipStream_t s);
+
+}  // namespace hcc

@@ -1,0 +1,653 @@
+// Bandwidth-bound kernels of the r-ccsd solver: T1 dressing, orbital
+// products, residual assembly, the denominator update (fused with the DIIS
+// push and the max-|residual| reduction), DIIS dot products / solve /
+// extrapolation, energy, guesses and the per-molecule iterator control.
+//
+// Every kernel runs over a ragged batch: workgroups are assigned to molecules
+// through a block prefix table, each workgroup serving one molecule, so a
+// launch covers all molecules and no workgroup straddles two.  Reductions are
+// two-pass (per-workgroup partials, then one fixed-order pass per molecule):
+// results are bitwise reproducible run to run.
+#include <cmath>
+
+#include "hcc_batch.h"
+#include "hcc_ops.h"
+
+namespace hcc {
+
+__device__ __forceinline__ int block_mol(const int* __restrict__ prefix, int nm, int b) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define RAGGED_PROLOGUE(SIZE)                                         \
+  const int mol = block_mol(rg.prefix, nm, blockIdx.x);              \
+  const int lb = blockIdx.x - rg.prefix[mol];                        \
+  const Mol& M = mols[mol];                                          \
+  const long e = (long)lb * kEwThreads + threadIdx.x;                \
+  const long size_ = (SIZE);                                         \
+  (void)size_;
+
+// ---------------------------------------------------------------------------
+// T1 dressing
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_dress(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                     Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const int N = M.n, o = M.o, v = M.v;
+  const int mu = e / N, p = e % N;
+  const double* C = M.C + (long)mu * N;
+  double xf, yf;
+  if (p < o) {
+    double s = C[p];
+    for (int a = 0; a < v; ++a) s += C[o + a] * M.t1[p * v + a];
+    xf = C[p];
+    yf = s;
+  } else {
+    const int a = p - o;
+    double s = C[p];
+    for (int k = 0; k < o; ++k) s -= C[k] * M.t1[k * v + a];
+    xf = s;
+    yf = C[p];
+  }
+  M.Xf[e] = xf;
+  M.Yf[e] = yf;
+}
+
+// M = Fao - gamma o dD + 2 diag(gamma diag(dD)),  dD = (Yo - Co) Co^T
+__device__ __forceinline__ double dressed_density(const Mol& M, int mu, int nu) {
+  const double* Y = M.Yf + (long)mu * M.n;
+  const double* Cm = M.C + (long)mu * M.n;
+  const double* Cn = M.C + (long)nu * M.n;
+  double s = 0.0;
+  for (int k = 0; k < M.o; ++k) s += (Y[k] - Cm[k]) * Cn[k];
+  return s;
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_fock_m(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                      Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const int N = M.n;
+  const int mu = e / N, nu = e % N;
+  double val = M.Fao[e] - M.G[e] * dressed_density(M, mu, nu);
+  if (mu == nu) {
+    double s = 0.0;
+    for (int l = 0; l < N; ++l) s += M.G[(long)mu * N + l] * dressed_density(M, l, l);
+    val += 2.0 * s;
+  }
+  M.Mf[e] = val;
+}
+
+// P[mu][p*N+q] = Xf[mu][p] * Yf[mu][q]
+__global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                      Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const long N = M.n, N2 = N * N;
+  const long mu = e / N2, pq = e % N2;
+  const long p = pq / N, q = pq % N;
+  M.P[e] = M.Xf[mu * N + p] * M.Yf[mu * N + q];
+}
+
+// u = 2 t2 - t2(a<->b)
+__global__ __launch_bounds__(kEwThreads) void k_u(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                 Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
+  if (e >= size_ || !st[mol].active) return;
+  const int v = M.v;
+  const long ab = e % ((long)v * v);
+  const long base = e - ab;
+  const int a = ab / v, b = ab % v;
+  M.u[e] = 2.0 * M.t2[e] - M.t2[base + (long)b * v + a];
+}
+
+// Xvv = Ft_vv, Xoo = Ft_oo (the Fock parts of the E2 intermediates)
+__global__ __launch_bounds__(kEwThreads) void k_init_xx(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                       Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].o);
+  if (e >= size_ || !st[mol].active) return;
+  const int N = M.n, o = M.o, v = M.v;
+  if (e < (long)v * v) {
+    const int b = e / v, c = e % v;
+    M.Xvv[e] = M.Ft[(long)(o + b) * N + o + c];
+  } else {
+    const long f = e - (long)v * v;
+    const int k = f / o, j = f % o;
+    M.Xoo[f] = M.Ft[(long)k * N + j];
+  }
+}
+
+// w2 += x + x^T - 1/2 (Zc[ai,bj] + Zc[bj,ai]) - (Zc[aj,bi] + Zc[bi,aj])
+__global__ __launch_bounds__(kEwThreads) void k_assemble(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                        Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
+  if (e >= size_ || !st[mol].active) return;
+  const int o = M.o, v = M.v;
+  const long ov = (long)o * v;
+  const int b = e % v;
+  const int a = (e / v) % v;
+  const int j = (e / ((long)v * v)) % o;
+  const int i = e / ((long)o * v * v);
+  const long eT = (((long)j * o + i) * v + b) * v + a;
+  const double* Z = M.Zc;
+  const long ai = (long)a * o + i, bj = (long)b * o + j, aj = (long)a * o + j, bi = (long)b * o + i;
+  double val = M.x[e] + M.x[eT];
+  val -= 0.5 * (Z[ai * ov + bj] + Z[bj * ov + ai]);
+  val -= Z[aj * ov + bi] + Z[bi * ov + aj];
+  M.w2[e] += val;
+}
+
+// ---------------------------------------------------------------------------
+// update (stepCC) fused with the DIIS push and max|d| partials
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long packed_index(long mm, long nn, long ne) {
+  // upper triangle, row-major: pairs (mm, nn) with mm <= nn
+  return mm * ne - mm * (mm - 1) / 2 + (nn - mm);
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int k = 1; k < kEwThreads / 64; ++k) r = fmax(r, sh[k]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int k = 1; k < kEwThreads / 64; ++k) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+// Singles: d1 = w1 + Ft_ai; t1 += s1 d1 / (F_ii - F_aa).  size o*v
+__global__ __launch_bounds__(kEwThreads) void k_update1(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                       Ragged rg, int nm, Params prm, double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v);
+  if (!st[mol].active) return;
+  double amax = 0.0;
+  if (e < size_) {
+    const int N = M.n, o = M.o, v = M.v;
+    const int i = e / v, a = e % v;
+    const double d1 = M.w1[e] + M.Ft[(long)(o + a) * N + i];
+    M.w1[e] = d1;
+    const double D = M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a];
+    const double tn = M.t1[e] + prm.s1 * d1 / D;
+    M.t1[e] = tn;
+    amax = fabs(d1);
+    if (prm.diis) {
+      const State& S = st[mol];
+      M.sd[(long)S.diis_next * M.nd + e] = d1;
+      M.st[(long)S.diis_next * M.nd + e] = tn;
+    }
+  }
+  const double r = block_max(amax, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Doubles, from the (i,a)<=(j,b) representative (I,J,A,B) of each pair, so
+// that t2 stays exactly pair-symmetric:
+//   d2 = 2 Omega - Omega(a<->b),  t2 += s2 (2 d2 + d2(a<->b)) / D2 = 3 s2 Omega / D2
+__global__ __launch_bounds__(kEwThreads) void k_update2(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                       Ragged rg, int nm, Params prm, double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
+  if (!st[mol].active) return;
+  double amax = 0.0;
+  if (e < size_) {
+    const int N = M.n, o = M.o, v = M.v;
+    const long ne = (long)o * v;
+    const int b = e % v;
+    const int a = (e / v) % v;
+    const int j = (e / ((long)v * v)) % o;
+    const int i = e / ((long)o * v * v);
+    const long mm = (long)i * v + a, nn = (long)j * v + b;
+    int I = i, J = j, A = a, B = b;
+    if (mm > nn) { I = j; J = i; A = b; B = a; }
+    const long r = (((long)I * o + J) * v + A) * v + B;
+    // Omega(a<->b) at the representative of the pair (I,B),(J,A)
+    const long mb = (long)I * v + B, na = (long)J * v + A;
+    const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
+    const double om = M.w2[r];
+    const double d2 = 2.0 * om - M.w2[rx];
+    amax = fabs(d2);
+    const double D = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
+                     M.F[(long)(o + B) * N + o + B];
+    const double num = prm.projtype == 2 ? 3.0 * om : d2;
+    if (mm <= nn) {
+      // the representative updates both elements of its pair; no other thread touches them
+      const double tn = M.t2[e] + prm.s2 * num / D;
+      M.t2[e] = tn;
+      M.t2[(((long)j * o + i) * v + b) * v + a] = tn;
+      if (prm.diis) {
+        const State& S = st[mol];
+        const long p = ne + packed_index(mm, nn, ne);
+        M.sd[(long)S.diis_next * M.nd + p] = num;
+        M.st[(long)S.diis_next * M.nd + p] = tn;
+      }
+    }
+  }
+  const double rr = block_max(amax, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// Reference-form residuals for the parity interface: d1 (already final in w1
+// after k_update1 is NOT assumed; computed here from Omega) and
+// d2 = 2 Omega_sym - Omega_sym(a<->b), both in row-major [i][a] / [i][j][a][b].
+__global__ __launch_bounds__(kEwThreads) void k_contravariant(const Mol* __restrict__ mols, Ragged rg, int nm,
+                                                             double* __restrict__ d1, double* __restrict__ d2,
+                                                             long d1_off, long d2_off) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  if (e >= size_) return;
+  const int N = M.n, o = M.o, v = M.v;
+  const long n2 = (long)o * o * v * v;
+  if (e >= n2) {
+    const long f = e - n2;
+    const int i = f / v, a = f % v;
+    d1[d1_off + f] = M.w1[f] + M.Ft[(long)(o + a) * N + i];
+    return;
+  }
+  const int b = e % v;
+  const int a = (e / v) % v;
+  const int j = (e / ((long)v * v)) % o;
+  const int i = e / ((long)o * v * v);
+  const long mm = (long)i * v + a, nn = (long)j * v + b;
+  const long r = mm <= nn ? e : (((long)j * o + i) * v + b) * v + a;
+  const long mb = (long)i * v + b, na = (long)j * v + a;
+  const long rx = mb <= na ? (((long)i * o + j) * v + b) * v + a : (((long)j * o + i) * v + a) * v + b;
+  d2[d2_off + e] = 2.0 * M.w2[r] - M.w2[rx];
+}
+
+// per-molecule finalisation of the update: accuracies, convergence, DIIS commit
+__global__ void k_control1(const Mol* __restrict__ mols, State* __restrict__ st, int nm, Params prm,
+                           const int* __restrict__ pre1, const double* __restrict__ part1,
+                           const int* __restrict__ pre2, const double* __restrict__ part2) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  if (!S.active) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int b = pre1[mol]; b < pre1[mol + 1]; ++b) a1 = fmax(a1, part1[b]);
+  for (int b = pre2[mol]; b < pre2[mol + 1]; ++b) a2 = fmax(a2, part2[b]);
+  S.acc[0] = a1;
+  S.acc[1] = a2;
+  S.iter += 1;
+  S.prev = S.cur;
+  S.cur = fmax(a1, a2);
+  if (S.cur < S.prev) S.successful += 1;
+  S.staged = 0;
+  if (S.cur < prm.eps) {
+    S.status = 1;
+    return;
+  }
+  if (prm.diis) {
+    const int m = mols[mol].m;
+    S.staged = 1;
+    S.diis_count = S.diis_count < m ? S.diis_count + 1 : m;
+  }
+}
+
+// after DIIS: divergence, iteration cap, stagnation and stop flag (gIterator.f90:165-206)
+__global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st, int* __restrict__ act, int nm,
+                           Params prm, const int* __restrict__ stop) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  if (!S.active) return;
+  if (S.staged) S.diis_next = (S.diis_next + 1) % mols[mol].m;
+  int status = 0;
+  if (S.status == 1) status = 1;
+  else if (stop != nullptr && stop[mol]) status = -4;
+  else if (prm.check_divergence && S.cur > prm.divergence) status = -2;
+  else if (S.iter + 1 > prm.maxiter) status = -1;
+  else if (prm.check_stagnation && S.iter > prm.stagnation_after &&
+           100.0 * S.successful / S.iter <= prm.stagnation_pct) status = -3;
+  if (status != 0) {
+    S.status = status;
+    S.active = 0;
+    act[mol] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DIIS
+// ---------------------------------------------------------------------------
+// partial dots <sd_k, sd_new> over this workgroup's chunk, for every valid slot k
+__global__ __launch_bounds__(kEwThreads) void k_diis_dot(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                        Ragged rg, int nm, double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  const int mol = block_mol(rg.prefix, nm, blockIdx.x);
+  const int lb = blockIdx.x - rg.prefix[mol];
+  const Mol& M = mols[mol];
+  const State& S = st[mol];
+  const int m = M.m;
+  if (!S.active || !S.staged) {
+    if (threadIdx.x < m) part[(long)blockIdx.x * m + threadIdx.x] = 0.0;
+    return;
+  }
+  const long base = (long)lb * kEwThreads * kDotItems;
+  const double* snew = M.sd + (long)S.diis_next * M.nd;
+  for (int k = 0; k < S.diis_count; ++k) {
+    const double* sk = M.sd + (long)k * M.nd;
+    double acc = 0.0;
+    for (int it = 0; it < kDotItems; ++it) {
+      const long e = base + (long)it * kEwThreads + threadIdx.x;
+      if (e < M.nd) acc += sk[e] * snew[e];
+    }
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * m + k] = r;
+  }
+}
+
+// finalise dots into B, build the bordered system, solve it (partial pivoting),
+// scaled by the largest diagonal of B so that it stays accurate near convergence
+__global__ void k_diis_solve(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
+                             const int* __restrict__ prefix, const double* __restrict__ part) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  const Mol& M = mols[mol];
+  State& S = st[mol];
+  if (!S.active || !S.staged) return;
+  const int m = M.m, cnt = S.diis_count, snew = S.diis_next;
+  for (int k = 0; k < cnt; ++k) {
+    double s = 0.0;
+    for (int b = prefix[mol]; b < prefix[mol + 1]; ++b) s += part[(long)b * m + k];
+    M.Bm[k * m + snew] = s;
+    M.Bm[snew * m + k] = s;
+  }
+  for (int k = 0; k < 64; ++k) S.coef[k] = 0.0;
+  if (cnt < 2) { S.coef[snew] = 1.0; return; }
+  double A[(kMaxDiis + 1) * (kMaxDiis + 2)];
+  const int n = cnt + 1;
+  double scale = 0.0;
+  for (int k = 0; k < cnt; ++k) scale = fmax(scale, M.Bm[k * m + k]);
+  if (!(scale > 0.0)) scale = 1.0;
+  for (int r = 0; r < cnt; ++r) {
+    for (int c = 0; c < cnt; ++c) A[r * (n + 1) + c] = M.Bm[r * m + c] / scale;
+    A[r * (n + 1) + cnt] = 1.0;
+    A[r * (n + 1) + n] = 0.0;
+  }
+  for (int c = 0; c < cnt; ++c) A[cnt * (n + 1) + c] = 1.0;
+  A[cnt * (n + 1) + cnt] = 0.0;
+  A[cnt * (n + 1) + n] = 1.0;
+  bool ok = true;
+  for (int c = 0; c < n && ok; ++c) {
+    int piv = c;
+    double best = fabs(A[c * (n + 1) + c]);
+    for (int r = c + 1; r < n; ++r) {
+      const double x = fabs(A[r * (n + 1) + c]);
+      if (x > best) { best = x; piv = r; }
+    }
+    if (!(best > 1e-300)) { ok = false; break; }
+    if (piv != c)
+      for (int q = 0; q <= n; ++q) {
+        const double tmp = A[c * (n + 1) + q];
+        A[c * (n + 1) + q] = A[piv * (n + 1) + q];
+        A[piv * (n + 1) + q] = tmp;
+      }
+    const double inv = 1.0 / A[c * (n + 1) + c];
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * (n + 1) + c] * inv;
+      if (f == 0.0) continue;
+      for (int q = c; q <= n; ++q) A[r * (n + 1) + q] -= f * A[c * (n + 1) + q];
+    }
+  }
+  if (!ok) { S.coef[snew] = 1.0; return; }
+  double x[kMaxDiis + 1];
+  for (int r = n - 1; r >= 0; --r) {
+    double s = A[r * (n + 1) + n];
+    for (int q = r + 1; q < n; ++q) s -= A[r * (n + 1) + q] * x[q];
+    x[r] = s / A[r * (n + 1) + r];
+  }
+  for (int k = 0; k < cnt; ++k) S.coef[k] = x[k];
+}
+
+// t = sum_k c_k st_k, unpacked with the pair mirror
+__global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                           Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  const State& S = st[mol];
+  if (e >= size_ || !S.active || !S.staged || S.diis_count < 2) return;
+  const int o = M.o, v = M.v;
+  const long ne = (long)o * v;
+  const long n2 = (long)o * o * v * v;
+  long p;
+  double* dst;
+  if (e >= n2) {
+    p = e - n2;
+    dst = M.t1 + p;
+  } else {
+    const int b = e % v;
+    const int a = (e / v) % v;
+    const int j = (e / ((long)v * v)) % o;
+    const int i = e / ((long)o * v * v);
+    long mm = (long)i * v + a, nn = (long)j * v + b;
+    if (mm > nn) { const long t = mm; mm = nn; nn = t; }
+    p = ne + packed_index(mm, nn, ne);
+    dst = M.t2 + e;
+  }
+  double s = 0.0;
+  for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + p];
+  *dst = s;
+}
+
+// ---------------------------------------------------------------------------
+// energy: E = eref + 2 sum F_ia t_ia + sum L_iajb (t2 + t1 t1)  (energyCC)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_energy_part(const Mol* __restrict__ mols, Ragged rg, int nm,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  const int N = M.n, o = M.o, v = M.v;
+  const long n2 = (long)o * o * v * v;
+  double s1 = 0.0, s2 = 0.0;
+  if (e < size_) {
+    if (e >= n2) {
+      const long f = e - n2;
+      const int i = f / v, a = f % v;
+      s1 = M.F[(long)i * N + o + a] * M.t1[f];
+    } else {
+      const int b = e % v;
+      const int a = (e / v) % v;
+      const int j = (e / ((long)v * v)) % o;
+      const int i = e / ((long)o * v * v);
+      s2 = M.Lt[e] * (M.t2[e] + M.t1[i * v + a] * M.t1[j * v + b]);
+    }
+  }
+  const double r1 = block_sum(s1, sh);
+  const double r2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[2 * (long)blockIdx.x] = r1;
+    part[2 * (long)blockIdx.x + 1] = r2;
+  }
+}
+
+__global__ void k_energy_final(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
+                               const int* __restrict__ prefix, const double* __restrict__ part) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = prefix[mol]; b < prefix[mol + 1]; ++b) {
+    s1 += part[2 * (long)b];
+    s2 += part[2 * (long)b + 1];
+  }
+  State& S = st[mol];
+  const double eref = mols[mol].eref;
+  S.energy[0] = eref + 2.0 * s1 + s2;
+  S.energy[1] = eref;
+  S.energy[2] = 2.0 * s1;
+  S.energy[3] = s2;
+  S.energy[4] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// precomputation and guesses
+// ---------------------------------------------------------------------------
+// Lt[i][j][a][b] = 2(ia|jb) - (ib|ja);  Lldkc[(l,d),(k,c)] = 2(ld|kc) - (lc|kd)
+// from Gklcd[k][l][c][d] = (kc|ld)
+__global__ __launch_bounds__(kEwThreads) void k_make_L(const Mol* __restrict__ mols, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
+  if (e >= size_) return;
+  const int o = M.o, v = M.v;
+  const long ov = (long)o * v;
+  const double* G = M.Gklcd;
+  {
+    const int b = e % v;
+    const int a = (e / v) % v;
+    const int j = (e / ((long)v * v)) % o;
+    const int i = e / ((long)o * v * v);
+    const long base = ((long)i * o + j) * v * v;
+    M.Lt[e] = 2.0 * G[base + (long)a * v + b] - G[base + (long)b * v + a];
+  }
+  {
+    const int c = e % v;
+    const int k = (e / v) % o;
+    const int d = (e / ov) % v;
+    const int l = e / (ov * v);
+    const long base = ((long)l * o + k) * v * v;
+    M.Lldkc[e] = 2.0 * G[base + (long)d * v + c] - G[base + (long)c * v + d];
+  }
+}
+
+// kind 0: the reference guess t2 = (2(ia|jb)-(ib|ja))/D (mod_cc.f90:634-650);
+// kind 1: MP2 amplitudes (ia|jb)/D.  t1 = F_ia/(F_ii - F_aa) for both.
+__global__ __launch_bounds__(kEwThreads) void k_guess(const Mol* __restrict__ mols, Ragged rg, int nm, int kind) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  if (e >= size_) return;
+  const int N = M.n, o = M.o, v = M.v;
+  const long n2 = (long)o * o * v * v;
+  if (e >= n2) {
+    const long f = e - n2;
+    const int i = f / v, a = f % v;
+    M.t1[f] = M.F[(long)i * N + o + a] / (M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a]);
+    return;
+  }
+  const int b = e % v;
+  const int a = (e / v) % v;
+  const int j = (e / ((long)v * v)) % o;
+  const int i = e / ((long)o * v * v);
+  const long mm = (long)i * v + a, nn = (long)j * v + b;
+  if (mm > nn) return;   // the representative of the pair writes both elements
+  const double D = M.F[(long)i * N + i] + M.F[(long)j * N + j] - M.F[(long)(o + a) * N + o + a] -
+                   M.F[(long)(o + b) * N + o + b];
+  const double num = kind == 0 ? M.Lt[e] : M.Gklcd[e];
+  const long eT = (((long)j * o + i) * v + b) * v + a;
+  M.t2[e] = num / D;
+  M.t2[eT] = num / D;
+  if (M.t2mp2 != nullptr) {
+    M.t2mp2[e] = M.Gklcd[e] / D;
+    M.t2mp2[eT] = M.Gklcd[e] / D;
+  }
+}
+
+__global__ void k_reset_state(State* __restrict__ st, int* __restrict__ act, int nm) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  S.iter = 0;
+  S.status = 0;
+  S.active = 1;
+  S.successful = 1;
+  S.diis_count = 0;
+  S.diis_next = 0;
+  S.staged = 0;
+  S.acc[0] = S.acc[1] = -1.0;
+  S.cur = 0.0;
+  S.prev = 10.0;
+  for (int k = 0; k < 5; ++k) S.energy[k] = 0.0;
+  act[mol] = 1;
+}
+
+// re-arm molecules for the single-step interface (the caller runs the iterator)
+__global__ void k_activate(State* __restrict__ st, int* __restrict__ act, int nm) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  st[mol].active = 1;
+  st[mol].status = 0;
+  act[mol] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+#define RAGGED_LAUNCH(K, rg, ...)                                                     \
+  do {                                                                              \
+    if ((rg).nblocks > 0) hipLaunchKernelGGL(K, dim3((rg).nblocks), dim3(kEwThreads), 0, s, __VA_ARGS__); \
+  } while (0)
+
+void ew_dress(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_dress, rgNN, mols, st, rgNN, nm);
+}
+void ew_fock_m(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_fock_m, rgNN, mols, st, rgNN, nm);
+}
+void ew_pouter(const Mol* mols, const State* st, Ragged rgN3, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_pouter, rgN3, mols, st, rgN3, nm);
+}
+void ew_u(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_u, rgT2, mols, st, rgT2, nm);
+}
+void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_init_xx, rgXX, mols, st, rgXX, nm);
+}
+void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_assemble, rgT2, mols, st, rgT2, nm);
+}
+void ew_update(const Mol* mols, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
+               double* part1, double* part2, hipStream_t s) {
+  RAGGED_LAUNCH(k_update1, rgOV, mols, st, rgOV, nm, prm, part1);
+  RAGGED_LAUNCH(k_update2, rgT2, mols, st, rgT2, nm, prm, part2);
+}
+void ew_contravariant(const Mol* mols, Ragged rgT2S, int nm, double* d1, double* d2, hipStream_t s) {
+  RAGGED_LAUNCH(k_contravariant, rgT2S, mols, rgT2S, nm, d1, d2, 0L, 0L);
+}
+void ctl_after_update(const Mol* mols, State* st, int nm, const Params& prm, Ragged rgOV,
+                      const double* part1, Ragged rgT2, const double* part2, hipStream_t s) {
+  hipLaunchKernelGGL(k_control1, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm, prm, rgOV.prefix,
+                     part1, rgT2.prefix, part2);
+}
+void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgT2S, double* part,
+               hipStream_t s) {
+  RAGGED_LAUNCH(k_diis_dot, rgND, mols, st, rgND, nm, part);
+  hipLaunchKernelGGL(k_diis_solve, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm, rgND.prefix,
+                     part);
+  RAGGED_LAUNCH(k_diis_extrap, rgT2S, mols, st, rgT2S, nm);
+}
+void ctl_after_diis(const Mol* mols, State* st, int* act, int nm, const Params& prm, const int* stop,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_control2, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, act, nm, prm, stop);
+}
+void energy(const Mol* mols, State* st, int nm, Ragged rgT2S, double* part, hipStream_t s) {
+  RAGGED_LAUNCH(k_energy_part, rgT2S, mols, rgT2S, nm, part);
+  hipLaunchKernelGGL(k_energy_final, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm, rgT2S.prefix,
+                     part);
+}
+void make_L(const Mol* mols, Ragged rgT2, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_make_L, rgT2, mols, rgT2, nm);
+}
+void guess(const Mol* mols, State* st, int* act, Ragged rgT2S, int nm, int kind, hipStream_t s) {
+  RAGGED_LAUNCH(k_guess, rgT2S, mols, rgT2S, nm, kind);
+  hipLaunchKernelGGL(k_reset_state, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
+}
+void reset_state(State* st, int* act, int nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_state, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
+}
+void activate(State* st, int* act, int nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_activate, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
+}
+
+}  // namespace hcc

@@ -1,0 +1,895 @@
+// Host driver of libhelios_ccsd: device arenas for a (ragged) batch of
+// molecules, the table of grouped GEMMs that make up one CCSD iteration, the
+// iteration sequence, and the C ABI of include/helios_ccsd.h.
+//
+// The residual is the biorthogonal CCSD residual Omega of the T1-transformed
+// PPP Hamiltonian.  With ZDO integrals every dressed integral block is one
+// GEMM over sites,
+//   g~(pq|rs) = sum_mu P[mu][pq] Q[mu][rs],   P[mu][p*N+q] = X[mu][p] Y[mu][q],
+//   Q = gamma P,   X = C(1 - t1^T) (bra), Y = C(1 + t1) (ket),
+// and the CCSD contractions (particle-particle ladder, hole-hole ladder, the
+// two ring intermediates and their contractions, the Fock-like intermediates)
+// are GEMMs over composite indices.  The reference's contravariant residuals
+// (proj-ccsd-spat-hf-1/2.f90) are d1 = Omega1 and d2 = 2 Omega2 - Omega2(a<->b);
+// the update follows stepCC (mod_cc.f90:987-1017) and the DIIS follows
+// pushCCVectors/newCCVectors (mod_cc.f90:1390-1423, 1514-1555) with a ring
+// buffer in place of the history shift.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/helios_ccsd.h"
+#include "hcc_batch.h"
+#include "hcc_ops.h"
+
+namespace hcc {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HCHECK(x)                                        \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) throw HipError{e_, #x};        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// operand helpers
+// ---------------------------------------------------------------------------
+static Operand rm(double* p, int ld) { return Operand{p, 1, 1, ld, 0, 1, 0}; }
+static Operand trn(double* p, int ld) { return Operand{p, 1, 1, 1, 0, ld, 0}; }
+// composite: row = (r / rr, r % rr) with strides (srh, srl), col likewise
+static Operand cmp(double* p, int rr, int srh, int srl, int rc, int sch, int scl) {
+  return Operand{p, rr, rc, srh, srl, sch, scl};
+}
+
+static bool unit_cols(const Operand& o) { return (o.rc == 1 ? o.sch : o.scl) == 1; }
+static bool unit_rows(const Operand& o) { return (o.rr == 1 ? o.srh : o.srl) == 1; }
+
+struct GemmTable {
+  GemmProb* d = nullptr;
+  int nprob = 0;
+  int tiles = 0;
+};
+
+struct DevBuf {
+  double* base = nullptr;
+  std::vector<long> off;   // per-molecule offsets
+  long total = 0;
+};
+
+// ---------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------
+struct Batch {
+  int nm = 0;
+  int m = 20;
+  std::vector<int> n, o, v;
+  std::vector<double> eref;
+  std::vector<Mol> hm;
+  Mol* dm = nullptr;
+  State* dst = nullptr;
+  int* dact = nullptr;
+  std::vector<void*> allocs;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND;
+  double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
+  std::vector<GemmTable> pre, it;
+  double *d1out = nullptr, *d2out = nullptr;
+
+  ~Batch() { release(); }
+  void release() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    dm = nullptr;
+    dst = nullptr;
+    dact = nullptr;
+  }
+  template <class T>
+  T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HCHECK(hipMalloc(&p, count * sizeof(T)));
+    allocs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+
+static long sz_nn(int n, int, int) { return (long)n * n; }
+static long sz_n3(int n, int, int) { return (long)n * n * n; }
+static long sz_t2(int, int o, int v) { return (long)o * o * v * v; }
+static long sz_ov(int, int o, int v) { return (long)o * v; }
+
+static Ragged make_ragged(Batch& b, const std::function<long(int, int, int)>& size, int items = 1) {
+  std::vector<int> pre(b.nm + 1, 0);
+  for (int k = 0; k < b.nm; ++k) {
+    long s = size(b.n[k], b.o[k], b.v[k]);
+    long per = (long)kEwThreads * items;
+    long blocks = std::max<long>(1, (s + per - 1) / per);
+    pre[k + 1] = pre[k] + (int)blocks;
+  }
+  Ragged r;
+  r.prefix = b.dalloc<int>(pre.size());
+  HCHECK(hipMemcpy(r.prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+  r.nblocks = pre.back();
+  return r;
+}
+
+static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build) {
+  std::vector<GemmProb> probs;
+  int tiles = 0;
+  for (int k = 0; k < b.nm; ++k) {
+    GemmProb p = build(k);
+    if (p.M <= 0 || p.N <= 0) continue;
+    p.mol = k;
+    p.tiles_n = (p.N + kBN - 1) / kBN;
+    const int tm = (p.M + kBM - 1) / kBM;
+    p.tile_begin = tiles;
+    tiles += tm * p.tiles_n;
+    p.a_kfast = unit_cols(p.A) ? 1 : (unit_rows(p.A) ? 0 : 1);
+    p.b_nfast = unit_cols(p.B) ? 1 : (unit_rows(p.B) ? 0 : 1);
+    probs.push_back(p);
+  }
+  GemmTable t;
+  t.nprob = (int)probs.size();
+  t.tiles = tiles;
+  if (t.nprob) {
+    t.d = b.dalloc<GemmProb>(probs.size());
+    HCHECK(hipMemcpy(t.d, probs.data(), probs.size() * sizeof(GemmProb), hipMemcpyHostToDevice));
+  }
+  return t;
+}
+
+static GemmProb prob(Operand A, Operand B, Operand C, int M, int N, int K, double alpha, double beta) {
+  GemmProb p{};
+  p.A = A;
+  p.B = B;
+  p.C = C;
+  p.M = M;
+  p.N = N;
+  p.K = K;
+  p.alpha = alpha;
+  p.beta = beta;
+  return p;
+}
+
+// Allocates all arenas and fills the per-molecule descriptors.
+static void allocate(Batch& b, bool keep_mp2) {
+  auto arena = [&](const std::function<long(int, int, int)>& size) {
+    long total = 0;
+    std::vector<long> off(b.nm);
+    for (int k = 0; k < b.nm; ++k) {
+      off[k] = total;
+      total += (size(b.n[k], b.o[k], b.v[k]) + 31) / 32 * 32;   // 256-byte aligned blocks
+    }
+    double* base = b.dalloc<double>(total);
+    std::vector<double*> ptr(b.nm);
+    for (int k = 0; k < b.nm; ++k) ptr[k] = base + off[k];
+    return ptr;
+  };
+  const int m = b.m;
+  auto C = arena(sz_nn), G = arena(sz_nn), F = arena(sz_nn), Fao = arena(sz_nn);
+  auto Xf = arena(sz_nn), Yf = arena(sz_nn), Mf = arena(sz_nn), Ft = arena(sz_nn), tmp = arena(sz_nn);
+  auto P = arena(sz_n3), Q = arena(sz_n3);
+  auto t1 = arena(sz_ov), w1 = arena(sz_ov);
+  auto t2 = arena(sz_t2), u = arena(sz_t2), w2 = arena(sz_t2), x = arena(sz_t2), Zc = arena(sz_t2);
+  auto WC = arena(sz_t2), WD = arena(sz_t2), Gklcd = arena(sz_t2), Lldkc = arena(sz_t2), Lt = arena(sz_t2);
+  auto G4 = arena([](int, int, int v) { return (long)v * v * v * v; });
+  auto Wkl = arena([](int, int o, int) { return (long)o * o * o * o; });
+  auto gvvov = arena([](int, int o, int v) { return (long)o * v * v * v; });
+  auto gooov = arena([](int, int o, int v) { return (long)o * o * o * v; });
+  auto Xvv = arena([](int, int, int v) { return (long)v * v; });
+  auto Xoo = arena([](int, int o, int) { return (long)o * o; });
+  auto ndsize = [m](int, int o, int v) {
+    long ov = (long)o * v;
+    return (long)m * (ov + ov * (ov + 1) / 2);
+  };
+  auto st = arena(ndsize), sd = arena(ndsize);
+  auto Bm = arena([m](int, int, int) { return (long)m * m; });
+  std::vector<double*> mp2(b.nm, nullptr);
+  if (keep_mp2) mp2 = arena(sz_t2);
+  b.hm.resize(b.nm);
+  for (int k = 0; k < b.nm; ++k) {
+    Mol& M = b.hm[k];
+    memset(&M, 0, sizeof(M));
+    M.n = b.n[k];
+    M.o = b.o[k];
+    M.v = b.v[k];
+    M.m = m;
+    const long ov = (long)M.o * M.v;
+    M.nd = (int)(ov + ov * (ov + 1) / 2);
+    M.eref = b.eref[k];
+    M.C = C[k]; M.G = G[k]; M.F = F[k]; M.Fao = Fao[k];
+    M.Xf = Xf[k]; M.Yf = Yf[k]; M.Mf = Mf[k]; M.P = P[k]; M.Q = Q[k]; M.Ft = Ft[k]; M.tmp = tmp[k];
+    M.t1 = t1[k]; M.t2 = t2[k]; M.u = u[k]; M.w1 = w1[k]; M.w2 = w2[k]; M.x = x[k]; M.Zc = Zc[k];
+    M.WC = WC[k]; M.WD = WD[k];
+    M.G4 = G4[k]; M.Wkl = Wkl[k]; M.Gklcd = Gklcd[k]; M.Lldkc = Lldkc[k]; M.Lt = Lt[k];
+    M.gvvov = gvvov[k]; M.gooov = gooov[k]; M.Xvv = Xvv[k]; M.Xoo = Xoo[k];
+    M.st = st[k]; M.sd = sd[k]; M.Bm = Bm[k];
+    M.t2mp2 = mp2[k];
+  }
+  b.dm = b.dalloc<Mol>(b.nm);
+  HCHECK(hipMemcpy(b.dm, b.hm.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
+  b.dst = b.dalloc<State>(b.nm);
+  HCHECK(hipMemset(b.dst, 0, b.nm * sizeof(State)));
+  b.dact = b.dalloc<int>(b.nm);
+  HCHECK(hipMemset(b.dact, 0, b.nm * sizeof(int)));
+
+  b.rgNN = make_ragged(b, sz_nn);
+  b.rgN3 = make_ragged(b, sz_n3);
+  b.rgT2 = make_ragged(b, sz_t2);
+  b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
+  b.rgOV = make_ragged(b, sz_ov);
+  b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
+  b.rgND = make_ragged(b, [](int, int o, int v) {
+    long ov = (long)o * v;
+    return ov + ov * (ov + 1) / 2;
+  }, kDotItems);
+  b.part1 = b.dalloc<double>(b.rgOV.nblocks);
+  b.part2 = b.dalloc<double>(b.rgT2.nblocks);
+  b.partE = b.dalloc<double>(2 * (long)b.rgT2S.nblocks);
+  b.partD = b.dalloc<double>((long)b.rgND.nblocks * m);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tables (see the derivation in the file header; offsets per molecule)
+// ---------------------------------------------------------------------------
+static void build_tables(Batch& b) {
+  std::vector<Mol>& H = b.hm;
+  // --- precompute: Fao = C F C^T, Gklcd = (kc|ld) from undressed P, Q
+  b.pre.clear();
+  b.pre.push_back(make_table(b, [&](int k) {   // tmp = C F
+    const Mol& M = H[k];
+    const int N = M.n;
+    return prob(rm(M.C, N), rm(M.F, N), rm(M.tmp, N), N, N, N, 1.0, 0.0);
+  }));
+  b.pre.push_back(make_table(b, [&](int k) {   // Fao = tmp C^T
+    const Mol& M = H[k];
+    const int N = M.n;
+    return prob(rm(M.tmp, N), trn(M.C, N), rm(M.Fao, N), N, N, N, 1.0, 0.0);
+  }));
+  b.pre.push_back(make_table(b, [&](int k) {   // Gklcd[k][l][c][d] = (kc|ld)
+    const Mol& M = H[k];
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P + o, v, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1),
+                cmp(M.Gklcd, v, ov * v, v, v, v * v, 1), ov, ov, N, 1.0, 0.0);
+  }));
+
+  // --- one iteration
+  b.it.clear();
+  auto add = [&](const std::function<GemmProb(const Mol&)>& f) {
+    b.it.push_back(make_table(b, [&](int k) { return f(H[k]); }));
+  };
+  // g1: Q = gamma P
+  add([](const Mol& M) {
+    const int N = M.n, N2 = N * N;
+    return prob(rm(M.G, N), rm(M.P, N2), rm(M.Q, N2), N, N2, N, 1.0, 0.0);
+  });
+  // g2: Ft = Xf^T (Mf Yf)
+  add([](const Mol& M) {
+    const int N = M.n;
+    return prob(rm(M.Mf, N), rm(M.Yf, N), rm(M.tmp, N), N, N, N, 1.0, 0.0);
+  });
+  add([](const Mol& M) {
+    const int N = M.n;
+    return prob(trn(M.Xf, N), rm(M.tmp, N), rm(M.Ft, N), N, N, N, 1.0, 0.0);
+  });
+  // g3: gvvov[k][c][d][a] = g~(ad|kc)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1),
+                cmp(M.gvvov, v, 1, v, v, v * v * v, v * v), v * v, ov, N, 1.0, 0.0);
+  });
+  // g4: w1[i][a] = sum_{k,cd} u[k][i][cd] gvvov[(k,c,d)][a]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.u, 1, v * v, 0, v * v, ov * v, 1), rm(M.gvvov, v), rm(M.w1, v), o, v, ov * v, 1.0,
+                0.0);
+  });
+  // g5: gooov[i][k][l][c] = g~(ki|lc)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P, o, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1),
+                cmp(M.gooov, o, ov, o * ov, v, v, 1), o * o, ov, N, 1.0, 0.0);
+  });
+  // g6: w1 -= gooov[i][(kl,c)] u[(kl),a,c]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(rm(M.gooov, o * o * v), cmp(M.u, v, v * v, 1, 1, v, 0), rm(M.w1, v), o, v, o * o * v,
+                -1.0, 1.0);
+  });
+  // g7: w1[(i,a)] += sum_{k,c} u[i][k][a][c] Ft[k][o+c]
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.u, v, ov * v, v, v, v * v, 1), cmp(M.Ft + o, v, N, 1, 1, 0, 0),
+                cmp(M.w1, 1, 1, 0, 1, 0, 0), ov, 1, ov, 1.0, 1.0);
+  });
+  // g8: w2[i][j][a][b] = g~(ai|bj)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P + o * N, o, N, 1, 1, N2, 0), cmp(M.Q + o * N, 1, N2, 0, o, N, 1),
+                cmp(M.w2, o, v, ov * v, o, 1, v * v), ov, ov, N, 1.0, 0.0);
+  });
+  // g9: G4[c][d][a][b] = g~(ac|bd)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N;
+    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q + o * N + o, 1, N2, 0, v, N, 1),
+                cmp(M.G4, v, v, v * v * v, v, 1, v * v), v * v, v * v, N, 1.0, 0.0);
+  });
+  // g10: particle-particle ladder w2 += t2 G4
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(rm(M.t2, v * v), rm(M.G4, v * v), rm(M.w2, v * v), o * o, v * v, v * v, 1.0, 1.0);
+  });
+  // g11: Wkl[(k,l),(i,j)] = g~(ki|lj)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, N2 = N * N;
+    return prob(cmp(M.P, o, N, 1, 1, N2, 0), cmp(M.Q, 1, N2, 0, o, N, 1),
+                cmp(M.Wkl, o, o * o * o, o, o, o * o, 1), o * o, o * o, N, 1.0, 0.0);
+  });
+  // g12: Wkl += Gklcd t2^T
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(rm(M.Gklcd, v * v), trn(M.t2, v * v), rm(M.Wkl, o * o), o * o, o * o, v * v, 1.0, 1.0);
+  });
+  // g13: hole-hole ladder w2 += Wkl^T t2
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(trn(M.Wkl, o * o), rm(M.t2, v * v), rm(M.w2, v * v), o * o, v * v, o * o, 1.0, 1.0);
+  });
+  // g14: WC[(a,i),(k,c)] = g~(ki|ac)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P, o, N, 1, 1, N2, 0), cmp(M.Q + o * N + o, 1, N2, 0, v, N, 1),
+                cmp(M.WC, o, v, ov, v, o * ov, 1), o * o, v * v, N, 1.0, 0.0);
+  });
+  // g15: WC -= 1/2 t2[l][i][a][d] (lc|kd)
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.t2, o, v, v * v, v, ov * v, 1), cmp(M.Gklcd, v, ov * v, 1, v, v * v, v), rm(M.WC, ov),
+                ov, ov, ov, -0.5, 1.0);
+  });
+  // g16: Zc = WC t2[k][j][b][c]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(rm(M.WC, ov), cmp(M.t2, v, ov * v, 1, o, v, v * v), rm(M.Zc, ov), ov, ov, ov, 1.0, 0.0);
+  });
+  // g17: WD = 2 g~(ai|kc)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P + o * N, o, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1), rm(M.WD, ov), ov, ov,
+                N, 2.0, 0.0);
+  });
+  // g18: WD -= g~(ac|ki)
+  add([](const Mol& M) {
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
+    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q, 1, N2, 0, o, N, 1),
+                cmp(M.WD, v, o * ov, 1, o, v, ov), v * v, o * o, N, -1.0, 1.0);
+  });
+  // g19: WD += 1/2 u[i][l][a][d] L[(l,d),(k,c)]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.u, o, v, ov * v, v, v * v, 1), rm(M.Lldkc, ov), rm(M.WD, ov), ov, ov, ov, 0.5, 1.0);
+  });
+  // g20: x[i][j][a][b] = 1/2 WD u[j][k][b][c]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(rm(M.WD, ov), cmp(M.u, v, v * v, 1, o, v, ov * v), cmp(M.x, o, v, ov * v, o, 1, v * v), ov,
+                ov, ov, 0.5, 0.0);
+  });
+  // g21: Xvv -= u[(kl)][b][d] Gklcd[(kl)][c][d]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(cmp(M.u, 1, v, 0, v, v * v, 1), cmp(M.Gklcd, v, v * v, 1, 1, v, 0), rm(M.Xvv, v), v, v,
+                o * o * v, -1.0, 1.0);
+  });
+  // g22: Xoo += Gklcd[l][k][cd] u[l][j][cd]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.Gklcd, 1, v * v, 0, v * v, ov * v, 1), cmp(M.u, v * v, ov * v, 1, 1, v * v, 0),
+                rm(M.Xoo, o), o, o, ov * v, 1.0, 1.0);
+  });
+  // g23: x += t2 Xvv^T
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v;
+    return prob(rm(M.t2, v), trn(M.Xvv, v), rm(M.x, v), o * o * v, v, v, 1.0, 1.0);
+  });
+  // g24: x -= t2[i][k][ab] Xoo[k][j]
+  add([](const Mol& M) {
+    const int o = M.o, v = M.v, ov = o * v;
+    return prob(cmp(M.t2, v * v, ov * v, 1, 1, v * v, 0), rm(M.Xoo, o), cmp(M.x, v * v, ov * v, 1, 1, v * v, 0),
+                ov * v, o, o, -1.0, 1.0);
+  });
+}
+
+enum ItOp {
+  G1_Q = 0, G2A, G2B, G3, G4_, G5, G6, G7, G8, G9, G10, G11, G12, G13, G14, G15, G16, G17, G18, G19, G20,
+  G21, G22, G23, G24, N_IT_OPS
+};
+
+}  // namespace hcc
+
+using namespace hcc;
+
+struct hcc_ctx {
+  int device = 0;
+  int precision = HCC_FP64;
+  int diis_steps = 20;
+  double s1 = 1.0, s2 = 1.0;
+  int projtype = 2;
+  int diis = 1;
+  hipStream_t stream = nullptr;
+  Batch* batch = nullptr;
+  std::string err;
+  bool have_guess = false;
+  double last_ms = 0.0;
+  double last_iters = 0.0;
+};
+
+namespace hcc {
+
+static void gemm(hcc_ctx* c, const GemmTable& t) {
+  if (t.nprob == 0) return;
+  HCHECK(launch_gemm_f64(t.d, t.nprob, t.tiles, c->batch->dact, c->stream));
+}
+
+static Params params(const hcc_ctx* c, double eps, int maxiter, bool iterator_rules) {
+  Params p{};
+  p.eps = eps;
+  p.divergence = 1e3;
+  p.s1 = c->s1;
+  p.s2 = c->s2;
+  p.projtype = c->projtype;
+  p.diis = c->diis;
+  p.maxiter = iterator_rules ? maxiter : 0x7fffffff;
+  p.stagnation_after = 200;
+  p.stagnation_pct = 60.0;
+  p.check_divergence = iterator_rules ? 1 : 0;
+  p.check_stagnation = iterator_rules ? 1 : 0;
+  return p;
+}
+
+// Omega1 -> w1 (without the F~_ai term), Omega2 -> w2
+static void residual(hcc_ctx* c) {
+  Batch& b = *c->batch;
+  hipStream_t s = c->stream;
+  ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
+  ew_fock_m(b.dm, b.dst, b.rgNN, b.nm, s);
+  ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  gemm(c, b.it[G1_Q]);
+  gemm(c, b.it[G2A]);
+  gemm(c, b.it[G2B]);
+  ew_u(b.dm, b.dst, b.rgT2, b.nm, s);
+  for (int g = G3; g <= G20; ++g) gemm(c, b.it[g]);
+  ew_init_xx(b.dm, b.dst, b.rgXX, b.nm, s);
+  for (int g = G21; g <= G24; ++g) gemm(c, b.it[g]);
+  ew_assemble(b.dm, b.dst, b.rgT2, b.nm, s);
+  HCHECK(hipGetLastError());
+}
+
+static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
+  Batch& b = *c->batch;
+  hipStream_t s = c->stream;
+  residual(c);
+  ew_update(b.dm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
+  ctl_after_update(b.dm, b.dst, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
+  if (prm.diis) diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgT2S, b.partD, s);
+  ctl_after_diis(b.dm, b.dst, b.dact, b.nm, prm, stop, s);
+  HCHECK(hipGetLastError());
+}
+
+// integral precompute (Fao, undressed (kc|ld), L matrices)
+static void precompute(hcc_ctx* c) {
+  Batch& b = *c->batch;
+  hipStream_t s = c->stream;
+  reset_state(b.dst, b.dact, b.nm, s);
+  // t1 = 0 -> undressed orbital products
+  for (int k = 0; k < b.nm; ++k)
+    HCHECK(hipMemsetAsync(b.hm[k].t1, 0, sizeof(double) * b.o[k] * b.v[k], s));
+  HCHECK(launch_gemm_f64(b.pre[0].d, b.pre[0].nprob, b.pre[0].tiles, b.dact, s));
+  HCHECK(launch_gemm_f64(b.pre[1].d, b.pre[1].nprob, b.pre[1].tiles, b.dact, s));
+  ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
+  ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  gemm(c, b.it[G1_Q]);
+  HCHECK(launch_gemm_f64(b.pre[2].d, b.pre[2].nprob, b.pre[2].tiles, b.dact, s));
+  make_L(b.dm, b.rgT2, b.nm, s);
+  HCHECK(hipGetLastError());
+}
+
+static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const double* C, const double* F,
+                 const double* G, const double* eref, bool keep_mp2) {
+  delete c->batch;
+  c->batch = nullptr;
+  Batch* b = new Batch();
+  c->batch = b;
+  b->nm = nmol;
+  b->m = c->diis_steps;
+  for (int k = 0; k < nmol; ++k) {
+    if (n[k] < 2 || nocc[k] < 1 || nocc[k] >= n[k]) throw std::string("invalid molecule dimensions");
+    b->n.push_back(n[k]);
+    b->o.push_back(nocc[k]);
+    b->v.push_back(n[k] - nocc[k]);
+    b->eref.push_back(eref[k]);
+  }
+  allocate(*b, keep_mp2);
+  build_tables(*b);
+  // upload, transposing the column-major inputs to row-major
+  long off = 0;
+  std::vector<double> h;
+  for (int k = 0; k < nmol; ++k) {
+    const int N = n[k];
+    const Mol& M = b->hm[k];
+    h.resize((size_t)N * N);
+    const double* srcs[3] = {C + off, F + off, G + off};
+    double* dsts[3] = {M.C, M.F, M.G};
+    for (int q = 0; q < 3; ++q) {
+      for (int r = 0; r < N; ++r)
+        for (int cc = 0; cc < N; ++cc) h[(size_t)r * N + cc] = srcs[q][r + (size_t)cc * N];
+      HCHECK(hipMemcpyAsync(dsts[q], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HCHECK(hipStreamSynchronize(c->stream));
+    }
+    off += (long)N * N;
+  }
+  precompute(c);
+  HCHECK(hipStreamSynchronize(c->stream));
+  c->have_guess = false;
+}
+
+}  // namespace hcc
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define API_BEGIN try {
+#define API_END                                                                     \
+  }                                                                                 \
+  catch (const HipError& he) {                                                      \
+    ctx->err = std::string(he.what) + ": " + hipGetErrorString(he.e);              \
+    return HCC_E_HIP;                                                               \
+  }                                                                                 \
+  catch (const std::string& se) {                                                   \
+    ctx->err = se;                                                                  \
+    return HCC_E_ARG;                                                               \
+  }                                                                                 \
+  catch (const std::bad_alloc&) {                                                   \
+    ctx->err = "host allocation failed";                                            \
+    return HCC_E_MEMORY;                                                            \
+  }                                                                                 \
+  return HCC_OK;
+
+extern "C" {
+
+int hcc_create(hcc_ctx** out, int device, int precision, int diis_steps) {
+  if (out == nullptr) return HCC_E_ARG;
+  *out = nullptr;
+  if (precision != HCC_FP64 && precision != HCC_MIXED) return HCC_E_ARG;
+  if (diis_steps < 2 || diis_steps > kMaxDiis) return HCC_E_ARG;
+  hcc_ctx* ctx = new hcc_ctx();
+  ctx->device = device;
+  ctx->precision = precision;
+  ctx->diis_steps = diis_steps;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return HCC_E_HIP;
+  }
+  *out = ctx;
+  return HCC_OK;
+}
+
+int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection_type, int diis_enabled) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (projection_type != 1 && projection_type != 2) {
+    ctx->err = "projection_type must be 1 or 2";
+    return HCC_E_ARG;
+  }
+  ctx->s1 = t1_step;
+  ctx->s2 = t2_step;
+  ctx->projtype = projection_type;
+  ctx->diis = diis_enabled ? 1 : 0;
+  return HCC_OK;
+}
+
+int hcc_set_molecule(hcc_ctx* ctx, int n, int n_occ, const double* C, const double* F_mo, const double* gamma,
+                     double e_ref) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (C == nullptr || F_mo == nullptr || gamma == nullptr) {
+    ctx->err = "null input array";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  load(ctx, 1, &n, &n_occ, C, F_mo, gamma, &e_ref, false);
+  API_END
+}
+
+int hcc_batch_load(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* C, const double* F_mo,
+                   const double* gamma, const double* e_ref) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (nmol < 1 || !n || !n_occ || !C || !F_mo || !gamma || !e_ref) {
+    ctx->err = "invalid batch arguments";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  load(ctx, nmol, n, n_occ, C, F_mo, gamma, e_ref, false);
+  API_END
+}
+
+static int need_batch(hcc_ctx* ctx) {
+  if (ctx->batch == nullptr) {
+    ctx->err = "no molecule loaded";
+    return HCC_E_STATE;
+  }
+  return HCC_OK;
+}
+
+int hcc_guess(hcc_ctx* ctx, int kind) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (kind != HCC_GUESS_REFERENCE && kind != HCC_GUESS_MP2) {
+    ctx->err = "unsupported guess kind";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind, ctx->stream);
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  ctx->have_guess = true;
+  API_END
+}
+
+int hcc_iterate(hcc_ctx* ctx, int iteration, double eps, double accuracy[5]) {
+  if (ctx == nullptr || accuracy == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (!ctx->have_guess) {
+    ctx->err = "hcc_guess must precede hcc_iterate";
+    return HCC_E_STATE;
+  }
+  (void)iteration;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  activate(b.dst, b.dact, b.nm, ctx->stream);
+  Params prm = params(ctx, eps, 0, false);
+  run_iteration(ctx, prm, nullptr);
+  State S;
+  HCHECK(hipMemcpyAsync(&S, b.dst, sizeof(State), hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  accuracy[0] = S.acc[0];
+  accuracy[1] = S.acc[1];
+  accuracy[2] = accuracy[3] = accuracy[4] = -1.0;
+  API_END
+}
+
+int hcc_energy(hcc_ctx* ctx, double energy_out[5]) {
+  if (ctx == nullptr || energy_out == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, ctx->stream);
+  State S;
+  HCHECK(hipMemcpyAsync(&S, b.dst, sizeof(State), hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 5; ++k) energy_out[k] = S.energy[k];
+  API_END
+}
+
+int hcc_get_amplitudes(hcc_ctx* ctx, double* t1, double* t2) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  const int o = b.o[0], v = b.v[0];
+  std::vector<double> h1((size_t)o * v), h2((size_t)o * o * v * v);
+  HCHECK(hipMemcpyAsync(h1.data(), b.hm[0].t1, h1.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipMemcpyAsync(h2.data(), b.hm[0].t2, h2.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  if (t1)
+    for (int i = 0; i < o; ++i)
+      for (int a = 0; a < v; ++a) t1[i + (size_t)a * o] = h1[(size_t)i * v + a];
+  if (t2)
+    for (int i = 0; i < o; ++i)
+      for (int j = 0; j < o; ++j)
+        for (int a = 0; a < v; ++a)
+          for (int bb = 0; bb < v; ++bb)
+            t2[i + (size_t)o * (j + (size_t)o * (a + (size_t)v * bb))] =
+                h2[(((size_t)i * o + j) * v + a) * v + bb];
+  API_END
+}
+
+int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2) {
+  if (ctx == nullptr || t1 == nullptr || t2 == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  const int o = b.o[0], v = b.v[0];
+  std::vector<double> h1((size_t)o * v), h2((size_t)o * o * v * v);
+  for (int i = 0; i < o; ++i)
+    for (int a = 0; a < v; ++a) h1[(size_t)i * v + a] = t1[i + (size_t)a * o];
+  for (int i = 0; i < o; ++i)
+    for (int j = 0; j < o; ++j)
+      for (int a = 0; a < v; ++a)
+        for (int bb = 0; bb < v; ++bb)
+          h2[(((size_t)i * o + j) * v + a) * v + bb] = t2[i + (size_t)o * (j + (size_t)o * (a + (size_t)v * bb))];
+  HCHECK(hipMemcpyAsync(b.hm[0].t1, h1.data(), h1.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  HCHECK(hipMemcpyAsync(b.hm[0].t2, h2.data(), h2.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->have_guess) reset_state(b.dst, b.dact, b.nm, ctx->stream);
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  ctx->have_guess = true;
+  API_END
+}
+
+int hcc_residual(hcc_ctx* ctx, double* d1, double* d2) {
+  if (ctx == nullptr || d1 == nullptr || d2 == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  activate(b.dst, b.dact, b.nm, ctx->stream);
+  residual(ctx);
+  const int o = b.o[0], v = b.v[0];
+  const size_t n1 = (size_t)o * v, n2 = (size_t)o * o * v * v;
+  if (b.d1out == nullptr) {
+    b.d1out = b.dalloc<double>(n1);
+    b.d2out = b.dalloc<double>(n2);
+  }
+  ew_contravariant(b.dm, b.rgT2S, b.nm, b.d1out, b.d2out, ctx->stream);
+  std::vector<double> h1(n1), h2(n2);
+  HCHECK(hipMemcpyAsync(h1.data(), b.d1out, n1 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipMemcpyAsync(h2.data(), b.d2out, n2 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < o; ++i)
+    for (int a = 0; a < v; ++a) d1[i + (size_t)a * o] = h1[(size_t)i * v + a];
+  for (int i = 0; i < o; ++i)
+    for (int j = 0; j < o; ++j)
+      for (int a = 0; a < v; ++a)
+        for (int bb = 0; bb < v; ++bb)
+          d2[i + (size_t)o * (j + (size_t)o * (a + (size_t)v * bb))] = h2[(((size_t)i * o + j) * v + a) * v + bb];
+  API_END
+}
+
+static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
+  Batch& b = *ctx->batch;
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventRecord(e0, s));
+  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, guess_kind, s);
+  Params prm = params(ctx, eps, maxiter, true);
+  std::vector<int> act(b.nm);
+  const int check_every = 4;
+  int it = 0;
+  double iters_total = 0.0;
+  while (it < maxiter) {
+    for (int q = 0; q < check_every && it < maxiter; ++q, ++it) run_iteration(ctx, prm, nullptr);
+    HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
+    HCHECK(hipStreamSynchronize(s));
+    int alive = 0;
+    for (int a : act) alive += a;
+    if (alive == 0) break;
+  }
+  energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
+  HCHECK(hipEventRecord(e1, s));
+  HCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, e0, e1));
+  ctx->last_ms = ms;
+  std::vector<State> S(b.nm);
+  HCHECK(hipMemcpy(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost));
+  for (auto& x : S) iters_total += x.iter;
+  ctx->last_iters = iters_total;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->have_guess = true;
+  return HCC_OK;
+}
+
+int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double energy_out[5], int* iterations,
+              int* status) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (maxiter < 1) {
+    ctx->err = "maxiter must be positive";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  run_solve(ctx, guess_kind, maxiter, eps);
+  State S;
+  HCHECK(hipMemcpy(&S, ctx->batch->dst, sizeof(State), hipMemcpyDeviceToHost));
+  if (energy_out)
+    for (int k = 0; k < 5; ++k) energy_out[k] = S.energy[k];
+  if (iterations) *iterations = S.iter;
+  if (status) *status = S.status;
+  API_END
+}
+
+int hcc_batch_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (maxiter < 1) {
+    ctx->err = "maxiter must be positive";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  run_solve(ctx, guess_kind, maxiter, eps);
+  API_END
+}
+
+int hcc_batch_results(hcc_ctx* ctx, double* energies, int* iterations, int* status) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  std::vector<State> S(b.nm);
+  HCHECK(hipMemcpy(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost));
+  for (int k = 0; k < b.nm; ++k) {
+    if (energies)
+      for (int q = 0; q < 5; ++q) energies[5 * k + q] = S[k].energy[q];
+    if (iterations) iterations[k] = S[k].iter;
+    if (status) status[k] = S[k].status;
+  }
+  API_END
+}
+
+int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (solve_ms) *solve_ms = ctx->last_ms;
+  if (iterations_total) *iterations_total = ctx->last_iters;
+  return HCC_OK;
+}
+
+void hcc_destroy(hcc_ctx* ctx) {
+  if (ctx == nullptr) return;
+  (void)hipSetDevice(ctx->device);
+  delete ctx->batch;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* hcc_last_error(const hcc_ctx* ctx) {
+  if (ctx == nullptr) return "null context";
+  return ctx->err.c_str();
+}
+
+int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C, double alpha, double beta,
+                      int precision) {
+  if (m < 1 || n < 1 || k < 1 || !A || !B || !C) return HCC_E_ARG;
+  if (precision != HCC_FP64 && precision != HCC_MIXED) return HCC_E_ARG;
+  double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  GemmProb* dp = nullptr;
+  int rc = HCC_OK;
+  try {
+    HCHECK(hipMalloc(&dA, sizeof(double) * m * k));
+    HCHECK(hipMalloc(&dB, sizeof(double) * k * n));
+    HCHECK(hipMalloc(&dC, sizeof(double) * m * n));
+    HCHECK(hipMalloc(&dp, sizeof(GemmProb)));
+    HCHECK(hipMemcpy(dA, A, sizeof(double) * m * k, hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(dB, B, sizeof(double) * k * n, hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(dC, C, sizeof(double) * m * n, hipMemcpyHostToDevice));
+    GemmProb p = prob(rm(dA, k), rm(dB, n), rm(dC, n), m, n, k, alpha, beta);
+    p.tiles_n = (n + kBN - 1) / kBN;
+    p.tile_begin = 0;
+    p.a_kfast = 1;
+    p.b_nfast = 1;
+    const int tiles = ((m + kBM - 1) / kBM) * p.tiles_n;
+    HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
+    if (precision == HCC_FP64) HCHECK(launch_gemm_f64(dp, 1, tiles, nullptr, nullptr));
+    else HCHECK(launch_gemm_bf16(dp, 1, tiles, nullptr, nullptr));
+    HCHECK(hipDeviceSynchronize());
+    HCHECK(hipMemcpy(C, dC, sizeof(double) * m * n, hipMemcpyDeviceToHost));
+  } catch (const HipError&) {
+    rc = HCC_E_HIP;
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  (void)hipFree(dp);
+  return rc;
+}
+
+}  // extern "C"

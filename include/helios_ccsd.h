@@ -1,0 +1,124 @@
+/*
+ * libhelios_ccsd -- closed-shell PPP CCSD (helios method "r-ccsd") on one
+ * AMD Instinct MI355X (gfx950).
+ *
+ * C ABI: plain pointers and sizes only.  Host arrays are owned by the caller
+ * and copied in or out; device memory, the HIP stream and the kernels belong
+ * to the context.  A context is used from one host thread at a time.
+ *
+ * Array conventions at this boundary follow the reference's Fortran storage:
+ *   n x n matrices          column-major, element (mu,p) at mu + p*n
+ *   t1(i,a)                 o x v column-major, element at i + a*o
+ *   t2(i,j,a,b)             i fastest: i + o*(j + o*(a + v*b))
+ * with occupied orbitals 0..o-1 and virtual orbitals counted from 0.
+ *
+ * Return codes: 0 success; negative values are errors (hcc_last_error()
+ * gives the message).  They map to the reference's fatal path
+ * (ierror + primaryInformation('error'), src/general/primaryInformation.f90:77-88).
+ *
+ * Iterator statuses (src/modules/mod_helios.f90:119-120):
+ *   1 converged, -1 iteration cap, -2 diverged, -3 stagnated, -4 interrupted.
+ */
+#ifndef HELIOS_CCSD_H
+#define HELIOS_CCSD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hcc_ctx hcc_ctx;
+
+#define HCC_OK 0
+#define HCC_E_ARG (-1)
+#define HCC_E_HIP (-2)
+#define HCC_E_STATE (-3)
+#define HCC_E_MEMORY (-4)
+
+#define HCC_FP64 0
+#define HCC_MIXED 1
+
+#define HCC_GUESS_REFERENCE 0 /* t2 = (2(ia|jb)-(ib|ja))/D, mod_cc.f90:634-650 */
+#define HCC_GUESS_MP2 1       /* t2 = (ia|jb)/D, mod_mbpt.f90:131-175 amplitudes */
+#define HCC_GUESS_LEARNED 2   /* MLP prediction from MP2 features */
+
+/* Create a context on HIP device `device`.  precision: HCC_FP64 or HCC_MIXED.
+ * diis_steps: DIIS depth (ccbd%diisSteps, src/input/definebd.f90:152), 2..32.
+ * Replaces the module state set up by setCCParameters('r-ccsd')
+ * (src/methods/coupledCluster/mod_cc.f90:82-131,415-428). */
+int hcc_create(hcc_ctx** ctx, int device, int precision, int diis_steps);
+
+/* Iteration options of the coupled-cluster block (src/input/definebd.f90:144-157):
+ * t1_step, t2_step (ccbd%iterStep), projection_type 1 or 2 (the '2-1' default),
+ * diis_enabled (ccbd%diisEnabled). */
+int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection_type,
+                    int diis_enabled);
+
+/* Upload one molecule and build its integrals: n orbitals (= sites), n_occ
+ * doubly occupied, C = hV (MO coefficients, columns ascending in energy),
+ * F_mo = MO Fock matrix, gamma = mol%G, e_ref = reference (SCF) energy.
+ * Replaces initCC's r-ccsd branch after the SCF: prepareFock + the R build
+ * (src/methods/coupledCluster/mod_cc.f90:538-558, 2336-2396). */
+int hcc_set_molecule(hcc_ctx* ctx, int n, int n_occ, const double* C, const double* F_mo,
+                     const double* gamma, double e_ref);
+
+/* Initial amplitudes: guessCC (mod_cc.f90:634-650) for HCC_GUESS_REFERENCE. */
+int hcc_guess(hcc_ctx* ctx, int kind);
+
+/* One iteration: residuals, denominator update, and -- unless
+ * max(accuracy) < eps -- DIIS push and extrapolation.  accuracy(1:2) =
+ * max|d1|, max|d2| of the reference's contravariant residuals, the rest -1.
+ * Mirrors iterationCC (mod_cc.f90:820-891) called by iterator
+ * (src/calculation/gIterator.f90:94-96). */
+int hcc_iterate(hcc_ctx* ctx, int iteration, double eps, double accuracy[5]);
+
+/* energy(1)=total, (2)=reference, (3)=singles, (4)=doubles, (5)=0.
+ * Mirrors energyCC r-ccsd (mod_cc.f90:1916-1952). */
+int hcc_energy(hcc_ctx* ctx, double energy[5]);
+
+/* Copy amplitudes out / in (the public t1, t2 of module coupledCluster,
+ * mod_cc.f90:69-74). */
+int hcc_get_amplitudes(hcc_ctx* ctx, double* t1, double* t2);
+int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2);
+
+/* Reference-form residuals d1(i,a), d2(i,j,a,b) at the current amplitudes
+ * (projection_ccsd_singles/doubles_spatial_hf, ccsd/proj-ccsd-spat-hf-1.f90,
+ * -2.f90), without changing the amplitudes. */
+int hcc_residual(hcc_ctx* ctx, double* d1, double* d2);
+
+/* Whole solve on the device: guess, then iterations with the iterator's
+ * convergence, divergence (1e3), iteration-cap and stagnation rules
+ * (gIterator.f90:165-206), then the energy.  Equivalent to getEnergy's CC
+ * branch after initCC (src/modules/mod_property.f90:159-177). */
+int hcc_solve(hcc_ctx* ctx, int guess, int maxiter, double eps, double energy[5], int* iterations,
+              int* status);
+
+/* Batched solves of independent molecules (no reference counterpart: the
+ * reference solves one molecule per getEnergy call; the batch serves the
+ * finite-field polarizability loop, mod_property.f90:405-506, and molecule
+ * screening).  Molecule k has n[k] sites and n_occ[k] occupied orbitals; C,
+ * F_mo and gamma hold the n[k] x n[k] column-major matrices back to back.
+ * hcc_batch_load uploads and builds integrals; hcc_batch_solve runs the solve
+ * from device-resident inputs; hcc_batch_results copies energies (5 per
+ * molecule), iteration counts and statuses out. */
+int hcc_batch_load(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* C,
+                   const double* F_mo, const double* gamma, const double* e_ref);
+int hcc_batch_solve(hcc_ctx* ctx, int guess, int maxiter, double eps);
+int hcc_batch_results(hcc_ctx* ctx, double* energies, int* iterations, int* status);
+
+/* Device timing of the last hcc_solve / hcc_batch_solve in milliseconds, and
+ * of the most expensive kernel class (see libhelios_ccsd's profile labels). */
+int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total);
+
+void hcc_destroy(hcc_ctx* ctx);
+const char* hcc_last_error(const hcc_ctx* ctx);
+
+/* Self-test of the grouped GEMM engine on row-major A (m x k), B (k x n):
+ * C = alpha A B + beta C.  No reference counterpart; used by the tests. */
+int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C,
+                      double alpha, double beta, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HELIOS_CCSD_H */
