@@ -38,6 +38,9 @@ SIGNATURES = {
     "hcc_batch_solve": (_i, [_vp, _i, _i, _d]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_profile": (_i, [_vp, _i]),
+    "hcc_profile_read": (_i, [_vp, _P(_d), _P(ctypes.c_long)]),
+    "hcc_gemm_flops": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_destroy": (None, [_vp]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
@@ -175,6 +178,21 @@ class Context:
         st = np.zeros(self.nmol, dtype=np.int32)
         self._check(self.lib.hcc_batch_results(self.h, _dp(e), _ip(it), _ip(st)), "hcc_batch_results")
         return e, it, st
+
+    def profile(self, enable=True):
+        self._check(self.lib.hcc_profile(self.h, int(enable)), "hcc_profile")
+
+    def profile_read(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        self._check(self.lib.hcc_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)), "hcc_profile_read")
+        return ms.value, n.value
+
+    def gemm_flops(self, nmol):
+        it = np.zeros(nmol)
+        pre = np.zeros(nmol)
+        self._check(self.lib.hcc_gemm_flops(self.h, _dp(it), _dp(pre)), "hcc_gemm_flops")
+        return it, pre
 
     def last_timing(self):
         ms = ctypes.c_double()

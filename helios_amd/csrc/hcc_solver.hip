@@ -429,13 +429,43 @@ struct hcc_ctx {
   bool have_guess = false;
   double last_ms = 0.0;
   double last_iters = 0.0;
+  // live timing of the contraction GEMM launches (HIP events on `stream`)
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  size_t prof_used = 0;
+  double prof_ms = 0.0;
+  long prof_launches = 0;
 };
 
 namespace hcc {
 
 static void gemm(hcc_ctx* c, const GemmTable& t) {
   if (t.nprob == 0) return;
+  if (c->prof) {
+    if (c->prof_used == c->prof_events.size()) {
+      hipEvent_t a, b;
+      HCHECK(hipEventCreate(&a));
+      HCHECK(hipEventCreate(&b));
+      c->prof_events.emplace_back(a, b);
+    }
+    auto& ev = c->prof_events[c->prof_used++];
+    HCHECK(hipEventRecord(ev.first, c->stream));
+    HCHECK(launch_gemm_f64(t.d, t.nprob, t.tiles, c->batch->dact, c->stream));
+    HCHECK(hipEventRecord(ev.second, c->stream));
+    return;
+  }
   HCHECK(launch_gemm_f64(t.d, t.nprob, t.tiles, c->batch->dact, c->stream));
+}
+
+// folds the recorded launch intervals into prof_ms (call after a synchronize)
+static void prof_collect(hcc_ctx* c) {
+  for (size_t q = 0; q < c->prof_used; ++q) {
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, c->prof_events[q].first, c->prof_events[q].second));
+    c->prof_ms += ms;
+  }
+  c->prof_launches += (long)c->prof_used;
+  c->prof_used = 0;
 }
 
 static Params params(const hcc_ctx* c, double eps, int maxiter, bool iterator_rules) {
@@ -491,12 +521,12 @@ static void precompute(hcc_ctx* c) {
   // t1 = 0 -> undressed orbital products
   for (int k = 0; k < b.nm; ++k)
     HCHECK(hipMemsetAsync(b.hm[k].t1, 0, sizeof(double) * b.o[k] * b.v[k], s));
-  HCHECK(launch_gemm_f64(b.pre[0].d, b.pre[0].nprob, b.pre[0].tiles, b.dact, s));
-  HCHECK(launch_gemm_f64(b.pre[1].d, b.pre[1].nprob, b.pre[1].tiles, b.dact, s));
+  gemm(c, b.pre[0]);
+  gemm(c, b.pre[1]);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
   gemm(c, b.it[G1_Q]);
-  HCHECK(launch_gemm_f64(b.pre[2].d, b.pre[2].nprob, b.pre[2].tiles, b.dact, s));
+  gemm(c, b.pre[2]);
   make_L(b.dm, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
 }
@@ -535,8 +565,6 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
     }
     off += (long)N * N;
   }
-  precompute(c);
-  HCHECK(hipStreamSynchronize(c->stream));
   c->have_guess = false;
 }
 
@@ -605,6 +633,8 @@ int hcc_set_molecule(hcc_ctx* ctx, int n, int n_occ, const double* C, const doub
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   load(ctx, 1, &n, &n_occ, C, F_mo, gamma, &e_ref, false);
+  precompute(ctx);
+  HCHECK(hipStreamSynchronize(ctx->stream));
   API_END
 }
 
@@ -752,13 +782,14 @@ int hcc_residual(hcc_ctx* ctx, double* d1, double* d2) {
   API_END
 }
 
-static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
+static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool build_integrals) {
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
   hipEvent_t e0, e1;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventRecord(e0, s));
+  if (build_integrals) precompute(ctx);
   guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, guess_kind, s);
   Params prm = params(ctx, eps, maxiter, true);
   std::vector<int> act(b.nm);
@@ -769,6 +800,7 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
     for (int q = 0; q < check_every && it < maxiter; ++q, ++it) run_iteration(ctx, prm, nullptr);
     HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
     HCHECK(hipStreamSynchronize(s));
+    if (ctx->prof) prof_collect(ctx);
     int alive = 0;
     for (int a : act) alive += a;
     if (alive == 0) break;
@@ -776,6 +808,7 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
   HCHECK(hipEventRecord(e1, s));
   HCHECK(hipEventSynchronize(e1));
+  if (ctx->prof) prof_collect(ctx);
   float ms = 0.f;
   HCHECK(hipEventElapsedTime(&ms, e0, e1));
   ctx->last_ms = ms;
@@ -798,7 +831,7 @@ int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double ener
     return HCC_E_ARG;
   }
   API_BEGIN
-  run_solve(ctx, guess_kind, maxiter, eps);
+  run_solve(ctx, guess_kind, maxiter, eps, false);
   State S;
   HCHECK(hipMemcpy(&S, ctx->batch->dst, sizeof(State), hipMemcpyDeviceToHost));
   if (energy_out)
@@ -816,7 +849,7 @@ int hcc_batch_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
     return HCC_E_ARG;
   }
   API_BEGIN
-  run_solve(ctx, guess_kind, maxiter, eps);
+  run_solve(ctx, guess_kind, maxiter, eps, true);
   API_END
 }
 
@@ -843,9 +876,52 @@ int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total) {
   return HCC_OK;
 }
 
+int hcc_profile(hcc_ctx* ctx, int enable) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  ctx->prof = enable != 0;
+  ctx->prof_used = 0;
+  ctx->prof_ms = 0.0;
+  ctx->prof_launches = 0;
+  return HCC_OK;
+}
+
+int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (gemm_ms) *gemm_ms = ctx->prof_ms;
+  if (gemm_launches) *gemm_launches = ctx->prof_launches;
+  return HCC_OK;
+}
+
+int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  std::vector<GemmProb> h;
+  auto table_flops = [&](const GemmTable& t, std::vector<double>& acc) {
+    h.resize(t.nprob);
+    if (t.nprob)
+      HCHECK(hipMemcpy(h.data(), t.d, t.nprob * sizeof(GemmProb), hipMemcpyDeviceToHost));
+    for (const auto& p : h) acc[p.mol] += 2.0 * p.M * (double)p.N * p.K;
+  };
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  std::vector<double> it(b.nm, 0.0), pre(b.nm, 0.0);
+  for (const auto& t : b.it) table_flops(t, it);
+  for (const auto& t : b.pre) table_flops(t, pre);
+  table_flops(b.it[G1_Q], pre);
+  for (int k = 0; k < b.nm; ++k) {
+    if (per_iteration) per_iteration[k] = it[k];
+    if (per_solve_setup) per_solve_setup[k] = pre[k];
+  }
+  API_END
+}
+
 void hcc_destroy(hcc_ctx* ctx) {
   if (ctx == nullptr) return;
   (void)hipSetDevice(ctx->device);
+  for (auto& ev : ctx->prof_events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
   delete ctx->batch;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

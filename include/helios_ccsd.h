@@ -97,9 +97,10 @@ int hcc_solve(hcc_ctx* ctx, int guess, int maxiter, double eps, double energy[5]
  * finite-field polarizability loop, mod_property.f90:405-506, and molecule
  * screening).  Molecule k has n[k] sites and n_occ[k] occupied orbitals; C,
  * F_mo and gamma hold the n[k] x n[k] column-major matrices back to back.
- * hcc_batch_load uploads and builds integrals; hcc_batch_solve runs the solve
- * from device-resident inputs; hcc_batch_results copies energies (5 per
- * molecule), iteration counts and statuses out. */
+ * hcc_batch_load uploads the inputs; hcc_batch_solve runs the whole solve from
+ * the device-resident inputs (integral build, guess, iterations, energy);
+ * hcc_batch_results copies energies (5 per molecule), iteration counts and
+ * statuses out. */
 int hcc_batch_load(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* C,
                    const double* F_mo, const double* gamma, const double* e_ref);
 int hcc_batch_solve(hcc_ctx* ctx, int guess, int maxiter, double eps);
@@ -108,6 +109,17 @@ int hcc_batch_results(hcc_ctx* ctx, double* energies, int* iterations, int* stat
 /* Device timing of the last hcc_solve / hcc_batch_solve in milliseconds, and
  * of the most expensive kernel class (see libhelios_ccsd's profile labels). */
 int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total);
+
+/* Live timing of the contraction GEMM launches (the dominant kernel): when
+ * enabled, every grouped-GEMM launch is bracketed by HIP events on the
+ * context's stream; hcc_profile_read returns the summed launch time and the
+ * launch count since hcc_profile(ctx, 1).  hcc_gemm_flops gives, per loaded
+ * molecule, the GEMM flops (2*M*N*K over its problems) of one iteration and of
+ * the once-per-solve integral set-up.  No reference counterpart (the
+ * reference has only wall-clock timers, src/modules/mod_global.f90:2025-2039). */
+int hcc_profile(hcc_ctx* ctx, int enable);
+int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches);
+int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup);
 
 void hcc_destroy(hcc_ctx* ctx);
 const char* hcc_last_error(const hcc_ctx* ctx);
