@@ -41,6 +41,7 @@ SIGNATURES = {
     "hcc_profile": (_i, [_vp, _i]),
     "hcc_profile_read": (_i, [_vp, _P(_d), _P(ctypes.c_long)]),
     "hcc_gemm_flops": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_profile_ops": (_i, [_vp, _P(_d)]),
     "hcc_destroy": (None, [_vp]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
@@ -187,6 +188,11 @@ class Context:
         n = ctypes.c_long()
         self._check(self.lib.hcc_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)), "hcc_profile_read")
         return ms.value, n.value
+
+    def profile_ops(self):
+        ms = np.zeros(64)
+        self._check(self.lib.hcc_profile_ops(self.h, _dp(ms)), "hcc_profile_ops")
+        return ms
 
     def gemm_flops(self, nmol):
         it = np.zeros(nmol)

@@ -23,6 +23,8 @@ struct Mol {
   double *st, *sd, *Bm;
   // guess features / MP2 amplitudes (o^2 v^2)
   double *t2mp2;
+  // dressed-density diagonal and its Coulomb field (2n)
+  double *dvec;
 };
 
 // Per-molecule iteration state (reference iterator semantics,
@@ -57,6 +59,7 @@ struct Params {
 // block -> molecule table for ragged elementwise launches
 struct Ragged {
   int* prefix;   // device, nm+1 block prefix
+  int* bmol;     // device, molecule of every block
   int nblocks;
 };
 

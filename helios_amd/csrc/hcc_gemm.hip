@@ -26,14 +26,10 @@ __device__ __forceinline__ int col_off(const Operand& o, int c) {
   return (c / o.rc) * o.sch + (c % o.rc) * o.scl;
 }
 
-// Finds the problem owning global tile `tile` (binary search on tile_begin).
-__device__ __forceinline__ int find_problem(const GemmProb* probs, int nprob, int tile) {
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (probs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-  }
-  return lo;
+// Problem owning global tile `tile`: one load from the table's tile map
+// (a binary search over problems would be a chain of dependent loads).
+__device__ __forceinline__ int find_problem(const int* __restrict__ tile_prob, int tile) {
+  return tile_prob[tile];
 }
 
 __device__ __forceinline__ int logical_tile(int total) {
@@ -41,112 +37,82 @@ __device__ __forceinline__ int logical_tile(int total) {
   return (int)(blockIdx.x & 7) * t8 + (int)(blockIdx.x >> 3);
 }
 
-constexpr int kLdA = kBM + 4;   // LDS row pitch (doubles) of the k-major A tile
-constexpr int kLdB = kBN + 4;
+// ---------------------------------------------------------------------------
+// FP64 kernel.  Template on the output tile (BM x BN, 32x32 per wave, 4 or 1
+// waves).  LDS layouts follow the operand's unit stride so that both the
+// staging stores and the MFMA fragment reads are free of bank conflicts:
+//   K-contiguous operands are staged [row][k] with a 17-double pitch,
+//   row-contiguous operands [k][row] with a (BM+2)-double pitch.
+// The composite-index offsets along K are precomputed once per work item into
+// LDS tables (chunks up to kKTab), so the slab loop does no integer division.
+// ---------------------------------------------------------------------------
+constexpr int kKTab = 2048;
+constexpr int kPk = kBK + 1;   // pitch of the [row][k] staging layout
 
-// Loads this thread's 4 A elements and 4 B elements of k-slab k0.
-struct Frag {
-  double a[4], b[4];
+template <int BM, int BN>
+struct F64Tile {
+  static constexpr int kWaves = (BM / 32) * (BN / 32);
+  static constexpr int kT = 64 * kWaves;
+  static constexpr int kPa = BM + 2;          // pitch of the [k][m] layout
+  static constexpr int kPb = BN + 2;
+  static constexpr int kLdsA = (BM * kPk > kBK * kPa) ? BM * kPk : kBK * kPa;
+  static constexpr int kLdsB = (BN * kPk > kBK * kPb) ? BN * kPk : kBK * kPb;
+  static constexpr int kNA = BM * kBK / kT;   // A elements per thread per slab
+  static constexpr int kNB = BN * kBK / kT;
 };
 
-__device__ __forceinline__ void load_slab(const GemmProb& P, int m0, int n0, int k0, int t,
-                                          const int (&ra)[4], const int (&cb)[4], Frag& f) {
-  const double* __restrict__ A = P.A.p;
-  const double* __restrict__ B = P.B.p;
-  if (P.a_kfast) {
-    int k = k0 + (t & 15);
-    bool kin = k < P.K;
-    int co = kin ? col_off(P.A, k) : 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int m = m0 + (t >> 4) + 16 * i;
-      f.a[i] = (kin && m < P.M) ? A[ra[i] + co] : 0.0;
-    }
-  } else {
-    int m = m0 + (t & 63);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int k = k0 + (t >> 6) + 4 * i;
-      f.a[i] = (m < P.M && k < P.K) ? A[ra[0] + col_off(P.A, k)] : 0.0;
-    }
-  }
-  if (P.b_nfast) {
-    int n = n0 + (t & 63);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int k = k0 + (t >> 6) + 4 * i;
-      f.b[i] = (n < P.N && k < P.K) ? B[row_off(P.B, k) + cb[0]] : 0.0;
-    }
-  } else {
-    int k = k0 + (t & 15);
-    bool kin = k < P.K;
-    int ro = kin ? row_off(P.B, k) : 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int n = n0 + (t >> 4) + 16 * i;
-      f.b[i] = (kin && n < P.N) ? B[ro + cb[i]] : 0.0;
-    }
-  }
-}
+#define HCC_F64_THREADS(BM_, BN_) (64 * ((BM_) / 32) * ((BN_) / 32))
 
-__device__ __forceinline__ void store_slab(const GemmProb& P, int t, const Frag& f,
-                                           double* As, double* Bs) {
-  if (P.a_kfast) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) As[(t & 15) * kLdA + (t >> 4) + 16 * i] = f.a[i];
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) As[((t >> 6) + 4 * i) * kLdA + (t & 63)] = f.a[i];
-  }
-  if (P.b_nfast) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Bs[((t >> 6) + 4 * i) * kLdB + (t & 63)] = f.b[i];
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Bs[(t & 15) * kLdB + (t >> 4) + 16 * i] = f.b[i];
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void gemm_f64_kernel(const GemmProb* __restrict__ probs,
-                                                            int nprob, int total_tiles,
-                                                            const int* __restrict__ active) {
-  __shared__ double As[kBK * kLdA];
-  __shared__ double Bs[kBK * kLdB];
+// AKF: A's unit stride runs along K; BNF: B's unit stride runs along N.  Both
+// are template parameters so the slab loop is straight-line code and the
+// accumulators stay in AGPRs across iterations.
+template <int BM, int BN, bool AKF, bool BNF>
+__global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const GemmProb* __restrict__ probs,
+                                                                           const int* __restrict__ tile_prob,
+                                                                           int total_tiles,
+                                                                           const int* __restrict__ active) {
+  using TL = F64Tile<BM, BN>;
+  constexpr int T = TL::kT;
+  __shared__ double As[TL::kLdsA];
+  __shared__ double Bs[TL::kLdsB];
+  __shared__ int koffA[kKTab];
+  __shared__ int koffB[kKTab];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
-  const GemmProb P = probs[find_problem(probs, nprob, tile)];
+  const GemmProb P = probs[find_problem(tile_prob, tile)];
   if (active != nullptr && active[P.mol] == 0) return;
-  const int lt = tile - P.tile_begin;
-  const int m0 = (lt / P.tiles_n) * kBM;
-  const int n0 = (lt % P.tiles_n) * kBN;
+  const int tiles_n = (P.N + BN - 1) / BN;
+  const int tiles_mn = ((P.M + BM - 1) / BM) * tiles_n;
+  int lt = tile - P.tile_begin;
+  const int ks = lt / tiles_mn;
+  lt -= ks * tiles_mn;
+  const int m0 = (lt / tiles_n) * BM;
+  const int n0 = (lt % tiles_n) * BN;
+  const int kbeg = ks * P.kchunk;
+  const int kend = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave / (BN / 32)) * 32, wn = (wave % (BN / 32)) * 32;
 
-  // fixed row offsets of A and column offsets of B for this thread
-  int ra[4], cb[4];
-  if (P.a_kfast) {
+  // fixed (non-K) offsets of this thread's staged elements; -1 marks padding
+  int fa[TL::kNA], fb[TL::kNB], kqa[TL::kNA], kqb[TL::kNB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int m = m0 + (t >> 4) + 16 * i;
-      ra[i] = m < P.M ? row_off(P.A, m) : 0;
-    }
-  } else {
-    int m = m0 + (t & 63);
-    ra[0] = m < P.M ? row_off(P.A, m) : 0;
-    ra[1] = ra[2] = ra[3] = 0;
+  for (int i = 0; i < TL::kNA; ++i) {
+    const int q = t + i * T;
+    const int m = m0 + (AKF ? q / kBK : q % BM);
+    kqa[i] = AKF ? q % kBK : q / BM;
+    fa[i] = m < P.M ? row_off(P.A, m) : -1;
   }
-  if (P.b_nfast) {
-    int n = n0 + (t & 63);
-    cb[0] = n < P.N ? col_off(P.B, n) : 0;
-    cb[1] = cb[2] = cb[3] = 0;
-  } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int n = n0 + (t >> 4) + 16 * i;
-      cb[i] = n < P.N ? col_off(P.B, n) : 0;
-    }
+  for (int i = 0; i < TL::kNB; ++i) {
+    const int q = t + i * T;
+    const int n = n0 + (BNF ? q % BN : q / kBK);
+    kqb[i] = BNF ? q / BN : q % kBK;
+    fb[i] = n < P.N ? col_off(P.B, n) : -1;
   }
+  const double* __restrict__ A = P.A.p;
+  const double* __restrict__ B = P.B.p;
+  double ra[TL::kNA], rb[TL::kNB];
 
   d4 acc[2][2];
 #pragma unroll
@@ -154,29 +120,82 @@ __global__ __launch_bounds__(kThreads) void gemm_f64_kernel(const GemmProb* __re
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  Frag f;
-  load_slab(P, m0, n0, 0, t, ra, cb, f);
-  for (int k0 = 0; k0 < P.K; k0 += kBK) {
-    store_slab(P, t, f, As, Bs);
+  for (int c0 = kbeg; c0 < kend; c0 += kKTab) {
+    const int cend = min(kend, c0 + kKTab);
     __syncthreads();
-    if (k0 + kBK < P.K) load_slab(P, m0, n0, k0 + kBK, t, ra, cb, f);
-#pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      double a0 = As[kr * kLdA + wm + (lane & 15)];
-      double a1 = As[kr * kLdA + wm + 16 + (lane & 15)];
-      double b0 = Bs[kr * kLdB + wn + (lane & 15)];
-      double b1 = Bs[kr * kLdB + wn + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int q = t; q < cend - c0; q += T) {
+      koffA[q] = col_off(P.A, c0 + q);
+      koffB[q] = row_off(P.B, c0 + q);
     }
     __syncthreads();
+    // branch-free staging: padded elements load the operand base and are zeroed
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < TL::kNA; ++i) {
+        const int kl = k0 - c0 + kqa[i];
+        const bool ok = fa[i] >= 0 && kl < cend - c0;
+        const double x = A[ok ? fa[i] + koffA[ok ? kl : 0] : 0];
+        ra[i] = ok ? x : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < TL::kNB; ++i) {
+        const int kl = k0 - c0 + kqb[i];
+        const bool ok = fb[i] >= 0 && kl < cend - c0;
+        const double x = B[ok ? fb[i] + koffB[ok ? kl : 0] : 0];
+        rb[i] = ok ? x : 0.0;
+      }
+    };
+    load(c0);
+    for (int k0 = c0; k0 < cend; k0 += kBK) {
+#pragma unroll
+      for (int i = 0; i < TL::kNA; ++i) {
+        const int q = t + i * T;
+        if (AKF) As[(q / kBK) * kPk + q % kBK] = ra[i];
+        else As[(q / BM) * TL::kPa + q % BM] = ra[i];
+      }
+#pragma unroll
+      for (int i = 0; i < TL::kNB; ++i) {
+        const int q = t + i * T;
+        if (BNF) Bs[(q / BN) * TL::kPb + q % BN] = rb[i];
+        else Bs[(q / kBK) * kPk + q % kBK] = rb[i];
+      }
+      __syncthreads();
+      load(min(k0 + kBK, cend));   // the last prefetch is fully masked
+#pragma unroll
+      for (int kk = 0; kk < kBK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        const int ma = wm + (lane & 15), nb = wn + (lane & 15);
+        const double a0 = AKF ? As[ma * kPk + kr] : As[kr * TL::kPa + ma];
+        const double a1 = AKF ? As[(ma + 16) * kPk + kr] : As[kr * TL::kPa + ma + 16];
+        const double b0 = BNF ? Bs[kr * TL::kPb + nb] : Bs[nb * kPk + kr];
+        const double b1 = BNF ? Bs[kr * TL::kPb + nb + 16] : Bs[(nb + 16) * kPk + kr];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
   }
 
   // epilogue: for the FP64 MFMA, D[row][col] of a 16x16 block sits in lane
   // (col + 16*(row%4)), element row/4
+  if (P.ksplit > 1) {
+    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 16 * j + (lane & 15);
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+          if (row < P.M) W[(long)row * P.N + col] = acc[i][j][r];
+        }
+    }
+    return;
+  }
   double* __restrict__ Cp = P.C.p;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -267,17 +286,23 @@ __device__ __forceinline__ void store_slab16(const GemmProb& P, int t, const dou
   }
 }
 
-__global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __restrict__ probs, int nprob,
+__global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __restrict__ probs,
+                                                             const int* __restrict__ tile_prob,
                                                              int total_tiles, const int* __restrict__ active) {
   __shared__ __attribute__((aligned(16))) __bf16 As[kBM * kLd16];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN * kLd16];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
-  const GemmProb P = probs[find_problem(probs, nprob, tile)];
+  GemmProb P = probs[find_problem(tile_prob, tile)];
   if (active != nullptr && active[P.mol] == 0) return;
-  const int lt = tile - P.tile_begin;
+  int lt = tile - P.tile_begin;
+  const int tiles_mn = ((P.M + kBM - 1) / kBM) * P.tiles_n;
+  const int ks = lt / tiles_mn;
+  lt -= ks * tiles_mn;
   const int m0 = (lt / P.tiles_n) * kBM;
   const int n0 = (lt % P.tiles_n) * kBN;
+  const int kbeg = ks * P.kchunk;
+  P.K = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -287,8 +312,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __r
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   double a[8], b[8];
-  load_slab16(P, m0, n0, 0, t, a, b);
-  for (int k0 = 0; k0 < P.K; k0 += kBK16) {
+  load_slab16(P, m0, n0, kbeg, t, a, b);
+  for (int k0 = kbeg; k0 < P.K; k0 += kBK16) {
     store_slab16(P, t, a, b, As, Bs);
     __syncthreads();
     if (k0 + kBK16 < P.K) load_slab16(P, m0, n0, k0 + kBK16, t, a, b);
@@ -302,6 +327,22 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __r
     acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
     acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
     __syncthreads();
+  }
+  if (P.ksplit > 1) {
+    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 16 * j + (lane & 15);
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+          if (row < P.M) W[(long)row * P.N + col] = (double)acc[i][j][r];
+        }
+    }
+    return;
   }
   double* __restrict__ Cp = P.C.p;
 #pragma unroll
@@ -324,20 +365,61 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __r
   }
 }
 
-hipError_t launch_gemm_bf16(const GemmProb* probs, int nprob, int total_tiles, const int* active,
+hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
                             hipStream_t s) {
-  if (total_tiles <= 0 || nprob <= 0) return hipSuccess;
+  if (total_tiles <= 0) return hipSuccess;
   int grid = ((total_tiles + 7) / 8) * 8;
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(grid), dim3(kThreads), 0, s, probs, nprob, total_tiles, active);
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(grid), dim3(kThreads), 0, s, probs, tile_prob, total_tiles, active);
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_f64(const GemmProb* probs, int nprob, int total_tiles, const int* active,
-                           hipStream_t s) {
-  if (total_tiles <= 0 || nprob <= 0) return hipSuccess;
+// fixed-order sum of the split-K partials into C
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmProb* __restrict__ probs,
+                                                            const int* __restrict__ prefix,
+                                                            const int* __restrict__ block_prob,
+                                                            const int* __restrict__ active) {
+  const int b = blockIdx.x;
+  const int lo = block_prob[b];
+  const GemmProb& P = probs[lo];
+  if (P.ksplit <= 1) return;
+  if (active != nullptr && active[P.mol] == 0) return;
+  const long e = (long)(b - prefix[lo]) * 256 + threadIdx.x;
+  const long mn = (long)P.M * P.N;
+  if (e >= mn) return;
+  const int row = e / P.N, col = e % P.N;
+  double s = 0.0;
+  for (int k = 0; k < P.ksplit; ++k) s += P.ws[k * mn + e];
+  const int o = row_off(P.C, row) + col_off(P.C, col);
+  double v = P.alpha * s;
+  if (P.beta != 0.0) v += P.beta * P.C.p[o];
+  P.C.p[o] = v;
+}
+
+hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const int* block_prob, int nblocks,
+                                const int* active, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(nblocks), dim3(256), 0, s, probs, prefix, block_prob, active);
+  return hipGetLastError();
+}
+
+template <int BM, int BN>
+static void launch_f64(const GemmProb* probs, const int* tp, int total_tiles, const int* active, hipStream_t s,
+                       int grid, int flags) {
+  constexpr int T = HCC_F64_THREADS(BM, BN);
+  switch (flags & 3) {
+    case 0: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    case 1: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    case 2: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    default: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+  }
+}
+
+hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
+                           hipStream_t s, int tile, int flags) {
+  if (total_tiles <= 0) return hipSuccess;
   int grid = ((total_tiles + 7) / 8) * 8;
-  hipLaunchKernelGGL(gemm_f64_kernel, dim3(grid), dim3(kThreads), 0, s, probs, nprob,
-                     total_tiles, active);
+  if (tile == 32) launch_f64<32, 32>(probs, tile_prob, total_tiles, active, s, grid, flags);
+  else launch_f64<64, 64>(probs, tile_prob, total_tiles, active, s, grid, flags);
   return hipGetLastError();
 }
 

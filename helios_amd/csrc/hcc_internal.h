@@ -28,21 +28,32 @@ struct GemmProb {
   Operand A, B, C;  // A: M x K, B: K x N, C: M x N
   int M, N, K;
   double alpha, beta;
-  int tiles_n;      // tiles along N for this problem
+  int tiles_n;      // tiles along N for this problem (for the table's tile size)
   int tile_begin;   // first global tile of this problem
   int mol;          // molecule index (for the active mask)
   int a_kfast;      // 1: A's unit stride runs along K, 0: along M
   int b_nfast;      // 1: B's unit stride runs along N, 0: along K
-  int pad_;
+  int ksplit;       // number of K chunks (1: no split)
+  int kchunk;       // K range of one chunk (multiple of the K tile)
+  double* ws;       // split-K partials: ksplit x M x N, row-major (ksplit > 1)
 };
 
 // Tile shape of the FP64 MFMA GEMM.
 constexpr int kBM = 64, kBN = 64, kBK = 16, kThreads = 256;
 
 // launches one grouped GEMM over all problems of a table (device memory)
-hipError_t launch_gemm_f64(const GemmProb* probs, int nprob, int total_tiles,
-                           const int* active, hipStream_t s);
-hipError_t launch_gemm_bf16(const GemmProb* probs, int nprob, int total_tiles,
+// tile: 64 (64x64 tiles, 256 threads) or 32 (32x32 tiles, 64 threads);
+// flags: bit 0 = A's unit stride along K, bit 1 = B's unit stride along N
+// (the same for every problem of a table)
+// tile_prob: problem index of every global tile of the table
+hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int total_tiles,
+                           const int* active, hipStream_t s, int tile = 64, int flags = 3);
+hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int total_tiles,
                             const int* active, hipStream_t s);
+// C = alpha * sum_k ws_k + beta * C for the split problems of a table;
+// prefix: block prefix over the table's problems (256 elements per block),
+// block_prob: problem index of every reduction block
+hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const int* block_prob, int nblocks,
+                                const int* active, hipStream_t s);
 
 }  // namespace hcc

@@ -9,7 +9,7 @@ constexpr int kDotItems = 8;   // elements per thread of a DIIS dot-product work
 constexpr int kMaxDiis = 32;
 
 void ew_dress(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s);
-void ew_fock_m(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s);
+void ew_fock_m(const Mol* mols, const State* st, Ragged rgN, Ragged rgNN, int nm, hipStream_t s);
 void ew_pouter(const Mol* mols, const State* st, Ragged rgN3, int nm, hipStream_t s);
 void ew_u(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
 void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
@@ -19,7 +19,7 @@ void ew_update(const Mol* mols, const State* st, Ragged rgOV, Ragged rgT2, int n
 void ew_contravariant(const Mol* mols, Ragged rgT2S, int nm, double* d1, double* d2, hipStream_t s);
 void ctl_after_update(const Mol* mols, State* st, int nm, const Params& prm, Ragged rgOV,
                       const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
-void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgT2S, double* part,
+void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part,
                hipStream_t s);
 void ctl_after_diis(const Mol* mols, State* st, int* act, int nm, const Params& prm, const int* stop,
                     hipStream_t s);

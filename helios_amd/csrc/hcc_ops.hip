@@ -25,7 +25,7 @@ __device__ __forceinline__ int block_mol(const int* __restrict__ prefix, int nm,
 }
 
 #define RAGGED_PROLOGUE(SIZE)                                         \
-  const int mol = block_mol(rg.prefix, nm, blockIdx.x);              \
+  const int mol = rg.bmol[blockIdx.x];                               \
   const int lb = blockIdx.x - rg.prefix[mol];                        \
   const Mol& M = mols[mol];                                          \
   const long e = (long)lb * kEwThreads + threadIdx.x;                \
@@ -69,6 +69,14 @@ __device__ __forceinline__ double dressed_density(const Mol& M, int mu, int nu) 
   return s;
 }
 
+// dvec[mu] = dD[mu][mu]
+__global__ __launch_bounds__(kEwThreads) void k_fock_diag(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                         Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  M.dvec[e] = dressed_density(M, (int)e, (int)e);
+}
+
 __global__ __launch_bounds__(kEwThreads) void k_fock_m(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                       Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
@@ -78,7 +86,7 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m(const Mol* __restrict__ m
   double val = M.Fao[e] - M.G[e] * dressed_density(M, mu, nu);
   if (mu == nu) {
     double s = 0.0;
-    for (int l = 0; l < N; ++l) s += M.G[(long)mu * N + l] * dressed_density(M, l, l);
+    for (int l = 0; l < N; ++l) s += M.G[(long)mu * N + l] * M.dvec[l];
     val += 2.0 * s;
   }
   M.Mf[e] = val;
@@ -123,23 +131,23 @@ __global__ __launch_bounds__(kEwThreads) void k_init_xx(const Mol* __restrict__ 
   }
 }
 
-// w2 += x + x^T - 1/2 (Zc[ai,bj] + Zc[bj,ai]) - (Zc[aj,bi] + Zc[bi,aj])
+// w2 += x + x^T - 1/2 (Z[ai,bj] + Z[bj,ai]) - (Z[aj,bi] + Z[bi,aj]), with
+// Zs[i][j][a][b] = Z[(a,i),(b,j)]
 __global__ __launch_bounds__(kEwThreads) void k_assemble(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
   const int o = M.o, v = M.v;
-  const long ov = (long)o * v;
   const int b = e % v;
   const int a = (e / v) % v;
   const int j = (e / ((long)v * v)) % o;
   const int i = e / ((long)o * v * v);
-  const long eT = (((long)j * o + i) * v + b) * v + a;
+  const long ij = ((long)i * o + j) * v * v, ji = ((long)j * o + i) * v * v;
+  const long ab = (long)a * v + b, ba = (long)b * v + a;
   const double* Z = M.Zc;
-  const long ai = (long)a * o + i, bj = (long)b * o + j, aj = (long)a * o + j, bi = (long)b * o + i;
-  double val = M.x[e] + M.x[eT];
-  val -= 0.5 * (Z[ai * ov + bj] + Z[bj * ov + ai]);
-  val -= Z[aj * ov + bi] + Z[bi * ov + aj];
+  double val = M.x[e] + M.x[ji + ba];
+  val -= 0.5 * (Z[ij + ab] + Z[ji + ba]);
+  val -= Z[ji + ab] + Z[ij + ba];
   M.w2[e] += val;
 }
 
@@ -273,16 +281,21 @@ __global__ __launch_bounds__(kEwThreads) void k_contravariant(const Mol* __restr
 }
 
 // per-molecule finalisation of the update: accuracies, convergence, DIIS commit
-__global__ void k_control1(const Mol* __restrict__ mols, State* __restrict__ st, int nm, Params prm,
-                           const int* __restrict__ pre1, const double* __restrict__ part1,
-                           const int* __restrict__ pre2, const double* __restrict__ part2) {
-  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mol >= nm) return;
+// (one workgroup per molecule reduces the partial maxima)
+__global__ __launch_bounds__(kEwThreads) void k_control1(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
+                                                        Params prm, const int* __restrict__ pre1,
+                                                        const double* __restrict__ part1, const int* __restrict__ pre2,
+                                                        const double* __restrict__ part2) {
+  __shared__ double sh[kEwThreads / 64];
+  const int mol = blockIdx.x;
   State& S = st[mol];
   if (!S.active) return;
   double a1 = 0.0, a2 = 0.0;
-  for (int b = pre1[mol]; b < pre1[mol + 1]; ++b) a1 = fmax(a1, part1[b]);
-  for (int b = pre2[mol]; b < pre2[mol + 1]; ++b) a2 = fmax(a2, part2[b]);
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  a1 = block_max(a1, sh);
+  a2 = block_max(a2, sh);
+  if (threadIdx.x != 0) return;
   S.acc[0] = a1;
   S.acc[1] = a2;
   S.iter += 1;
@@ -330,7 +343,7 @@ __global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st,
 __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm, double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
-  const int mol = block_mol(rg.prefix, nm, blockIdx.x);
+  const int mol = rg.bmol[blockIdx.x];
   const int lb = blockIdx.x - rg.prefix[mol];
   const Mol& M = mols[mol];
   const State& S = st[mol];
@@ -355,20 +368,23 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const Mol* __restrict__
 
 // finalise dots into B, build the bordered system, solve it (partial pivoting),
 // scaled by the largest diagonal of B so that it stays accurate near convergence
-__global__ void k_diis_solve(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
-                             const int* __restrict__ prefix, const double* __restrict__ part) {
-  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mol >= nm) return;
+__global__ __launch_bounds__(64) void k_diis_solve(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
+                                                   const int* __restrict__ prefix, const double* __restrict__ part) {
+  const int mol = blockIdx.x;
   const Mol& M = mols[mol];
   State& S = st[mol];
   if (!S.active || !S.staged) return;
   const int m = M.m, cnt = S.diis_count, snew = S.diis_next;
-  for (int k = 0; k < cnt; ++k) {
+  // lane k sums slot k's partials in workgroup order (fixed order: reproducible)
+  if ((int)threadIdx.x < cnt) {
+    const int k = threadIdx.x;
     double s = 0.0;
     for (int b = prefix[mol]; b < prefix[mol + 1]; ++b) s += part[(long)b * m + k];
     M.Bm[k * m + snew] = s;
     M.Bm[snew * m + k] = s;
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   for (int k = 0; k < 64; ++k) S.coef[k] = 0.0;
   if (cnt < 2) { S.coef[snew] = 1.0; return; }
   double A[(kMaxDiis + 1) * (kMaxDiis + 2)];
@@ -416,33 +432,31 @@ __global__ void k_diis_solve(const Mol* __restrict__ mols, State* __restrict__ s
   for (int k = 0; k < cnt; ++k) S.coef[k] = x[k];
 }
 
-// t = sum_k c_k st_k, unpacked with the pair mirror
+// t = sum_k c_k st_k.  Grid: singles, then the (mm, nn) square of excitation
+// pairs; each upper-triangle element reads its packed entry once (coalesced
+// along nn) and writes both mirror elements of t2.
 __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                            Ragged rg, int nm) {
-  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v * (1L + (long)mols[mol].o * mols[mol].v));
   const State& S = st[mol];
   if (e >= size_ || !S.active || !S.staged || S.diis_count < 2) return;
   const int o = M.o, v = M.v;
   const long ne = (long)o * v;
-  const long n2 = (long)o * o * v * v;
-  long p;
-  double* dst;
-  if (e >= n2) {
-    p = e - n2;
-    dst = M.t1 + p;
-  } else {
-    const int b = e % v;
-    const int a = (e / v) % v;
-    const int j = (e / ((long)v * v)) % o;
-    const int i = e / ((long)o * v * v);
-    long mm = (long)i * v + a, nn = (long)j * v + b;
-    if (mm > nn) { const long t = mm; mm = nn; nn = t; }
-    p = ne + packed_index(mm, nn, ne);
-    dst = M.t2 + e;
+  if (e < ne) {
+    double s = 0.0;
+    for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + e];
+    M.t1[e] = s;
+    return;
   }
+  const long f = e - ne;
+  const long mm = f / ne, nn = f % ne;
+  if (mm > nn) return;
+  const long p = ne + packed_index(mm, nn, ne);
   double s = 0.0;
   for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + p];
-  *dst = s;
+  const int i = mm / v, a = mm % v, j = nn / v, b = nn % v;
+  M.t2[(((long)i * o + j) * v + a) * v + b] = s;
+  M.t2[(((long)j * o + i) * v + b) * v + a] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -592,7 +606,8 @@ __global__ void k_activate(State* __restrict__ st, int* __restrict__ act, int nm
 void ew_dress(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_dress, rgNN, mols, st, rgNN, nm);
 }
-void ew_fock_m(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s) {
+void ew_fock_m(const Mol* mols, const State* st, Ragged rgN, Ragged rgNN, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_fock_diag, rgN, mols, st, rgN, nm);
   RAGGED_LAUNCH(k_fock_m, rgNN, mols, st, rgNN, nm);
 }
 void ew_pouter(const Mol* mols, const State* st, Ragged rgN3, int nm, hipStream_t s) {
@@ -617,15 +632,14 @@ void ew_contravariant(const Mol* mols, Ragged rgT2S, int nm, double* d1, double*
 }
 void ctl_after_update(const Mol* mols, State* st, int nm, const Params& prm, Ragged rgOV,
                       const double* part1, Ragged rgT2, const double* part2, hipStream_t s) {
-  hipLaunchKernelGGL(k_control1, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm, prm, rgOV.prefix,
+  hipLaunchKernelGGL(k_control1, dim3(nm), dim3(kEwThreads), 0, s, mols, st, nm, prm, rgOV.prefix,
                      part1, rgT2.prefix, part2);
 }
-void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgT2S, double* part,
+void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part,
                hipStream_t s) {
   RAGGED_LAUNCH(k_diis_dot, rgND, mols, st, rgND, nm, part);
-  hipLaunchKernelGGL(k_diis_solve, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm, rgND.prefix,
-                     part);
-  RAGGED_LAUNCH(k_diis_extrap, rgT2S, mols, st, rgT2S, nm);
+  hipLaunchKernelGGL(k_diis_solve, dim3(nm), dim3(64), 0, s, mols, st, nm, rgND.prefix, part);
+  RAGGED_LAUNCH(k_diis_extrap, rgP2, mols, st, rgP2, nm);
 }
 void ctl_after_diis(const Mol* mols, State* st, int* act, int nm, const Params& prm, const int* stop,
                     hipStream_t s) {

@@ -56,7 +56,35 @@ struct GemmTable {
   GemmProb* d = nullptr;
   int nprob = 0;
   int tiles = 0;
+  int* tile_prob = nullptr;    // problem index of every tile
+  int* red_prefix = nullptr;   // split-K reduction blocks per problem (null: no split)
+  int* red_bprob = nullptr;    // problem index of every reduction block
+  int red_blocks = 0;
+  int tile = 64;               // output tile edge (64 or 32)
+  int flags = 3;               // operand unit-stride flags (a_kfast | b_nfast << 1)
 };
+
+// Split K when a problem has few output tiles and a long K: aim for >= 16
+// work items per problem with K chunks of at least 128.
+static void choose_split(GemmProb& p, int ts) {
+  const int tmn = ((p.M + ts - 1) / ts) * ((p.N + ts - 1) / ts);
+  p.ksplit = 1;
+  p.kchunk = p.K;
+  if (tmn >= 16 || p.K < 512) return;
+  int want = (16 + tmn - 1) / tmn;
+  int maxs = p.K / 128;
+  int ks = std::min(want, std::max(1, maxs));
+  ks = std::min(ks, 64);
+  if (ks <= 1) return;
+  int chunk = (p.K + ks - 1) / ks;
+  chunk = (chunk + 31) / 32 * 32;      // multiple of both K tiles (16 for FP64, 32 for BF16)
+  p.kchunk = chunk;
+  p.ksplit = (p.K + chunk - 1) / chunk;
+  if (p.ksplit <= 1) {
+    p.ksplit = 1;
+    p.kchunk = p.K;
+  }
+}
 
 struct DevBuf {
   double* base = nullptr;
@@ -77,7 +105,7 @@ struct Batch {
   State* dst = nullptr;
   int* dact = nullptr;
   std::vector<void*> allocs;
-  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2;
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   double *d1out = nullptr, *d2out = nullptr;
@@ -117,27 +145,97 @@ static Ragged make_ragged(Batch& b, const std::function<long(int, int, int)>& si
   r.prefix = b.dalloc<int>(pre.size());
   HCHECK(hipMemcpy(r.prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
   r.nblocks = pre.back();
+  std::vector<int> bm(r.nblocks);
+  for (int k = 0; k < b.nm; ++k)
+    for (int q = pre[k]; q < pre[k + 1]; ++q) bm[q] = k;
+  r.bmol = b.dalloc<int>(bm.size());
+  HCHECK(hipMemcpy(r.bmol, bm.data(), bm.size() * sizeof(int), hipMemcpyHostToDevice));
   return r;
 }
 
-static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build) {
+// Tile edge for a table: 32x32 tiles (one wave) only when the padding of
+// 64x64 tiles would cost more than the (measured, about 2.5x) lower per-flop
+// efficiency of the small tile.
+static int choose_tile(const std::vector<GemmProb>& probs) {
+  double c64 = 0.0, c32 = 0.0;
+  for (const auto& p : probs) {
+    const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
+    const double t32 = (double)((p.M + 31) / 32) * ((p.N + 31) / 32) * 32 * 32;
+    c64 += t64 * p.K;
+    c32 += 2.5 * t32 * p.K;
+  }
+  return c32 < c64 ? 32 : 64;
+}
+
+static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true) {
   std::vector<GemmProb> probs;
   int tiles = 0;
+  long ws_total = 0;
+  std::vector<GemmProb> raw;
   for (int k = 0; k < b.nm; ++k) {
     GemmProb p = build(k);
     if (p.M <= 0 || p.N <= 0) continue;
     p.mol = k;
-    p.tiles_n = (p.N + kBN - 1) / kBN;
-    const int tm = (p.M + kBM - 1) / kBM;
+    raw.push_back(p);
+  }
+  const int ts = choose_tile(raw);
+  for (GemmProb p : raw) {
+    p.tiles_n = (p.N + ts - 1) / ts;
+    const int tm = (p.M + ts - 1) / ts;
+    if (allow_split) choose_split(p, ts);
+    else { p.ksplit = 1; p.kchunk = p.K; }
     p.tile_begin = tiles;
-    tiles += tm * p.tiles_n;
+    tiles += tm * p.tiles_n * p.ksplit;
     p.a_kfast = unit_cols(p.A) ? 1 : (unit_rows(p.A) ? 0 : 1);
     p.b_nfast = unit_cols(p.B) ? 1 : (unit_rows(p.B) ? 0 : 1);
+    if (p.ksplit > 1) ws_total += (long)p.ksplit * p.M * p.N;
     probs.push_back(p);
   }
   GemmTable t;
   t.nprob = (int)probs.size();
   t.tiles = tiles;
+  t.tile = ts;
+  if (!probs.empty()) {
+    // the staging layout is a performance choice only (addressing is general):
+    // take the largest problem's unit-stride flags for the whole table
+    size_t big = 0;
+    for (size_t q = 1; q < probs.size(); ++q)
+      if ((double)probs[q].M * probs[q].N * probs[q].K > (double)probs[big].M * probs[big].N * probs[big].K) big = q;
+    t.flags = probs[big].a_kfast | (probs[big].b_nfast << 1);
+  }
+  if (ws_total > 0) {
+    double* ws = b.dalloc<double>(ws_total);
+    long off = 0;
+    std::vector<int> pre(probs.size() + 1, 0);
+    for (size_t q = 0; q < probs.size(); ++q) {
+      GemmProb& p = probs[q];
+      int blocks = 0;
+      if (p.ksplit > 1) {
+        p.ws = ws + off;
+        off += (long)p.ksplit * p.M * p.N;
+        blocks = (int)(((long)p.M * p.N + 255) / 256);
+      }
+      pre[q + 1] = pre[q] + blocks;
+    }
+    t.red_prefix = b.dalloc<int>(pre.size());
+    HCHECK(hipMemcpy(t.red_prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+    t.red_blocks = pre.back();
+    std::vector<int> bp(t.red_blocks);
+    for (size_t q = 0; q < probs.size(); ++q)
+      for (int x = pre[q]; x < pre[q + 1]; ++x) bp[x] = (int)q;
+    t.red_bprob = b.dalloc<int>(std::max<size_t>(1, bp.size()));
+    if (!bp.empty())
+      HCHECK(hipMemcpy(t.red_bprob, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if (tiles > 0) {
+    std::vector<int> tp(tiles);
+    for (size_t q = 0; q < probs.size(); ++q) {
+      const int end = q + 1 < probs.size() ? probs[q + 1].tile_begin : tiles;
+      for (int x = probs[q].tile_begin; x < end; ++x) tp[x] = (int)q;
+    }
+    t.tile_prob = b.dalloc<int>(tp.size());
+    HCHECK(hipMemcpy(t.tile_prob, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   if (t.nprob) {
     t.d = b.dalloc<GemmProb>(probs.size());
     HCHECK(hipMemcpy(t.d, probs.data(), probs.size() * sizeof(GemmProb), hipMemcpyHostToDevice));
@@ -184,6 +282,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   auto gvvov = arena([](int, int o, int v) { return (long)o * v * v * v; });
   auto gooov = arena([](int, int o, int v) { return (long)o * o * o * v; });
   auto Xvv = arena([](int, int, int v) { return (long)v * v; });
+  auto dvec = arena([](int n, int, int) { return 2L * n; });
   auto Xoo = arena([](int, int o, int) { return (long)o * o; });
   auto ndsize = [m](int, int o, int v) {
     long ov = (long)o * v;
@@ -209,7 +308,7 @@ static void allocate(Batch& b, bool keep_mp2) {
     M.t1 = t1[k]; M.t2 = t2[k]; M.u = u[k]; M.w1 = w1[k]; M.w2 = w2[k]; M.x = x[k]; M.Zc = Zc[k];
     M.WC = WC[k]; M.WD = WD[k];
     M.G4 = G4[k]; M.Wkl = Wkl[k]; M.Gklcd = Gklcd[k]; M.Lldkc = Lldkc[k]; M.Lt = Lt[k];
-    M.gvvov = gvvov[k]; M.gooov = gooov[k]; M.Xvv = Xvv[k]; M.Xoo = Xoo[k];
+    M.gvvov = gvvov[k]; M.gooov = gooov[k]; M.Xvv = Xvv[k]; M.Xoo = Xoo[k]; M.dvec = dvec[k];
     M.st = st[k]; M.sd = sd[k]; M.Bm = Bm[k];
     M.t2mp2 = mp2[k];
   }
@@ -226,6 +325,9 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
   b.rgOV = make_ragged(b, sz_ov);
   b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
+  b.rgN = make_ragged(b, [](int n, int, int) { return (long)n; });
+  // DIIS extrapolation grid: singles, then the (mm, nn) square of pair pairs (upper triangle used)
+  b.rgP2 = make_ragged(b, [](int, int o, int v) { return (long)o * v + (long)o * v * o * v; });
   b.rgND = make_ragged(b, [](int, int o, int v) {
     long ov = (long)o * v;
     return ov + ov * (ov + 1) / 2;
@@ -354,10 +456,11 @@ static void build_tables(Batch& b) {
     return prob(cmp(M.t2, o, v, v * v, v, ov * v, 1), cmp(M.Gklcd, v, ov * v, 1, v, v * v, v), rm(M.WC, ov),
                 ov, ov, ov, -0.5, 1.0);
   });
-  // g16: Zc = WC t2[k][j][b][c]
+  // g16: Zc[(a,i),(b,j)] = WC t2[k][j][b][c], stored at [i][j][a][b]
   add([](const Mol& M) {
     const int o = M.o, v = M.v, ov = o * v;
-    return prob(rm(M.WC, ov), cmp(M.t2, v, ov * v, 1, o, v, v * v), rm(M.Zc, ov), ov, ov, ov, 1.0, 0.0);
+    return prob(rm(M.WC, ov), cmp(M.t2, v, ov * v, 1, o, v, v * v), cmp(M.Zc, o, v, ov * v, o, 1, v * v), ov,
+                ov, ov, 1.0, 0.0);
   });
   // g17: WD = 2 g~(ai|kc)
   add([](const Mol& M) {
@@ -432,14 +535,22 @@ struct hcc_ctx {
   // live timing of the contraction GEMM launches (HIP events on `stream`)
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  std::vector<int> prof_ids;
   size_t prof_used = 0;
   double prof_ms = 0.0;
   long prof_launches = 0;
+  double prof_op_ms[64] = {0};
 };
 
 namespace hcc {
 
-static void gemm(hcc_ctx* c, const GemmTable& t) {
+static void launch_table(hcc_ctx* c, const GemmTable& t) {
+  HCHECK(launch_gemm_f64(t.d, t.tile_prob, t.tiles, c->batch->dact, c->stream, t.tile, t.flags));
+  if (t.red_blocks > 0)
+    HCHECK(launch_splitk_reduce(t.d, t.red_prefix, t.red_bprob, t.red_blocks, c->batch->dact, c->stream));
+}
+
+static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
   if (t.nprob == 0) return;
   if (c->prof) {
     if (c->prof_used == c->prof_events.size()) {
@@ -447,14 +558,16 @@ static void gemm(hcc_ctx* c, const GemmTable& t) {
       HCHECK(hipEventCreate(&a));
       HCHECK(hipEventCreate(&b));
       c->prof_events.emplace_back(a, b);
+      c->prof_ids.push_back(0);
     }
+    c->prof_ids[c->prof_used] = id;
     auto& ev = c->prof_events[c->prof_used++];
     HCHECK(hipEventRecord(ev.first, c->stream));
-    HCHECK(launch_gemm_f64(t.d, t.nprob, t.tiles, c->batch->dact, c->stream));
+    launch_table(c, t);
     HCHECK(hipEventRecord(ev.second, c->stream));
     return;
   }
-  HCHECK(launch_gemm_f64(t.d, t.nprob, t.tiles, c->batch->dact, c->stream));
+  launch_table(c, t);
 }
 
 // folds the recorded launch intervals into prof_ms (call after a synchronize)
@@ -463,6 +576,7 @@ static void prof_collect(hcc_ctx* c) {
     float ms = 0.f;
     HCHECK(hipEventElapsedTime(&ms, c->prof_events[q].first, c->prof_events[q].second));
     c->prof_ms += ms;
+    c->prof_op_ms[c->prof_ids[q]] += ms;
   }
   c->prof_launches += (long)c->prof_used;
   c->prof_used = 0;
@@ -489,15 +603,15 @@ static void residual(hcc_ctx* c) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
-  ew_fock_m(b.dm, b.dst, b.rgNN, b.nm, s);
+  ew_fock_m(b.dm, b.dst, b.rgN, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
-  gemm(c, b.it[G1_Q]);
-  gemm(c, b.it[G2A]);
-  gemm(c, b.it[G2B]);
+  gemm(c, b.it[G1_Q], G1_Q);
+  gemm(c, b.it[G2A], G2A);
+  gemm(c, b.it[G2B], G2B);
   ew_u(b.dm, b.dst, b.rgT2, b.nm, s);
-  for (int g = G3; g <= G20; ++g) gemm(c, b.it[g]);
+  for (int g = G3; g <= G20; ++g) gemm(c, b.it[g], g);
   ew_init_xx(b.dm, b.dst, b.rgXX, b.nm, s);
-  for (int g = G21; g <= G24; ++g) gemm(c, b.it[g]);
+  for (int g = G21; g <= G24; ++g) gemm(c, b.it[g], g);
   ew_assemble(b.dm, b.dst, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
 }
@@ -508,7 +622,7 @@ static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
   residual(c);
   ew_update(b.dm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
   ctl_after_update(b.dm, b.dst, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
-  if (prm.diis) diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgT2S, b.partD, s);
+  if (prm.diis) diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
   ctl_after_diis(b.dm, b.dst, b.dact, b.nm, prm, stop, s);
   HCHECK(hipGetLastError());
 }
@@ -521,12 +635,12 @@ static void precompute(hcc_ctx* c) {
   // t1 = 0 -> undressed orbital products
   for (int k = 0; k < b.nm; ++k)
     HCHECK(hipMemsetAsync(b.hm[k].t1, 0, sizeof(double) * b.o[k] * b.v[k], s));
-  gemm(c, b.pre[0]);
-  gemm(c, b.pre[1]);
+  gemm(c, b.pre[0], 60);
+  gemm(c, b.pre[1], 61);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
-  gemm(c, b.it[G1_Q]);
-  gemm(c, b.pre[2]);
+  gemm(c, b.it[G1_Q], 62);
+  gemm(c, b.pre[2], 63);
   make_L(b.dm, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
 }
@@ -882,6 +996,13 @@ int hcc_profile(hcc_ctx* ctx, int enable) {
   ctx->prof_used = 0;
   ctx->prof_ms = 0.0;
   ctx->prof_launches = 0;
+  for (double& x : ctx->prof_op_ms) x = 0.0;
+  return HCC_OK;
+}
+
+int hcc_profile_ops(hcc_ctx* ctx, double* ms64) {
+  if (ctx == nullptr || ms64 == nullptr) return HCC_E_ARG;
+  for (int k = 0; k < 64; ++k) ms64[k] = ctx->prof_op_ms[k];
   return HCC_OK;
 }
 
@@ -938,6 +1059,7 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
   if (precision != HCC_FP64 && precision != HCC_MIXED) return HCC_E_ARG;
   double *dA = nullptr, *dB = nullptr, *dC = nullptr;
   GemmProb* dp = nullptr;
+  int* dtp = nullptr;
   int rc = HCC_OK;
   try {
     HCHECK(hipMalloc(&dA, sizeof(double) * m * k));
@@ -952,10 +1074,14 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
     p.tile_begin = 0;
     p.a_kfast = 1;
     p.b_nfast = 1;
+    p.ksplit = 1;
+    p.kchunk = k;
     const int tiles = ((m + kBM - 1) / kBM) * p.tiles_n;
     HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
-    if (precision == HCC_FP64) HCHECK(launch_gemm_f64(dp, 1, tiles, nullptr, nullptr));
-    else HCHECK(launch_gemm_bf16(dp, 1, tiles, nullptr, nullptr));
+    HCHECK(hipMalloc(&dtp, sizeof(int) * tiles));
+    HCHECK(hipMemset(dtp, 0, sizeof(int) * tiles));
+    if (precision == HCC_FP64) HCHECK(launch_gemm_f64(dp, dtp, tiles, nullptr, nullptr, 64, 3));
+    else HCHECK(launch_gemm_bf16(dp, dtp, tiles, nullptr, nullptr));
     HCHECK(hipDeviceSynchronize());
     HCHECK(hipMemcpy(C, dC, sizeof(double) * m * n, hipMemcpyDeviceToHost));
   } catch (const HipError&) {
@@ -965,6 +1091,7 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
   (void)hipFree(dB);
   (void)hipFree(dC);
   (void)hipFree(dp);
+  (void)hipFree(dtp);
   return rc;
 }
 

@@ -120,6 +120,9 @@ int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total);
 int hcc_profile(hcc_ctx* ctx, int enable);
 int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches);
 int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup);
+/* Per-contraction split of the profiled GEMM time: ms64[k] for GEMM table k
+ * of the iteration (0..N-1 in launch order) and 60..63 for the set-up GEMMs. */
+int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
 void hcc_destroy(hcc_ctx* ctx);
 const char* hcc_last_error(const hcc_ctx* ctx);
