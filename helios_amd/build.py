@@ -61,9 +61,13 @@ def build_fortran():
         return None
     fb = os.path.join(BUILD, "fortran")
     os.makedirs(fb, exist_ok=True)
+    objs = []
+    for f in files:   # module order matters: one file at a time
+        obj = os.path.join(fb, os.path.basename(f).replace(".f90", ".o"))
+        _run([FLANG, "-O2", "-c", "-module-dir", fb, "-I" + fb, f, "-o", obj])
+        objs.append(obj)
     exe = os.path.join(HERE, "helios_gpu")
-    _run([FLANG, "-O2", "-module-dir", fb, "-o", exe] + files +
-         ["-L" + HERE, "-lhelios_ccsd", "-Wl,-rpath," + HERE])
+    _run([FLANG, "-O2", "-o", exe] + objs + ["-L" + HERE, "-lhelios_ccsd", "-Wl,-rpath,$ORIGIN"])
     return exe
 
 

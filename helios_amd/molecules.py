@@ -178,3 +178,19 @@ def batch_member(k, base_seed=3, nmin=10, nmax=48):
     m = random_fragment(n, rng=rng)
     m.name = f"frag-{base_seed + k}-{n}"
     return m
+
+
+def to_input(mol, method="r-ccsd", cc_accuracy=10, scf_accuracy=12, extra=""):
+    """Input file text in the reference's block syntax (src/input/definebd.f90,
+    molecule layout src/input/definebdDescription.f90:218-242)."""
+    lines = [f"%general{{method={method}, jobtype=energy}}",
+             "%parametrization{coulomb-integrals=ohno-klopman}",
+             f"%scf{{accuracy={scf_accuracy}}}",
+             f"%coupled-cluster{{accuracy={cc_accuracy}{', ' + extra if extra else ''}}}",
+             "%molecule{", mol.name, str(mol.n), str(len(mol.bonds)), str(mol.n // 2), str(mol.n // 2)]
+    for x, y, z in mol.coords:
+        lines.append(f"C {x:.12f} {y:.12f} {z:.12f} {mol.alpha} {mol.U} 1")
+    for i, j, b in mol.bonds:
+        lines.append(f"{i + 1} {j + 1} {b} nonset")
+    lines.append("}")
+    return "\n".join(lines) + "\n"

@@ -1,0 +1,51 @@
+"""The Fortran host (helios_amd/helios_gpu): input blocks, PPP model, RHF on
+the CPU; r-ccsd through the iso_c_binding module on the GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helios_amd import molecules as M
+from oracle import ccsd
+from tests._molprep import prepare
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "helios_amd", "helios_gpu")
+
+
+def run(tmp_path, mol, *args, extra=""):
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran host not built")
+    inp = tmp_path / f"{mol.name}.inp"
+    inp.write_text(M.to_input(mol, extra=extra))
+    r = subprocess.run([EXE, "-f", str(inp), *args], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def energy_of(out, label="Resulting energy"):
+    return float(re.search(label + r"\s+(-?\d+\.\d+)", out).group(1))
+
+
+@pytest.mark.parametrize("mol", [M.octatetraene(), M.batch_member(9), M.polyene(20)])
+def test_host_scf_matches_oracle(tmp_path, mol):
+    out = run(tmp_path, mol, "-scf")
+    p = prepare(mol)
+    assert "rhf status 1" in out
+    assert abs(energy_of(out) - p["eref"]) < 1e-9
+    eps = np.array([float(x) for x in out.split("Orbital energies (eV):")[1].split()])
+    assert np.abs(np.sort(eps) - p["E"]).max() < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("driver", ["host", "device"])
+def test_host_rccsd_on_gpu(tmp_path, driver):
+    mol = M.octatetraene()
+    out = run(tmp_path, mol, extra=f"driver={driver}, diis-steps=20")
+    p = prepare(mol)
+    ref = ccsd.solve(p["C"], p["F"], p["G"], mol.nocc, p["eref"], eps=1e-10, diis_scale=True)
+    print(out[-600:])
+    assert "r-ccsd status 1" in out
+    assert abs(energy_of(out) - ref["energy"][0]) < 1e-8
+    assert abs(energy_of(out, "Reference energy") - p["eref"]) < 1e-9
