@@ -26,6 +26,8 @@ _vp = ctypes.c_void_p
 SIGNATURES = {
     "hcc_create": (_i, [_P(_vp), _i, _i, _i]),
     "hcc_set_options": (_i, [_vp, _d, _d, _i, _i]),
+    "hcc_set_mixed": (_i, [_vp, _i, _d, _i, _i]),
+    "hcc_batch_statistics": (_i, [_vp, _P(_i), _P(_i)]),
     "hcc_set_molecule": (_i, [_vp, _i, _i, _P(_d), _P(_d), _P(_d), _d]),
     "hcc_guess": (_i, [_vp, _i]),
     "hcc_iterate": (_i, [_vp, _i, _d, _P(_d)]),
@@ -112,6 +114,15 @@ class Context:
     def set_options(self, t1_step=1.0, t2_step=1.0, projection_type=2, diis=True):
         self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
                     "hcc_set_options")
+
+    def set_mixed(self, bf16=True, kappa=0.03, max_inner=40, max_outer=12):
+        self._check(self.lib.hcc_set_mixed(self.h, int(bf16), kappa, max_inner, max_outer), "hcc_set_mixed")
+
+    def batch_statistics(self):
+        outer = np.zeros(self.nmol, dtype=np.int32)
+        inner = np.zeros(self.nmol, dtype=np.int32)
+        self._check(self.lib.hcc_batch_statistics(self.h, _ip(outer), _ip(inner)), "hcc_batch_statistics")
+        return outer, inner
 
     def set_molecule(self, C, F, G, nocc, eref):
         n = C.shape[0]

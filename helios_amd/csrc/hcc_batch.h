@@ -37,7 +37,10 @@ struct State {
   int diis_count;    // vectors in history
   int diis_next;     // next ring slot
   int staged;        // 1 when the current iteration staged a DIIS vector
-  int pad;
+  int inner;         // mixed solver: 1 while the inner (correction) solve runs
+  int inner_its;     // mixed solver: inner iterations of the current outer step
+  int outer_its;     // mixed solver: FP64 residual evaluations
+  double acc0;       // mixed solver: max|d| of the FP64 residual at the outer point
   double acc[2];     // max|d1|, max|d2|
   double cur, prev;  // current / previous max accuracy
   double energy[5];
@@ -46,6 +49,9 @@ struct State {
 
 struct Params {
   double eps;            // convergence threshold on max(acc)
+  double kappa;          // mixed solver: inner solve stops at acc < kappa * acc0
+  int max_inner;         // mixed solver: inner iterations per outer step
+  int pad_;
   double divergence;     // divergence threshold
   double s1, s2;         // t1/t2 steps
   int projtype;          // 1 or 2 (the reference's '2-1')

@@ -373,6 +373,310 @@ hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int tot
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Correction-form ("dual") GEMM of the mixed-precision inner iterations:
+//   C = alpha * (A0 dB + dA (B0 + dB)) + beta * C
+// which is the change of A B between the outer point (A0, B0) and the current
+// point (A0 + dA, B0 + dB).  The differences and the full second factor are
+// formed in FP64 before rounding, so with BF16 operands the rounding error is
+// proportional to the size of the correction, not of A B.  64x64 tiles, 256
+// threads; FP64 path: BK 16 and 16x16x4 MFMA; BF16 path: BK 32 and
+// 16x16x32 MFMA with [row][k] LDS staging.
+// ---------------------------------------------------------------------------
+template <bool AKF, bool BNF, bool BF>
+struct DualCfg {
+  static constexpr int BK = BF ? 32 : 16;
+  static constexpr int T = 256;
+  static constexpr int NE = 64 * BK / T;     // elements per operand per thread per slab
+};
+
+template <bool AKF, bool BNF>
+__global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict__ probs,
+                                                     const int* __restrict__ tile_prob, int total_tiles,
+                                                     const int* __restrict__ active) {
+  constexpr int BM = 64, BN = 64, T = 256, BK = 16, NE = 4;
+  constexpr int kPa = BM + 2, kPb = BN + 2;
+  __shared__ double A0s[BM * kPk > BK * kPa ? BM * kPk : BK * kPa];
+  __shared__ double dAs[BM * kPk > BK * kPa ? BM * kPk : BK * kPa];
+  __shared__ double dBs[BN * kPk > BK * kPb ? BN * kPk : BK * kPb];
+  __shared__ double Bfs[BN * kPk > BK * kPb ? BN * kPk : BK * kPb];
+  const int tile = logical_tile(total_tiles);
+  if (tile >= total_tiles) return;
+  const GemmProb P = probs[find_problem(tile_prob, tile)];
+  if (active != nullptr && active[P.mol] == 0) return;
+  const int tiles_n = (P.N + BN - 1) / BN;
+  const int tiles_mn = ((P.M + BM - 1) / BM) * tiles_n;
+  int lt = tile - P.tile_begin;
+  const int ks = lt / tiles_mn;
+  lt -= ks * tiles_mn;
+  const int m0 = (lt / tiles_n) * BM, n0 = (lt % tiles_n) * BN;
+  const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const bool hasA = P.dA != nullptr, hasB = P.dB != nullptr;
+  int fa[NE], fb[NE], kqa[NE], kqb[NE];
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int q = t + i * T;
+    const int m = m0 + (AKF ? q / BK : q % BM);
+    kqa[i] = AKF ? q % BK : q / BM;
+    fa[i] = m < P.M ? row_off(P.A, m) : -1;
+    const int n = n0 + (BNF ? q % BN : q / BK);
+    kqb[i] = BNF ? q / BN : q % BK;
+    fb[i] = n < P.N ? col_off(P.B, n) : -1;
+  }
+  const double* __restrict__ A0 = P.A.p;
+  const double* __restrict__ dA = hasA ? P.dA : P.A.p;
+  const double* __restrict__ B0 = P.B.p;
+  const double* __restrict__ dB = hasB ? P.dB : P.B.p;
+  double ra0[NE], rda[NE], rdb[NE], rbf[NE];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      int k = k0 + kqa[i];
+      bool ok = fa[i] >= 0 && k < kend;
+      int o = ok ? fa[i] + col_off(P.A, k) : 0;
+      const double a0 = A0[o], da = dA[o];
+      ra0[i] = ok ? a0 : 0.0;
+      rda[i] = (ok && hasA) ? da : 0.0;
+      k = k0 + kqb[i];
+      ok = fb[i] >= 0 && k < kend;
+      o = ok ? fb[i] + row_off(P.B, k) : 0;
+      const double b0 = B0[o], db = dB[o];
+      rdb[i] = (ok && hasB) ? db : 0.0;
+      rbf[i] = ok ? b0 + rdb[i] : 0.0;
+    }
+  };
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int q = t + i * T;
+      const int ia = AKF ? (q / BK) * kPk + q % BK : (q / BM) * kPa + q % BM;
+      const int ib = BNF ? (q / BN) * kPb + q % BN : (q / BK) * kPk + q % BK;
+      A0s[ia] = ra0[i];
+      dAs[ia] = rda[i];
+      dBs[ib] = rdb[i];
+      Bfs[ib] = rbf[i];
+    }
+    __syncthreads();
+    load(min(k0 + BK, kend));
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const int ma = wm + (lane & 15), nb = wn + (lane & 15);
+      const int ia0 = AKF ? ma * kPk + kr : kr * kPa + ma;
+      const int ia1 = AKF ? (ma + 16) * kPk + kr : kr * kPa + ma + 16;
+      const int ib0 = BNF ? kr * kPb + nb : nb * kPk + kr;
+      const int ib1 = BNF ? kr * kPb + nb + 16 : (nb + 16) * kPk + kr;
+      const double a0 = A0s[ia0], a1 = A0s[ia1], e0 = dAs[ia0], e1 = dAs[ia1];
+      const double b0 = dBs[ib0], b1 = dBs[ib1], f0 = Bfs[ib0], f1 = Bfs[ib1];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(e0, f0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(e0, f1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(e1, f0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(e1, f1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (P.ksplit > 1) {
+    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 16 * j + (lane & 15);
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+          if (row < P.M) W[(long)row * P.N + col] = acc[i][j][r];
+        }
+    }
+    return;
+  }
+  double* __restrict__ Cp = P.C.p;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn + 16 * j + (lane & 15);
+    if (col >= P.N) continue;
+    const int co = col_off(P.C, col);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+        if (row >= P.M) continue;
+        const int o = row_off(P.C, row) + co;
+        double v = P.alpha * acc[i][j][r];
+        if (P.beta != 0.0) v += P.beta * Cp[o];
+        Cp[o] = v;
+      }
+  }
+}
+
+template <bool AKF, bool BNF>
+__global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict__ probs,
+                                                      const int* __restrict__ tile_prob, int total_tiles,
+                                                      const int* __restrict__ active) {
+  constexpr int BM = 64, BN = 64, T = 256, BK = 32, NE = 8, LD = BK + 8;
+  __shared__ __attribute__((aligned(16))) __bf16 A0s[BM * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 dAs[BM * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 dBs[BN * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bfs[BN * LD];
+  const int tile = logical_tile(total_tiles);
+  if (tile >= total_tiles) return;
+  const GemmProb P = probs[find_problem(tile_prob, tile)];
+  if (active != nullptr && active[P.mol] == 0) return;
+  const int tiles_n = (P.N + BN - 1) / BN;
+  const int tiles_mn = ((P.M + BM - 1) / BM) * tiles_n;
+  int lt = tile - P.tile_begin;
+  const int ks = lt / tiles_mn;
+  lt -= ks * tiles_mn;
+  const int m0 = (lt / tiles_n) * BM, n0 = (lt % tiles_n) * BN;
+  const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const bool hasA = P.dA != nullptr, hasB = P.dB != nullptr;
+  int fa[NE], fb[NE], kqa[NE], kqb[NE], sa[NE], sb[NE];
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int q = t + i * T;
+    const int ml = AKF ? q / BK : q % BM;
+    kqa[i] = AKF ? q % BK : q / BM;
+    fa[i] = m0 + ml < P.M ? row_off(P.A, m0 + ml) : -1;
+    sa[i] = ml * LD + kqa[i];
+    const int nl = BNF ? q % BN : q / BK;
+    kqb[i] = BNF ? q / BN : q % BK;
+    fb[i] = n0 + nl < P.N ? col_off(P.B, n0 + nl) : -1;
+    sb[i] = nl * LD + kqb[i];
+  }
+  const double* __restrict__ A0 = P.A.p;
+  const double* __restrict__ dA = hasA ? P.dA : P.A.p;
+  const double* __restrict__ B0 = P.B.p;
+  const double* __restrict__ dB = hasB ? P.dB : P.B.p;
+  float ra0[NE], rda[NE], rdb[NE], rbf[NE];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      int k = k0 + kqa[i];
+      bool ok = fa[i] >= 0 && k < kend;
+      int o = ok ? fa[i] + col_off(P.A, k) : 0;
+      const double a0 = A0[o], da = dA[o];
+      ra0[i] = ok ? (float)a0 : 0.f;
+      rda[i] = (ok && hasA) ? (float)da : 0.f;
+      k = k0 + kqb[i];
+      ok = fb[i] >= 0 && k < kend;
+      o = ok ? fb[i] + row_off(P.B, k) : 0;
+      const double b0 = B0[o], db = dB[o];
+      const double dbv = (ok && hasB) ? db : 0.0;
+      rdb[i] = (float)dbv;
+      rbf[i] = ok ? (float)(b0 + dbv) : 0.f;
+    }
+  };
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      A0s[sa[i]] = (__bf16)ra0[i];
+      dAs[sa[i]] = (__bf16)rda[i];
+      dBs[sb[i]] = (__bf16)rdb[i];
+      Bfs[sb[i]] = (__bf16)rbf[i];
+    }
+    __syncthreads();
+    load(min(k0 + BK, kend));
+    const int kq = 8 * (lane >> 4);
+    const int ra = (wm + (lane & 15)) * LD + kq, rb = (wn + (lane & 15)) * LD + kq;
+    const bf8 a0 = *reinterpret_cast<const bf8*>(&A0s[ra]);
+    const bf8 a1 = *reinterpret_cast<const bf8*>(&A0s[ra + 16 * LD]);
+    const bf8 e0 = *reinterpret_cast<const bf8*>(&dAs[ra]);
+    const bf8 e1 = *reinterpret_cast<const bf8*>(&dAs[ra + 16 * LD]);
+    const bf8 b0 = *reinterpret_cast<const bf8*>(&dBs[rb]);
+    const bf8 b1 = *reinterpret_cast<const bf8*>(&dBs[rb + 16 * LD]);
+    const bf8 f0 = *reinterpret_cast<const bf8*>(&Bfs[rb]);
+    const bf8 f1 = *reinterpret_cast<const bf8*>(&Bfs[rb + 16 * LD]);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f1, acc[1][1], 0, 0, 0);
+    __syncthreads();
+  }
+  if (P.ksplit > 1) {
+    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 16 * j + (lane & 15);
+      if (col >= P.N) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+          if (row < P.M) W[(long)row * P.N + col] = (double)acc[i][j][r];
+        }
+    }
+    return;
+  }
+  double* __restrict__ Cp = P.C.p;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn + 16 * j + (lane & 15);
+    if (col >= P.N) continue;
+    const int co = col_off(P.C, col);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        if (row >= P.M) continue;
+        const int o = row_off(P.C, row) + co;
+        double v = P.alpha * (double)acc[i][j][r];
+        if (P.beta != 0.0) v += P.beta * Cp[o];
+        Cp[o] = v;
+      }
+  }
+}
+
+hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tiles, const int* active,
+                            hipStream_t s, int flags, int bf16) {
+  if (total_tiles <= 0) return hipSuccess;
+  const int grid = ((total_tiles + 7) / 8) * 8;
+  if (bf16) {
+    switch (flags & 3) {
+      case 0: hipLaunchKernelGGL((gemm_dual_bf16<false, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      case 1: hipLaunchKernelGGL((gemm_dual_bf16<true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      case 2: hipLaunchKernelGGL((gemm_dual_bf16<false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      default: hipLaunchKernelGGL((gemm_dual_bf16<true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+    }
+  } else {
+    switch (flags & 3) {
+      case 0: hipLaunchKernelGGL((gemm_dual_f64<false, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      case 1: hipLaunchKernelGGL((gemm_dual_f64<true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      case 2: hipLaunchKernelGGL((gemm_dual_f64<false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      default: hipLaunchKernelGGL((gemm_dual_f64<true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+    }
+  }
+  return hipGetLastError();
+}
+
 // fixed-order sum of the split-K partials into C
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmProb* __restrict__ probs,
                                                             const int* __restrict__ prefix,

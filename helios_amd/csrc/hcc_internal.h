@@ -36,6 +36,11 @@ struct GemmProb {
   int ksplit;       // number of K chunks (1: no split)
   int kchunk;       // K range of one chunk (multiple of the K tile)
   double* ws;       // split-K partials: ksplit x M x N, row-major (ksplit > 1)
+  // correction ("delta") form, used by the mixed-precision inner iterations:
+  // C += alpha * (A0 dB + dA (B0 + dB)) with A0 = A.p, B0 = B.p; a null dA or
+  // dB marks an operand that does not change with the amplitudes
+  double* dA;
+  double* dB;
 };
 
 // Tile shape of the FP64 MFMA GEMM.
@@ -50,6 +55,10 @@ hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int tota
                            const int* active, hipStream_t s, int tile = 64, int flags = 3);
 hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int total_tiles,
                             const int* active, hipStream_t s);
+// correction-form GEMM (see GemmProb::dA/dB); bf16 = 1 rounds the operands to
+// BF16 for V_MFMA_F32_16X16X32_BF16 (FP32 accumulation), 0 keeps FP64
+hipError_t launch_gemm_dual(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
+                            hipStream_t s, int flags, int bf16);
 // C = alpha * sum_k ws_k + beta * C for the split problems of a table;
 // prefix: block prefix over the table's problems (256 elements per block),
 // block_prob: problem index of every reduction block

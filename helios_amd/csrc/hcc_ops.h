@@ -29,4 +29,18 @@ void guess(const Mol* mols, State* st, int* act, Ragged rgT2S, int nm, int kind,
 void reset_state(State* st, int* act, int nm, hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 
+// mixed-precision solver (correction form)
+void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
+                 hipStream_t s);
+void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
+void mixed_commit(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
+void mixed_update(const Mol* m0, const Mol* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
+                  double* part1, double* part2, hipStream_t s);
+void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, int max_outer, Ragged rgOV,
+                         const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
+void mixed_inner_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
+                         const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
+void mixed_inner_advance(const Mol* mols, State* st, int nm, hipStream_t s);
+void set_phase(State* st, int* act, int nm, int phase, hipStream_t s);
+
 }  // namespace hcc

@@ -582,6 +582,10 @@ __global__ void k_reset_state(State* __restrict__ st, int* __restrict__ act, int
   S.acc[0] = S.acc[1] = -1.0;
   S.cur = 0.0;
   S.prev = 10.0;
+  S.inner = 0;
+  S.inner_its = 0;
+  S.outer_its = 0;
+  S.acc0 = 0.0;
   for (int k = 0; k < 5; ++k) S.energy[k] = 0.0;
   act[mol] = 1;
 }
@@ -662,6 +666,288 @@ void reset_state(State* st, int* act, int nm, hipStream_t s) {
 }
 void activate(State* st, int* act, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_activate, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solver: correction ("delta") forms of the elementwise steps.
+// M0 holds the outer-point buffers, Md the corrections; every step is the
+// exact change of its plain counterpart, linear (dressing, dressed density,
+// Fock matrix) or bilinear (orbital products).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_dress_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                       const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const Mol& D = dmols[mol];
+  const int N = M.n, o = M.o, v = M.v;
+  const int mu = e / N, p = e % N;
+  const double* C = M.C + (long)mu * N;
+  double dx = 0.0, dy = 0.0;
+  if (p < o) {
+    for (int a = 0; a < v; ++a) dy += C[o + a] * D.t1[p * v + a];
+  } else {
+    const int a = p - o;
+    for (int k = 0; k < o; ++k) dx -= C[k] * D.t1[k * v + a];
+  }
+  D.Xf[e] = dx;
+  D.Yf[e] = dy;
+}
+
+__device__ __forceinline__ double dressed_density_d(const Mol& M, const Mol& D, int mu, int nu) {
+  const double* dY = D.Yf + (long)mu * M.n;
+  const double* Cn = M.C + (long)nu * M.n;
+  double s = 0.0;
+  for (int k = 0; k < M.o; ++k) s += dY[k] * Cn[k];
+  return s;
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_fock_diag_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                           const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  dmols[mol].dvec[e] = dressed_density_d(M, dmols[mol], (int)e, (int)e);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                        const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const Mol& D = dmols[mol];
+  const int N = M.n;
+  const int mu = e / N, nu = e % N;
+  double val = -M.G[e] * dressed_density_d(M, D, mu, nu);
+  if (mu == nu) {
+    double s = 0.0;
+    for (int l = 0; l < N; ++l) s += M.G[(long)mu * N + l] * D.dvec[l];
+    val += 2.0 * s;
+  }
+  D.Mf[e] = val;
+}
+
+// dP = X0 (x) dY + dX (x) (Y0 + dY)
+__global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                        const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
+  if (e >= size_ || !st[mol].active) return;
+  const Mol& D = dmols[mol];
+  const long N = M.n, N2 = N * N;
+  const long mu = e / N2, pq = e % N2;
+  const long p = pq / N, q = pq % N;
+  const long ip = mu * N + p, iq = mu * N + q;
+  D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + D.Yf[iq]);
+}
+
+// t (x) 0 for the corrections of active molecules
+__global__ __launch_bounds__(kEwThreads) void k_zero_delta(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                          const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  if (e >= size_ || !st[mol].active) return;
+  const long n2 = (long)M.o * M.o * M.v * M.v;
+  if (e < n2) dmols[mol].t2[e] = 0.0;
+  else dmols[mol].t1[e - n2] = 0.0;
+}
+
+// t0 += dt
+__global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                            const State* __restrict__ st, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
+  if (e >= size_ || st[mol].status != 0) return;
+  const long n2 = (long)M.o * M.o * M.v * M.v;
+  if (e < n2) M.t2[e] += dmols[mol].t2[e];
+  else M.t1[e - n2] += dmols[mol].t1[e - n2];
+}
+
+// mixed update, singles: Omega1 = (w1_0 incl. F~_ai) + dw1 + dF~_ai; T = t1_0 + dt1
+__global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                             const State* __restrict__ st, Ragged rg, int nm, Params prm,
+                                                             double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v);
+  if (!st[mol].active) return;
+  double amax = 0.0;
+  if (e < size_) {
+    const Mol& D = dmols[mol];
+    const int N = M.n, o = M.o, v = M.v;
+    const int i = e / v, a = e % v;
+    const double d1 = M.w1[e] + D.w1[e] + D.Ft[(long)(o + a) * N + i];
+    const double Dn = M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a];
+    const double tn = M.t1[e] + D.t1[e] + prm.s1 * d1 / Dn;
+    D.t1[e] = tn - M.t1[e];
+    amax = fabs(d1);
+    if (prm.diis) {
+      const State& S = st[mol];
+      M.sd[(long)S.diis_next * M.nd + e] = d1;
+      M.st[(long)S.diis_next * M.nd + e] = D.t1[e];
+    }
+  }
+  const double r = block_max(amax, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// mixed update, doubles (pair representatives as in k_update2); DIIS stores corrections
+__global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+                                                             const State* __restrict__ st, Ragged rg, int nm, Params prm,
+                                                             double* __restrict__ part) {
+  __shared__ double sh[kEwThreads / 64];
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
+  if (!st[mol].active) return;
+  double amax = 0.0;
+  if (e < size_) {
+    const Mol& Dm = dmols[mol];
+    const int N = M.n, o = M.o, v = M.v;
+    const long ne = (long)o * v;
+    const int b = e % v;
+    const int a = (e / v) % v;
+    const int j = (e / ((long)v * v)) % o;
+    const int i = e / ((long)o * v * v);
+    const long mm = (long)i * v + a, nn = (long)j * v + b;
+    int I = i, J = j, A = a, B = b;
+    if (mm > nn) { I = j; J = i; A = b; B = a; }
+    const long r = (((long)I * o + J) * v + A) * v + B;
+    const long mb = (long)I * v + B, na = (long)J * v + A;
+    const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
+    const double om = M.w2[r] + Dm.w2[r];
+    const double d2 = 2.0 * om - (M.w2[rx] + Dm.w2[rx]);
+    amax = fabs(d2);
+    const double Dn = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
+                      M.F[(long)(o + B) * N + o + B];
+    const double num = prm.projtype == 2 ? 3.0 * om : d2;
+    if (mm <= nn) {
+      const double tn = M.t2[e] + Dm.t2[e] + prm.s2 * num / Dn;
+      const double dt = tn - M.t2[e];
+      Dm.t2[e] = dt;
+      Dm.t2[(((long)j * o + i) * v + b) * v + a] = dt;
+      if (prm.diis) {
+        const State& S = st[mol];
+        const long p = ne + packed_index(mm, nn, ne);
+        M.sd[(long)S.diis_next * M.nd + p] = num;
+        M.st[(long)S.diis_next * M.nd + p] = dt;
+      }
+    }
+  }
+  const double rr = block_max(amax, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// after the FP64 residual at the outer point: acc0, convergence, inner set-up
+__global__ __launch_bounds__(kEwThreads) void k_outer_control(const Mol* __restrict__ mols, State* __restrict__ st,
+                                                             int* __restrict__ act, int nm, Params prm, int max_outer,
+                                                             const int* __restrict__ pre1, const double* __restrict__ part1,
+                                                             const int* __restrict__ pre2, const double* __restrict__ part2) {
+  __shared__ double sh[kEwThreads / 64];
+  const int mol = blockIdx.x;
+  State& S = st[mol];
+  if (S.status != 0) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  a1 = block_max(a1, sh);
+  a2 = block_max(a2, sh);
+  if (threadIdx.x != 0) return;
+  S.acc[0] = a1;
+  S.acc[1] = a2;
+  S.acc0 = fmax(a1, a2);
+  S.cur = S.acc0;
+  S.iter += 1;
+  S.outer_its += 1;
+  S.inner = 0;
+  S.inner_its = 0;
+  S.diis_count = 0;
+  S.diis_next = 0;
+  S.staged = 0;
+  if (S.acc0 < prm.eps) { S.status = 1; act[mol] = 0; return; }
+  if (!(S.acc0 < prm.divergence)) { S.status = -2; act[mol] = 0; return; }
+  if (S.outer_its >= max_outer || S.iter + 1 > prm.maxiter) { S.status = -1; act[mol] = 0; return; }
+  S.inner = 1;
+}
+
+// inner-iteration control: stop at acc < kappa*acc0 (or below eps/2, or at the cap)
+__global__ __launch_bounds__(kEwThreads) void k_inner_control(const Mol* __restrict__ mols, State* __restrict__ st,
+                                                             int* __restrict__ act, int nm, Params prm,
+                                                             const int* __restrict__ pre1, const double* __restrict__ part1,
+                                                             const int* __restrict__ pre2, const double* __restrict__ part2) {
+  __shared__ double sh[kEwThreads / 64];
+  const int mol = blockIdx.x;
+  State& S = st[mol];
+  if (!S.active) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  a1 = block_max(a1, sh);
+  a2 = block_max(a2, sh);
+  if (threadIdx.x != 0) return;
+  S.acc[0] = a1;
+  S.acc[1] = a2;
+  S.cur = fmax(a1, a2);
+  S.iter += 1;
+  S.inner_its += 1;
+  S.staged = 0;
+  const bool done = S.cur < prm.kappa * S.acc0 || S.cur < 0.5 * prm.eps || S.inner_its >= prm.max_inner ||
+                    !(S.cur == S.cur);
+  if (done) {
+    S.inner = 0;
+    S.active = 0;
+    act[mol] = 0;
+    return;
+  }
+  if (prm.diis) {
+    const int m = mols[mol].m;
+    S.staged = 1;
+    S.diis_count = S.diis_count < m ? S.diis_count + 1 : m;
+  }
+}
+
+// DIIS slot advance for the inner phase (k_control2 without the iterator tests)
+__global__ void k_inner_advance(const Mol* __restrict__ mols, State* __restrict__ st, int nm) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  if (S.active && S.staged) S.diis_next = (S.diis_next + 1) % mols[mol].m;
+}
+
+// phase masks: 0 = outer (every running molecule), 1 = inner (molecules still solving)
+__global__ void k_set_phase(State* __restrict__ st, int* __restrict__ act, int nm, int phase) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  const int a = S.status == 0 && (phase == 0 || S.inner);
+  S.active = a;
+  act[mol] = a;
+}
+
+void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
+                 hipStream_t s) {
+  RAGGED_LAUNCH(k_dress_d, rgNN, m0, md, st, rgNN, nm);
+  RAGGED_LAUNCH(k_fock_diag_d, rgN, m0, md, st, rgN, nm);
+  RAGGED_LAUNCH(k_fock_m_d, rgNN, m0, md, st, rgNN, nm);
+  RAGGED_LAUNCH(k_pouter_d, rgN3, m0, md, st, rgN3, nm);
+}
+void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_zero_delta, rgT2S, m0, md, st, rgT2S, nm);
+}
+void mixed_commit(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_commit_delta, rgT2S, m0, md, st, rgT2S, nm);
+}
+void mixed_update(const Mol* m0, const Mol* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
+                  double* part1, double* part2, hipStream_t s) {
+  RAGGED_LAUNCH(k_update1_mixed, rgOV, m0, md, st, rgOV, nm, prm, part1);
+  RAGGED_LAUNCH(k_update2_mixed, rgT2, m0, md, st, rgT2, nm, prm, part2);
+}
+void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, int max_outer, Ragged rgOV,
+                         const double* part1, Ragged rgT2, const double* part2, hipStream_t s) {
+  hipLaunchKernelGGL(k_outer_control, dim3(nm), dim3(kEwThreads), 0, s, mols, st, act, nm, prm, max_outer,
+                     rgOV.prefix, part1, rgT2.prefix, part2);
+}
+void mixed_inner_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
+                         const double* part1, Ragged rgT2, const double* part2, hipStream_t s) {
+  hipLaunchKernelGGL(k_inner_control, dim3(nm), dim3(kEwThreads), 0, s, mols, st, act, nm, prm, rgOV.prefix,
+                     part1, rgT2.prefix, part2);
+}
+void mixed_inner_advance(const Mol* mols, State* st, int nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_inner_advance, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm);
+}
+void set_phase(State* st, int* act, int nm, int phase, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_phase, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm, phase);
 }
 
 }  // namespace hcc

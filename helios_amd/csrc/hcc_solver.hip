@@ -109,6 +109,12 @@ struct Batch {
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   double *d1out = nullptr, *d2out = nullptr;
+  // mixed precision: correction buffers and correction-form GEMM tables
+  bool mixed = false;
+  std::vector<Mol> hd;
+  Mol* ddm = nullptr;
+  std::vector<std::function<GemmProb(const Mol&)>> ops;
+  std::vector<GemmTable> dit;
 
   ~Batch() { release(); }
   void release() {
@@ -167,7 +173,8 @@ static int choose_tile(const std::vector<GemmProb>& probs) {
   return c32 < c64 ? 32 : 64;
 }
 
-static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true) {
+static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
+                            int force_tile = 0) {
   std::vector<GemmProb> probs;
   int tiles = 0;
   long ws_total = 0;
@@ -178,7 +185,7 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
     p.mol = k;
     raw.push_back(p);
   }
-  const int ts = choose_tile(raw);
+  const int ts = force_tile ? force_tile : choose_tile(raw);
   for (GemmProb p : raw) {
     p.tiles_n = (p.N + ts - 1) / ts;
     const int tm = (p.M + ts - 1) / ts;
@@ -256,6 +263,43 @@ static GemmProb prob(Operand A, Operand B, Operand C, int M, int N, int K, doubl
   return p;
 }
 
+// Buffers that change with the amplitudes (the ones that get a correction twin
+// in the mixed-precision solver), with their per-molecule sizes.
+struct FieldDef {
+  double* Mol::*member;
+  std::function<long(int, int, int)> size;
+};
+
+static const std::vector<FieldDef>& variable_fields() {
+  static const std::vector<FieldDef> f = {
+      {&Mol::Xf, sz_nn}, {&Mol::Yf, sz_nn}, {&Mol::Mf, sz_nn}, {&Mol::Ft, sz_nn}, {&Mol::tmp, sz_nn},
+      {&Mol::P, sz_n3}, {&Mol::Q, sz_n3}, {&Mol::t1, sz_ov}, {&Mol::w1, sz_ov}, {&Mol::t2, sz_t2},
+      {&Mol::u, sz_t2}, {&Mol::w2, sz_t2}, {&Mol::x, sz_t2}, {&Mol::Zc, sz_t2}, {&Mol::WC, sz_t2},
+      {&Mol::WD, sz_t2},
+      {&Mol::G4, [](int, int, int v) { return (long)v * v * v * v; }},
+      {&Mol::Wkl, [](int, int o, int) { return (long)o * o * o * o; }},
+      {&Mol::gvvov, [](int, int o, int v) { return (long)o * v * v * v; }},
+      {&Mol::gooov, [](int, int o, int v) { return (long)o * o * o * v; }},
+      {&Mol::Xvv, [](int, int, int v) { return (long)v * v; }},
+      {&Mol::Xoo, [](int, int o, int) { return (long)o * o; }},
+      {&Mol::dvec, [](int n, int, int) { return 2L * n; }},
+  };
+  return f;
+}
+
+// Correction twin of an operand pointer of molecule k, or null for a constant
+// input (MO coefficients, gamma, Fock, undressed integrals).
+static double* delta_of(const Batch& b, int k, double* p) {
+  const Mol& H = b.hm[k];
+  const Mol& D = b.hd[k];
+  for (const auto& f : variable_fields()) {
+    double* base = H.*(f.member);
+    const long size = f.size(H.n, H.o, H.v);
+    if (p >= base && p < base + size) return D.*(f.member) + (p - base);
+  }
+  return nullptr;
+}
+
 // Allocates all arenas and fills the per-molecule descriptors.
 static void allocate(Batch& b, bool keep_mp2) {
   auto arena = [&](const std::function<long(int, int, int)>& size) {
@@ -314,6 +358,15 @@ static void allocate(Batch& b, bool keep_mp2) {
   }
   b.dm = b.dalloc<Mol>(b.nm);
   HCHECK(hipMemcpy(b.dm, b.hm.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
+  if (b.mixed) {
+    b.hd = b.hm;   // constant inputs, DIIS storage: shared with the outer point
+    for (const auto& f : variable_fields()) {
+      auto ptr = arena([&](int n, int o, int v) { return f.size(n, o, v); });
+      for (int k = 0; k < b.nm; ++k) b.hd[k].*(f.member) = ptr[k];
+    }
+    b.ddm = b.dalloc<Mol>(b.nm);
+    HCHECK(hipMemcpy(b.ddm, b.hd.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
+  }
   b.dst = b.dalloc<State>(b.nm);
   HCHECK(hipMemset(b.dst, 0, b.nm * sizeof(State)));
   b.dact = b.dalloc<int>(b.nm);
@@ -364,7 +417,9 @@ static void build_tables(Batch& b) {
 
   // --- one iteration
   b.it.clear();
+  b.ops.clear();
   auto add = [&](const std::function<GemmProb(const Mol&)>& f) {
+    b.ops.push_back(f);
     b.it.push_back(make_table(b, [&](int k) { return f(H[k]); }));
   };
   // g1: Q = gamma P
@@ -510,6 +565,22 @@ static void build_tables(Batch& b) {
   });
 }
 
+// correction-form twins of the iteration GEMMs (64x64 tiles)
+static void build_delta_tables(Batch& b) {
+  b.dit.clear();
+  for (const auto& f : b.ops) {
+    b.dit.push_back(make_table(b, [&](int k) {
+      GemmProb p = f(b.hm[k]);
+      p.dA = delta_of(b, k, p.A.p);
+      p.dB = delta_of(b, k, p.B.p);
+      double* dc = delta_of(b, k, p.C.p);
+      if (dc == nullptr) throw std::string("correction table: constant output operand");
+      p.C.p = dc;
+      return p;
+    }, true, 64));
+  }
+}
+
 enum ItOp {
   G1_Q = 0, G2A, G2B, G3, G4_, G5, G6, G7, G8, G9, G10, G11, G12, G13, G14, G15, G16, G17, G18, G19, G20,
   G21, G22, G23, G24, N_IT_OPS
@@ -532,6 +603,11 @@ struct hcc_ctx {
   bool have_guess = false;
   double last_ms = 0.0;
   double last_iters = 0.0;
+  // mixed-precision solver options
+  int mixed_bf16 = 1;
+  double kappa = 0.03;
+  int max_inner = 40;
+  int max_outer = 12;
   // live timing of the contraction GEMM launches (HIP events on `stream`)
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -627,6 +703,43 @@ static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
   HCHECK(hipGetLastError());
 }
 
+static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
+  if (t.nprob == 0) return;
+  hipEvent_t* ev = nullptr;
+  if (c->prof) {
+    if (c->prof_used == c->prof_events.size()) {
+      hipEvent_t a, b2;
+      HCHECK(hipEventCreate(&a));
+      HCHECK(hipEventCreate(&b2));
+      c->prof_events.emplace_back(a, b2);
+      c->prof_ids.push_back(0);
+    }
+    c->prof_ids[c->prof_used] = id;
+    ev = &c->prof_events[c->prof_used++].first;
+    HCHECK(hipEventRecord(ev[0], c->stream));
+  }
+  HCHECK(launch_gemm_dual(t.d, t.tile_prob, t.tiles, c->batch->dact, c->stream, t.flags, c->mixed_bf16));
+  if (t.red_blocks > 0)
+    HCHECK(launch_splitk_reduce(t.d, t.red_prefix, t.red_bprob, t.red_blocks, c->batch->dact, c->stream));
+  if (ev) HCHECK(hipEventRecord(c->prof_events[c->prof_used - 1].second, c->stream));
+}
+
+// correction residual dOmega at (t0 + dt): every step in correction form
+static void residual_delta(hcc_ctx* c) {
+  Batch& b = *c->batch;
+  hipStream_t s = c->stream;
+  mixed_dress(b.dm, b.ddm, b.dst, b.rgNN, b.rgN, b.rgN3, b.nm, s);
+  dgemm(c, b.dit[G1_Q], 32 + G1_Q);
+  dgemm(c, b.dit[G2A], 32 + G2A);
+  dgemm(c, b.dit[G2B], 32 + G2B);
+  ew_u(b.ddm, b.dst, b.rgT2, b.nm, s);
+  for (int g = G3; g <= G20; ++g) dgemm(c, b.dit[g], 32 + g);
+  ew_init_xx(b.ddm, b.dst, b.rgXX, b.nm, s);
+  for (int g = G21; g <= G24; ++g) dgemm(c, b.dit[g], 32 + g);
+  ew_assemble(b.ddm, b.dst, b.rgT2, b.nm, s);
+  HCHECK(hipGetLastError());
+}
+
 // integral precompute (Fao, undressed (kc|ld), L matrices)
 static void precompute(hcc_ctx* c) {
   Batch& b = *c->batch;
@@ -653,6 +766,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   c->batch = b;
   b->nm = nmol;
   b->m = c->diis_steps;
+  b->mixed = c->precision == HCC_MIXED;
   for (int k = 0; k < nmol; ++k) {
     if (n[k] < 2 || nocc[k] < 1 || nocc[k] >= n[k]) throw std::string("invalid molecule dimensions");
     b->n.push_back(n[k]);
@@ -662,6 +776,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   }
   allocate(*b, keep_mp2);
   build_tables(*b);
+  if (b->mixed) build_delta_tables(*b);
   // upload, transposing the column-major inputs to row-major
   long off = 0;
   std::vector<double> h;
@@ -735,6 +850,35 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
   ctx->projtype = projection_type;
   ctx->diis = diis_enabled ? 1 : 0;
   return HCC_OK;
+}
+
+static int need_batch(hcc_ctx* ctx);
+
+int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (!(kappa > 0.0 && kappa < 1.0) || max_inner < 1 || max_outer < 1) {
+    ctx->err = "invalid mixed-precision options";
+    return HCC_E_ARG;
+  }
+  ctx->mixed_bf16 = bf16 ? 1 : 0;
+  ctx->kappa = kappa;
+  ctx->max_inner = max_inner;
+  ctx->max_outer = max_outer;
+  return HCC_OK;
+}
+
+int hcc_batch_statistics(hcc_ctx* ctx, int* outer_evaluations, int* inner_iterations) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  std::vector<State> S(b.nm);
+  HCHECK(hipMemcpy(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost));
+  for (int k = 0; k < b.nm; ++k) {
+    if (outer_evaluations) outer_evaluations[k] = S[k].outer_its;
+    if (inner_iterations) inner_iterations[k] = S[k].iter - S[k].outer_its;
+  }
+  API_END
 }
 
 int hcc_set_molecule(hcc_ctx* ctx, int n, int n_occ, const double* C, const double* F_mo, const double* gamma,
@@ -936,6 +1080,74 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   return HCC_OK;
 }
 
+// Defect-correction solve: FP64 residual at the outer point t0, then inner
+// iterations on the correction dt whose residual is Omega(t0) + dOmega with
+// dOmega from correction-form GEMMs (BF16 operands, FP32 accumulation), until
+// max|d| < kappa * max|d(t0)|; then t0 += dt.  The FP64 residual at each outer
+// point decides convergence, so the answer is the FP64 fixed point.
+static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool build_integrals) {
+  Batch& b = *ctx->batch;
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventRecord(e0, s));
+  if (build_integrals) precompute(ctx);
+  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, guess_kind, s);
+  Params prm = params(ctx, eps, maxiter, true);
+  prm.kappa = ctx->kappa;
+  prm.max_inner = ctx->max_inner;
+  Params prm0 = prm;   // residual-norm pass at the outer point: no step, no DIIS
+  prm0.s1 = prm0.s2 = 0.0;
+  prm0.diis = 0;
+  std::vector<int> act(b.nm);
+  auto alive = [&]() {
+    HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
+    HCHECK(hipStreamSynchronize(s));
+    if (ctx->prof) prof_collect(ctx);
+    int n = 0;
+    for (int a : act) n += a;
+    return n;
+  };
+  for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
+    set_phase(b.dst, b.dact, b.nm, 0, s);
+    residual(ctx);
+    ew_update(b.dm, b.dst, b.rgOV, b.rgT2, b.nm, prm0, b.part1, b.part2, s);
+    mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    if (alive() == 0) break;
+    set_phase(b.dst, b.dact, b.nm, 1, s);
+    mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
+    for (int inner = 0; inner < ctx->max_inner; ++inner) {
+      residual_delta(ctx);
+      mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
+      mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
+      if (prm.diis) {
+        diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
+        mixed_inner_advance(b.dm, b.dst, b.nm, s);
+      }
+      if ((inner & 3) == 3 && alive() == 0) break;
+    }
+    mixed_commit(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
+  }
+  set_phase(b.dst, b.dact, b.nm, 0, s);
+  energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
+  HCHECK(hipEventRecord(e1, s));
+  HCHECK(hipEventSynchronize(e1));
+  if (ctx->prof) prof_collect(ctx);
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, e0, e1));
+  ctx->last_ms = ms;
+  std::vector<State> S(b.nm);
+  HCHECK(hipMemcpy(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost));
+  double its = 0.0;
+  for (auto& x : S) its += x.iter;
+  ctx->last_iters = its;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->have_guess = true;
+  return HCC_OK;
+}
+
 int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double energy_out[5], int* iterations,
               int* status) {
   if (ctx == nullptr) return HCC_E_ARG;
@@ -945,7 +1157,8 @@ int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double ener
     return HCC_E_ARG;
   }
   API_BEGIN
-  run_solve(ctx, guess_kind, maxiter, eps, false);
+  if (ctx->precision == HCC_MIXED) run_solve_mixed(ctx, guess_kind, maxiter, eps, false);
+  else run_solve(ctx, guess_kind, maxiter, eps, false);
   State S;
   HCHECK(hipMemcpy(&S, ctx->batch->dst, sizeof(State), hipMemcpyDeviceToHost));
   if (energy_out)
@@ -963,7 +1176,8 @@ int hcc_batch_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
     return HCC_E_ARG;
   }
   API_BEGIN
-  run_solve(ctx, guess_kind, maxiter, eps, true);
+  if (ctx->precision == HCC_MIXED) run_solve_mixed(ctx, guess_kind, maxiter, eps, true);
+  else run_solve(ctx, guess_kind, maxiter, eps, true);
   API_END
 }
 

@@ -53,6 +53,20 @@ int hcc_create(hcc_ctx** ctx, int device, int precision, int diis_steps);
 int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection_type,
                     int diis_enabled);
 
+/* Options of the mixed-precision solver (precision HCC_MIXED; no reference
+ * counterpart -- the reference is FP64 only).  The solve is a defect
+ * correction: an FP64 residual at the outer point t0 decides convergence;
+ * between outer points, inner iterations solve for the correction dt with
+ * correction-form contractions C += A0 dB + dA (B0 + dB) whose operands are
+ * rounded to BF16 (bf16 = 1; 0 keeps FP64, for testing), until
+ * max|d| < kappa * max|d(t0)| or max_inner iterations; at most max_outer
+ * outer points.  Defaults: 1, 0.03, 40, 12. */
+int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer);
+
+/* Per molecule of the last mixed-precision batch solve: FP64 residual
+ * evaluations (outer points) and inner iterations. */
+int hcc_batch_statistics(hcc_ctx* ctx, int* outer_evaluations, int* inner_iterations);
+
 /* Upload one molecule and build its integrals: n orbitals (= sites), n_occ
  * doubly occupied, C = hV (MO coefficients, columns ascending in energy),
  * F_mo = MO Fock matrix, gamma = mol%G, e_ref = reference (SCF) energy.
