@@ -80,11 +80,15 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nmol", type=int, default=4096)
-    ap.add_argument("--eps", type=float, default=1e-6, help="residual threshold of the timed solves")
+    ap.add_argument("--eps", type=float, default=2e-7, help="residual threshold of the timed solves")
     ap.add_argument("--diis", type=int, default=8)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--kappa", type=float, default=0.03)
+    ap.add_argument("--max-inner", type=int, default=40)
+    ap.add_argument("--max-outer", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one MI355X; "
@@ -97,61 +101,89 @@ def main():
     o = np.array([m["nocc"] for m in mols])
     v = np.array([m["n"] - m["nocc"] for m in mols])
 
+    # FP64 reference energies (untimed, tight threshold)
     ctx = capi.Context(0, capi.HCC_FP64, args.diis)
     ctx.batch_load(mols)
-    # FP64 reference energies (untimed, tight threshold)
     ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
     e_ref, it_ref, st_ref = ctx.batch_results()
+    ctx.close()
     print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
           f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        ctx.batch_solve(capi.GUESS_MP2, 500, args.eps)
-    ctx.profile(True)
-    t_steps = []
-    iters = None
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        ctx.batch_solve(capi.GUESS_MP2, 500, args.eps)   # returns after a device synchronize
-        t_steps.append(time.perf_counter() - t1)
+    def timed(precision, guess):
+        ctx = capi.Context(0, precision, args.diis)
+        if precision == capi.HCC_MIXED:
+            ctx.set_mixed(True, args.kappa, args.max_inner, args.max_outer)
+        ctx.batch_load(mols)
+        for _ in range(args.warmup):
+            ctx.batch_solve(guess, 2000, args.eps)
+        ctx.profile(True)
+        t_steps = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            ctx.batch_solve(guess, 2000, args.eps)   # returns after a device synchronize
+            t_steps.append(time.perf_counter() - t1)
         e, iters, st = ctx.batch_results()
-    gemm_ms, launches = ctx.profile_read()
-    ctx.profile(False)
-    fl_it, fl_pre = ctx.gemm_flops(len(mols))
-    ctx.close()
+        gemm_ms, launches = ctx.profile_read()
+        ops_ms = ctx.profile_ops()
+        fl_it, fl_pre = ctx.gemm_flops(len(mols))
+        outer = inner = None
+        if precision == capi.HCC_MIXED:
+            outer, inner = ctx.batch_statistics()
+        ctx.close()
+        de = np.abs(e[:, 0] - e_ref[:, 0])
+        ok = (de < 1e-6) & (st == 1) & (st_ref == 1)
+        step_s = float(np.mean(t_steps))
+        return dict(e=e, iters=iters, st=st, de=de, ok=ok, step_s=step_s, gemm_ms=gemm_ms, launches=launches,
+                    ops_ms=ops_ms, fl_it=fl_it, fl_pre=fl_pre, outer=outer, inner=inner,
+                    solves=ok.sum() / step_s)
 
-    de = np.abs(e[:, 0] - e_ref[:, 0])
-    ok = (de < 1e-6) & (st == 1) & (st_ref == 1)
-    step_s = float(np.mean(t_steps))
-    solves_per_s = ok.sum() / step_s
-    gemm_flops = args.steps * float((fl_it * iters).sum() + fl_pre.sum())
-    achieved = gemm_flops / (gemm_ms * 1e-3)
-    falg = float((f_iter(o, v) * iters).sum())
+    r64 = timed(capi.HCC_FP64, capi.GUESS_MP2)
+    print(f"# fp64: {r64['solves']:.1f} solves/s, {r64['step_s']*1e3:.1f} ms/step, "
+          f"converged {int(r64['ok'].sum())}/{len(mols)}, max|dE| {r64['de'].max():.2e}", file=sys.stderr, flush=True)
+    rmx = timed(capi.HCC_MIXED, capi.GUESS_MP2) if not args.fp64_only else None
+    if rmx is not None:
+        print(f"# mixed: {rmx['solves']:.1f} solves/s, {rmx['step_s']*1e3:.1f} ms/step, "
+              f"converged {int(rmx['ok'].sum())}/{len(mols)}, max|dE| {rmx['de'].max():.2e}, "
+              f"outer {rmx['outer'].mean():.2f} inner {rmx['inner'].mean():.2f}", file=sys.stderr, flush=True)
+    head = rmx if rmx is not None else r64
+
+    # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
+    gemm_flops = args.steps * float((r64["fl_it"] * r64["iters"]).sum() + r64["fl_pre"].sum())
+    achieved = gemm_flops / (r64["gemm_ms"] * 1e-3)
+    falg = float((f_iter(o, v) * r64["iters"]).sum())
 
     line = {
         "metric": "converged CCSD solves/s at |dE|<1e-6 eV",
-        "value": float(solves_per_s),
+        "value": float(head["solves"]),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_s * 1e3,
+        "ms_per_step": head["step_s"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16+f64" if rmx is not None else "f64",
         "data": "synthetic",
         "config": {"workload": f"batch of {len(mols)} seeded honeycomb fragments, N in [10,48], "
-                               "plain FP64 batched solver, MP2 guess",
-                   "molecules": len(mols), "solver": "fp64", "residual_threshold": args.eps,
-                   "diis_steps": args.diis, "mean_iterations": float(iters.mean()),
-                   "max_abs_dE_eV": float(de.max()), "converged": int(ok.sum())},
+                               + ("mixed-precision defect-correction solver" if rmx is not None else
+                                  "plain FP64 batched solver") + ", MP2 guess",
+                   "molecules": len(mols), "residual_threshold": args.eps, "diis_steps": args.diis,
+                   "converged": int(head["ok"].sum()), "max_abs_dE_eV": float(head["de"].max()),
+                   "fp64_solves_per_s": float(r64["solves"]), "fp64_ms_per_step": r64["step_s"] * 1e3,
+                   "fp64_mean_iterations": float(r64["iters"].mean()),
+                   "speedup_vs_fp64": float(head["solves"] / r64["solves"]) if rmx is not None else 1.0},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                     "kernel": "gemm_f64_kernel", "launches": int(launches),
-                     "kernel_ms": gemm_ms, "gemm_flops": gemm_flops},
-        "f_iter_rate_tflops": falg / step_s / 1e12,
+                     "kernel": "gemm_f64_kernel (FP64 solver)", "launches": int(r64["launches"]),
+                     "kernel_ms": r64["gemm_ms"], "gemm_flops": gemm_flops},
+        "f_iter_rate_tflops": falg / r64["step_s"] / 1e12,
     }
+    if rmx is not None:
+        line["config"].update({"mixed_mean_outer": float(rmx["outer"].mean()),
+                               "mixed_mean_inner": float(rmx["inner"].mean()),
+                               "mixed_gemm_ms": rmx["gemm_ms"]})
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mols, args.cpu_budget)
     print(json.dumps(line), flush=True)
