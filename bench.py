@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from helios_amd import capi, molecules, ppp  # noqa: E402
+from helios_amd import capi, learned_guess, molecules, ppp  # noqa: E402
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--ablation", action="store_true", help="also time FP64+learned and mixed+MP2")
     ap.add_argument("--kappa", type=float, default=0.03)
     ap.add_argument("--max-inner", type=int, default=40)
     ap.add_argument("--max-outer", type=int, default=12)
@@ -110,10 +111,13 @@ def main():
     print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
           f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
+    net = learned_guess.load_params()
+
     def timed(precision, guess):
         ctx = capi.Context(0, precision, args.diis)
         if precision == capi.HCC_MIXED:
             ctx.set_mixed(True, args.kappa, args.max_inner, args.max_outer)
+        ctx.set_guess_network(net)
         ctx.batch_load(mols)
         for _ in range(args.warmup):
             ctx.batch_solve(guess, 2000, args.eps)
@@ -141,11 +145,20 @@ def main():
     r64 = timed(capi.HCC_FP64, capi.GUESS_MP2)
     print(f"# fp64: {r64['solves']:.1f} solves/s, {r64['step_s']*1e3:.1f} ms/step, "
           f"converged {int(r64['ok'].sum())}/{len(mols)}, max|dE| {r64['de'].max():.2e}", file=sys.stderr, flush=True)
-    rmx = timed(capi.HCC_MIXED, capi.GUESS_MP2) if not args.fp64_only else None
+    rmx = timed(capi.HCC_MIXED, capi.GUESS_LEARNED) if not args.fp64_only else None
     if rmx is not None:
-        print(f"# mixed: {rmx['solves']:.1f} solves/s, {rmx['step_s']*1e3:.1f} ms/step, "
+        print(f"# mixed+learned: {rmx['solves']:.1f} solves/s, {rmx['step_s']*1e3:.1f} ms/step, "
               f"converged {int(rmx['ok'].sum())}/{len(mols)}, max|dE| {rmx['de'].max():.2e}, "
               f"outer {rmx['outer'].mean():.2f} inner {rmx['inner'].mean():.2f}", file=sys.stderr, flush=True)
+    ablation = {}
+    if args.ablation:
+        for name, prec, guess in (("fp64_learned", capi.HCC_FP64, capi.GUESS_LEARNED),
+                                  ("mixed_mp2", capi.HCC_MIXED, capi.GUESS_MP2)):
+            r = timed(prec, guess)
+            ablation[name] = {"solves_per_s": float(r["solves"]), "ms_per_step": r["step_s"] * 1e3,
+                              "mean_iterations": float(r["iters"].mean()), "converged": int(r["ok"].sum())}
+            print(f"# {name}: {r['solves']:.1f} solves/s, mean iterations {r['iters'].mean():.2f}",
+                  file=sys.stderr, flush=True)
     head = rmx if rmx is not None else r64
 
     # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
@@ -167,8 +180,8 @@ def main():
         "dtype": "bf16+f64" if rmx is not None else "f64",
         "data": "synthetic",
         "config": {"workload": f"batch of {len(mols)} seeded honeycomb fragments, N in [10,48], "
-                               + ("mixed-precision defect-correction solver" if rmx is not None else
-                                  "plain FP64 batched solver") + ", MP2 guess",
+                               + ("mixed-precision defect-correction solver, learned T2 guess" if rmx is not None
+                                  else "plain FP64 batched solver, MP2 guess"),
                    "molecules": len(mols), "residual_threshold": args.eps, "diis_steps": args.diis,
                    "converged": int(head["ok"].sum()), "max_abs_dE_eV": float(head["de"].max()),
                    "fp64_solves_per_s": float(r64["solves"]), "fp64_ms_per_step": r64["step_s"] * 1e3,
@@ -184,6 +197,8 @@ def main():
         line["config"].update({"mixed_mean_outer": float(rmx["outer"].mean()),
                                "mixed_mean_inner": float(rmx["inner"].mean()),
                                "mixed_gemm_ms": rmx["gemm_ms"]})
+    if ablation:
+        line["config"]["ablation"] = ablation
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mols, args.cpu_budget)
     print(json.dumps(line), flush=True)

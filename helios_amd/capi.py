@@ -16,11 +16,16 @@ HCC_MIXED = 1
 GUESS_REFERENCE = 0
 GUESS_MP2 = 1
 GUESS_LEARNED = 2
+MLP_PARAMS = 5313
+MLP_FEATURES = 16
+MLP_ROW = 18
 
 _P = ctypes.POINTER
 _d = ctypes.c_double
 _i = ctypes.c_int
 _vp = ctypes.c_void_p
+_f = ctypes.c_float
+_l = ctypes.c_long
 
 # name -> (restype, argtypes); mirrors include/helios_ccsd.h
 SIGNATURES = {
@@ -47,6 +52,20 @@ SIGNATURES = {
     "hcc_destroy": (None, [_vp]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
+    "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
+    "hcc_mlp_create": (_i, [_P(_vp), _i, _l]),
+    "hcc_mlp_destroy": (None, [_vp]),
+    "hcc_mlp_last_error": (ctypes.c_char_p, [_vp]),
+    "hcc_mlp_set_params": (_i, [_vp, _P(_f), _i]),
+    "hcc_mlp_get_params": (_i, [_vp, _P(_f), _i]),
+    "hcc_mlp_collect": (_i, [_vp, _vp, _i, ctypes.c_uint, _P(_l)]),
+    "hcc_mlp_samples": (_i, [_vp, _P(_l)]),
+    "hcc_mlp_read_samples": (_i, [_vp, _l, _l, _P(_f), _P(_i)]),
+    "hcc_mlp_write_samples": (_i, [_vp, _P(_f), _l]),
+    "hcc_mlp_gradient": (_i, [_vp, _l, _i, _P(_d), _P(_f)]),
+    "hcc_mlp_train": (_i, [_vp, _i, _i, _f, _f, ctypes.c_uint, _P(_d)]),
+    "hcc_mlp_evaluate": (_i, [_vp, _P(_d)]),
+    "hcc_mlp_predict": (_i, [_vp, _P(_f), _l, _P(_f)]),
 }
 
 _lib = None
@@ -75,6 +94,10 @@ def _ip(a):
     return a.ctypes.data_as(_P(_i))
 
 
+def _fp(a):
+    return a.ctypes.data_as(_P(_f))
+
+
 def fortran(a):
     """Column-major float64 copy (the boundary's storage order)."""
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
@@ -94,6 +117,7 @@ class Context:
         if rc != 0:
             raise HccError(f"hcc_create failed ({rc})")
         self.o = self.v = None
+        self.nmol = 0
 
     def _check(self, rc, what):
         if rc != 0:
@@ -130,6 +154,7 @@ class Context:
         self._check(self.lib.hcc_set_molecule(self.h, n, nocc, _dp(self._C), _dp(self._F), _dp(self._G),
                                               float(eref)), "hcc_set_molecule")
         self.o, self.v = nocc, n - nocc
+        self.nmol = 1
 
     def guess(self, kind=GUESS_REFERENCE):
         self._check(self.lib.hcc_guess(self.h, kind), "hcc_guess")
@@ -211,11 +236,97 @@ class Context:
         self._check(self.lib.hcc_gemm_flops(self.h, _dp(it), _dp(pre)), "hcc_gemm_flops")
         return it, pre
 
+    def set_guess_network(self, params):
+        """Parameters (MLP_PARAMS float32) of the network used by GUESS_LEARNED."""
+        self._w = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(self.lib.hcc_set_guess_network(self.h, _fp(self._w), self._w.size), "hcc_set_guess_network")
+
     def last_timing(self):
         ms = ctypes.c_double()
         its = ctypes.c_double()
         self._check(self.lib.hcc_last_timing(self.h, ctypes.byref(ms), ctypes.byref(its)), "hcc_last_timing")
         return ms.value, its.value
+
+
+class Trainer:
+    """Owner of an hcc_mlp: device sample set, parameters and GPU training."""
+
+    def __init__(self, capacity, device=0):
+        self.lib = load()
+        self.h = _vp()
+        rc = self.lib.hcc_mlp_create(ctypes.byref(self.h), device, int(capacity))
+        if rc != 0:
+            raise HccError(f"hcc_mlp_create failed ({rc})")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.hcc_mlp_last_error(self.h)
+            raise HccError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.hcc_mlp_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        self._check(self.lib.hcc_mlp_set_params(self.h, _fp(p), p.size), "hcc_mlp_set_params")
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.hcc_mlp_get_params(self.h, _fp(p), p.size), "hcc_mlp_get_params")
+        return p
+
+    def collect(self, ctx, per_molecule, seed=0):
+        n = ctypes.c_long()
+        self._check(self.lib.hcc_mlp_collect(self.h, ctx.h, per_molecule, seed, ctypes.byref(n)), "hcc_mlp_collect")
+        return n.value
+
+    def count(self):
+        n = ctypes.c_long()
+        self._check(self.lib.hcc_mlp_samples(self.h, ctypes.byref(n)), "hcc_mlp_samples")
+        return n.value
+
+    def read_samples(self, first=0, count=None):
+        count = self.count() - first if count is None else count
+        rows = np.zeros((count, MLP_ROW), dtype=np.float32)
+        where = np.zeros((count, 2), dtype=np.int32)
+        self._check(self.lib.hcc_mlp_read_samples(self.h, first, count, _fp(rows), _ip(where)),
+                    "hcc_mlp_read_samples")
+        return rows, where
+
+    def write_samples(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        self._check(self.lib.hcc_mlp_write_samples(self.h, _fp(rows), rows.shape[0]), "hcc_mlp_write_samples")
+
+    def gradient(self, first=0, count=None):
+        count = self.count() - first if count is None else count
+        loss = ctypes.c_double()
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.hcc_mlp_gradient(self.h, first, count, ctypes.byref(loss), _fp(g)), "hcc_mlp_gradient")
+        return loss.value, g
+
+    def train(self, epochs, batch=4096, lr=2e-3, lr_final=1e-4, seed=0):
+        hist = np.zeros(epochs)
+        self._check(self.lib.hcc_mlp_train(self.h, epochs, batch, lr, lr_final, seed, _dp(hist)), "hcc_mlp_train")
+        return hist
+
+    def evaluate(self):
+        loss = ctypes.c_double()
+        self._check(self.lib.hcc_mlp_evaluate(self.h, ctypes.byref(loss)), "hcc_mlp_evaluate")
+        return loss.value
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        self._check(self.lib.hcc_mlp_predict(self.h, _fp(X), X.shape[0], _fp(y)), "hcc_mlp_predict")
+        return y
 
 
 def selftest_gemm(A, B, C=None, alpha=1.0, beta=0.0, precision=HCC_FP64):

@@ -25,6 +25,8 @@ struct Mol {
   double *t2mp2;
   // dressed-density diagonal and its Coulomb field (2n)
   double *dvec;
+  // site-localisation overlaps of the MOs (n x n), features of the learned guess
+  double *Ov;
 };
 
 // Per-molecule iteration state (reference iterator semantics,

@@ -524,7 +524,9 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
   }
 }
 
-template <bool AKF, bool BNF>
+// HA / HB: the operand has a correction (dA / dB non-null for every problem
+// of the table).  A constant A needs only A0 dB, a constant B only dA B0.
+template <bool AKF, bool BNF, bool HA, bool HB>
 __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict__ probs,
                                                       const int* __restrict__ tile_prob, int total_tiles,
                                                       const int* __restrict__ active) {
@@ -533,6 +535,8 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   __shared__ __attribute__((aligned(16))) __bf16 dAs[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 dBs[BN * LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bfs[BN * LD];
+  __shared__ int koffA[kKTab];
+  __shared__ int koffB[kKTab];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
   const GemmProb P = probs[find_problem(tile_prob, tile)];
@@ -546,7 +550,6 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const bool hasA = P.dA != nullptr, hasB = P.dB != nullptr;
   int fa[NE], fb[NE], kqa[NE], kqb[NE], sa[NE], sb[NE];
 #pragma unroll
   for (int i = 0; i < NE; ++i) {
@@ -561,63 +564,80 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     sb[i] = nl * LD + kqb[i];
   }
   const double* __restrict__ A0 = P.A.p;
-  const double* __restrict__ dA = hasA ? P.dA : P.A.p;
+  const double* __restrict__ dA = P.dA;
   const double* __restrict__ B0 = P.B.p;
-  const double* __restrict__ dB = hasB ? P.dB : P.B.p;
+  const double* __restrict__ dB = P.dB;
+  // staged values: A0 and dB feed A0 dB (needs HB), dA and B0 + dB feed dA (B0 + dB) (needs HA)
   float ra0[NE], rda[NE], rdb[NE], rbf[NE];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < NE; ++i) {
-      int k = k0 + kqa[i];
-      bool ok = fa[i] >= 0 && k < kend;
-      int o = ok ? fa[i] + col_off(P.A, k) : 0;
-      const double a0 = A0[o], da = dA[o];
-      ra0[i] = ok ? (float)a0 : 0.f;
-      rda[i] = (ok && hasA) ? (float)da : 0.f;
-      k = k0 + kqb[i];
-      ok = fb[i] >= 0 && k < kend;
-      o = ok ? fb[i] + row_off(P.B, k) : 0;
-      const double b0 = B0[o], db = dB[o];
-      const double dbv = (ok && hasB) ? db : 0.0;
-      rdb[i] = (float)dbv;
-      rbf[i] = ok ? (float)(b0 + dbv) : 0.f;
-    }
-  };
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  load(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-#pragma unroll
-    for (int i = 0; i < NE; ++i) {
-      A0s[sa[i]] = (__bf16)ra0[i];
-      dAs[sa[i]] = (__bf16)rda[i];
-      dBs[sb[i]] = (__bf16)rdb[i];
-      Bfs[sb[i]] = (__bf16)rbf[i];
+  for (int c0 = kbeg; c0 < kend; c0 += kKTab) {
+    const int cend = min(kend, c0 + kKTab);
+    __syncthreads();
+    for (int q = t; q < cend - c0; q += T) {
+      koffA[q] = col_off(P.A, c0 + q);
+      koffB[q] = row_off(P.B, c0 + q);
     }
     __syncthreads();
-    load(min(k0 + BK, kend));
-    const int kq = 8 * (lane >> 4);
-    const int ra = (wm + (lane & 15)) * LD + kq, rb = (wn + (lane & 15)) * LD + kq;
-    const bf8 a0 = *reinterpret_cast<const bf8*>(&A0s[ra]);
-    const bf8 a1 = *reinterpret_cast<const bf8*>(&A0s[ra + 16 * LD]);
-    const bf8 e0 = *reinterpret_cast<const bf8*>(&dAs[ra]);
-    const bf8 e1 = *reinterpret_cast<const bf8*>(&dAs[ra + 16 * LD]);
-    const bf8 b0 = *reinterpret_cast<const bf8*>(&dBs[rb]);
-    const bf8 b1 = *reinterpret_cast<const bf8*>(&dBs[rb + 16 * LD]);
-    const bf8 f0 = *reinterpret_cast<const bf8*>(&Bfs[rb]);
-    const bf8 f1 = *reinterpret_cast<const bf8*>(&Bfs[rb + 16 * LD]);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f1, acc[1][1], 0, 0, 0);
-    __syncthreads();
+    // branch-free staging: padded elements read the operand base and are zeroed
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        int kl = k0 - c0 + kqa[i];
+        bool ok = fa[i] >= 0 && kl < cend - c0;
+        int o = ok ? fa[i] + koffA[ok ? kl : 0] : 0;
+        if (HB) ra0[i] = ok ? (float)A0[o] : 0.f;
+        if (HA) rda[i] = ok ? (float)dA[o] : 0.f;
+        kl = k0 - c0 + kqb[i];
+        ok = fb[i] >= 0 && kl < cend - c0;
+        o = ok ? fb[i] + koffB[ok ? kl : 0] : 0;
+        const double db = HB ? dB[o] : 0.0;
+        if (HB) rdb[i] = ok ? (float)db : 0.f;
+        if (HA) rbf[i] = ok ? (float)(B0[o] + db) : 0.f;
+      }
+    };
+    load(c0);
+    for (int k0 = c0; k0 < cend; k0 += BK) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        if (HB) {
+          A0s[sa[i]] = (__bf16)ra0[i];
+          dBs[sb[i]] = (__bf16)rdb[i];
+        }
+        if (HA) {
+          dAs[sa[i]] = (__bf16)rda[i];
+          Bfs[sb[i]] = (__bf16)rbf[i];
+        }
+      }
+      __syncthreads();
+      load(min(k0 + BK, cend));   // the last prefetch is fully masked
+      const int kq = 8 * (lane >> 4);
+      const int ra = (wm + (lane & 15)) * LD + kq, rb = (wn + (lane & 15)) * LD + kq;
+      if (HB) {
+        const bf8 a0 = *reinterpret_cast<const bf8*>(&A0s[ra]);
+        const bf8 a1 = *reinterpret_cast<const bf8*>(&A0s[ra + 16 * LD]);
+        const bf8 b0 = *reinterpret_cast<const bf8*>(&dBs[rb]);
+        const bf8 b1 = *reinterpret_cast<const bf8*>(&dBs[rb + 16 * LD]);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      if (HA) {
+        const bf8 e0 = *reinterpret_cast<const bf8*>(&dAs[ra]);
+        const bf8 e1 = *reinterpret_cast<const bf8*>(&dAs[ra + 16 * LD]);
+        const bf8 f0 = *reinterpret_cast<const bf8*>(&Bfs[rb]);
+        const bf8 f1 = *reinterpret_cast<const bf8*>(&Bfs[rb + 16 * LD]);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
   }
   if (P.ksplit > 1) {
     double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
@@ -660,12 +680,20 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tile
   if (total_tiles <= 0) return hipSuccess;
   const int grid = ((total_tiles + 7) / 8) * 8;
   if (bf16) {
+#define HCC_DUAL16(AK, BN)                                                                              \
+  switch ((flags >> 2) & 3) {                                                                           \
+    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    default: return hipErrorInvalidValue; /* both operands constant: not a correction product */     \
+  }
     switch (flags & 3) {
-      case 0: hipLaunchKernelGGL((gemm_dual_bf16<false, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
-      case 1: hipLaunchKernelGGL((gemm_dual_bf16<true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
-      case 2: hipLaunchKernelGGL((gemm_dual_bf16<false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
-      default: hipLaunchKernelGGL((gemm_dual_bf16<true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
+      case 0: HCC_DUAL16(false, false) break;
+      case 1: HCC_DUAL16(true, false) break;
+      case 2: HCC_DUAL16(false, true) break;
+      default: HCC_DUAL16(true, true) break;
     }
+#undef HCC_DUAL16
   } else {
     switch (flags & 3) {
       case 0: hipLaunchKernelGGL((gemm_dual_f64<false, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;

@@ -17,6 +17,18 @@
 
 namespace hcc {
 
+// HIP failure inside the library; the C ABI turns it into HCC_E_HIP
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HCHECK(x)                                        \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) throw HipError{e_, #x};        \
+  } while (0)
+
 struct Operand {
   double* p;        // base of this molecule's block
   int rr, rc;       // radices of the row and column composite indices (>= 1)

@@ -159,13 +159,17 @@ __device__ __forceinline__ long packed_index(long mm, long nn, long ne) {
   return mm * ne - mm * (mm - 1) / 2 + (nn - mm);
 }
 
+// maximum that propagates NaN (fmax drops it), so that a NaN residual can
+// never read as a converged one
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
 __device__ __forceinline__ double block_max(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) sh[w] = v;
   __syncthreads();
   double r = sh[0];
-  for (int k = 1; k < kEwThreads / 64; ++k) r = fmax(r, sh[k]);
+  for (int k = 1; k < kEwThreads / 64; ++k) r = nmax(r, sh[k]);
   __syncthreads();
   return r;
 }
@@ -291,8 +295,8 @@ __global__ __launch_bounds__(kEwThreads) void k_control1(const Mol* __restrict__
   State& S = st[mol];
   if (!S.active) return;
   double a1 = 0.0, a2 = 0.0;
-  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
-  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = nmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = nmax(a2, part2[b]);
   a1 = block_max(a1, sh);
   a2 = block_max(a2, sh);
   if (threadIdx.x != 0) return;
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(kEwThreads) void k_control1(const Mol* __restrict__
   S.acc[1] = a2;
   S.iter += 1;
   S.prev = S.cur;
-  S.cur = fmax(a1, a2);
+  S.cur = nmax(a1, a2);
   if (S.cur < S.prev) S.successful += 1;
   S.staged = 0;
   if (S.cur < prm.eps) {
@@ -325,7 +329,7 @@ __global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st,
   int status = 0;
   if (S.status == 1) status = 1;
   else if (stop != nullptr && stop[mol]) status = -4;
-  else if (prm.check_divergence && S.cur > prm.divergence) status = -2;
+  else if (prm.check_divergence && !(S.cur <= prm.divergence)) status = -2;   // NaN diverges too
   else if (S.iter + 1 > prm.maxiter) status = -1;
   else if (prm.check_stagnation && S.iter > prm.stagnation_after &&
            100.0 * S.successful / S.iter <= prm.stagnation_pct) status = -3;
@@ -839,14 +843,14 @@ __global__ __launch_bounds__(kEwThreads) void k_outer_control(const Mol* __restr
   State& S = st[mol];
   if (S.status != 0) return;
   double a1 = 0.0, a2 = 0.0;
-  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
-  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = nmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = nmax(a2, part2[b]);
   a1 = block_max(a1, sh);
   a2 = block_max(a2, sh);
   if (threadIdx.x != 0) return;
   S.acc[0] = a1;
   S.acc[1] = a2;
-  S.acc0 = fmax(a1, a2);
+  S.acc0 = nmax(a1, a2);
   S.cur = S.acc0;
   S.iter += 1;
   S.outer_its += 1;
@@ -871,14 +875,14 @@ __global__ __launch_bounds__(kEwThreads) void k_inner_control(const Mol* __restr
   State& S = st[mol];
   if (!S.active) return;
   double a1 = 0.0, a2 = 0.0;
-  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = fmax(a1, part1[b]);
-  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = fmax(a2, part2[b]);
+  for (int b = pre1[mol] + threadIdx.x; b < pre1[mol + 1]; b += kEwThreads) a1 = nmax(a1, part1[b]);
+  for (int b = pre2[mol] + threadIdx.x; b < pre2[mol + 1]; b += kEwThreads) a2 = nmax(a2, part2[b]);
   a1 = block_max(a1, sh);
   a2 = block_max(a2, sh);
   if (threadIdx.x != 0) return;
   S.acc[0] = a1;
   S.acc[1] = a2;
-  S.cur = fmax(a1, a2);
+  S.cur = nmax(a1, a2);
   S.iter += 1;
   S.inner_its += 1;
   S.staged = 0;

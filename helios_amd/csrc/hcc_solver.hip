@@ -25,19 +25,9 @@
 #include "../../include/helios_ccsd.h"
 #include "hcc_batch.h"
 #include "hcc_ops.h"
+#include "hcc_mlp.h"
 
 namespace hcc {
-
-struct HipError {
-  hipError_t e;
-  const char* what;
-};
-
-#define HCHECK(x)                                        \
-  do {                                                   \
-    hipError_t e_ = (x);                                 \
-    if (e_ != hipSuccess) throw HipError{e_, #x};        \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // operand helpers
@@ -61,7 +51,7 @@ struct GemmTable {
   int* red_bprob = nullptr;    // problem index of every reduction block
   int red_blocks = 0;
   int tile = 64;               // output tile edge (64 or 32)
-  int flags = 3;               // operand unit-stride flags (a_kfast | b_nfast << 1)
+  int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3
 };
 
 // Split K when a problem has few output tiles and a long K: aim for >= 16
@@ -105,7 +95,7 @@ struct Batch {
   State* dst = nullptr;
   int* dact = nullptr;
   std::vector<void*> allocs;
-  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgT2L;
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   double *d1out = nullptr, *d2out = nullptr;
@@ -208,7 +198,8 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
     size_t big = 0;
     for (size_t q = 1; q < probs.size(); ++q)
       if ((double)probs[q].M * probs[q].N * probs[q].K > (double)probs[big].M * probs[big].N * probs[big].K) big = q;
-    t.flags = probs[big].a_kfast | (probs[big].b_nfast << 1);
+    t.flags = probs[big].a_kfast | (probs[big].b_nfast << 1) | ((probs[big].dA != nullptr) << 2) |
+              ((probs[big].dB != nullptr) << 3);
   }
   if (ws_total > 0) {
     double* ws = b.dalloc<double>(ws_total);
@@ -328,6 +319,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   auto Xvv = arena([](int, int, int v) { return (long)v * v; });
   auto dvec = arena([](int n, int, int) { return 2L * n; });
   auto Xoo = arena([](int, int o, int) { return (long)o * o; });
+  auto Ov = arena(sz_nn);
   auto ndsize = [m](int, int o, int v) {
     long ov = (long)o * v;
     return (long)m * (ov + ov * (ov + 1) / 2);
@@ -354,6 +346,7 @@ static void allocate(Batch& b, bool keep_mp2) {
     M.G4 = G4[k]; M.Wkl = Wkl[k]; M.Gklcd = Gklcd[k]; M.Lldkc = Lldkc[k]; M.Lt = Lt[k];
     M.gvvov = gvvov[k]; M.gooov = gooov[k]; M.Xvv = Xvv[k]; M.Xoo = Xoo[k]; M.dvec = dvec[k];
     M.st = st[k]; M.sd = sd[k]; M.Bm = Bm[k];
+    M.Ov = Ov[k];
     M.t2mp2 = mp2[k];
   }
   b.dm = b.dalloc<Mol>(b.nm);
@@ -375,6 +368,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgNN = make_ragged(b, sz_nn);
   b.rgN3 = make_ragged(b, sz_n3);
   b.rgT2 = make_ragged(b, sz_t2);
+  b.rgT2L = make_ragged(b, sz_t2, 4);   // learned guess: 1024 amplitudes per workgroup
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
   b.rgOV = make_ragged(b, sz_ov);
   b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
@@ -575,6 +569,7 @@ static void build_delta_tables(Batch& b) {
       p.dB = delta_of(b, k, p.B.p);
       double* dc = delta_of(b, k, p.C.p);
       if (dc == nullptr) throw std::string("correction table: constant output operand");
+      if (p.dA == nullptr && p.dB == nullptr) throw std::string("correction table: both operands constant");
       p.C.p = dc;
       return p;
     }, true, 64));
@@ -616,6 +611,8 @@ struct hcc_ctx {
   double prof_ms = 0.0;
   long prof_launches = 0;
   double prof_op_ms[64] = {0};
+  // learned-guess network (FP32 parameters on the device), null until set
+  float* guess_net = nullptr;
 };
 
 namespace hcc {
@@ -756,6 +753,29 @@ static void precompute(hcc_ctx* c) {
   gemm(c, b.pre[2], 63);
   make_L(b.dm, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
+}
+
+// initial amplitudes of every molecule; the learned guess corrects the MP2 one
+static void apply_guess(hcc_ctx* c, int kind) {
+  Batch& b = *c->batch;
+  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind == HCC_GUESS_LEARNED ? HCC_GUESS_MP2 : kind, c->stream);
+  if (kind == HCC_GUESS_LEARNED) {
+    mlp_overlaps(b.dm, b.rgNN, b.nm, c->stream);
+    mlp_guess(b.dm, b.rgT2L, b.nm, c->guess_net, c->stream);
+    HCHECK(hipGetLastError());
+  }
+}
+
+bool batch_view(hcc_ctx* c, BatchView* v) {
+  if (c == nullptr || c->batch == nullptr || v == nullptr) return false;
+  const Batch& b = *c->batch;
+  v->dm = b.dm;
+  v->dst = b.dst;
+  v->nm = b.nm;
+  v->device = c->device;
+  v->stream = c->stream;
+  v->rgNN = b.rgNN;
+  return true;
 }
 
 static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const double* C, const double* F,
@@ -917,16 +937,38 @@ static int need_batch(hcc_ctx* ctx) {
   return HCC_OK;
 }
 
-int hcc_guess(hcc_ctx* ctx, int kind) {
-  if (ctx == nullptr) return HCC_E_ARG;
-  if (int r = need_batch(ctx)) return r;
-  if (kind != HCC_GUESS_REFERENCE && kind != HCC_GUESS_MP2) {
+static int check_guess(hcc_ctx* ctx, int kind) {
+  if (kind != HCC_GUESS_REFERENCE && kind != HCC_GUESS_MP2 && kind != HCC_GUESS_LEARNED) {
     ctx->err = "unsupported guess kind";
     return HCC_E_ARG;
   }
+  if (kind == HCC_GUESS_LEARNED && ctx->guess_net == nullptr) {
+    ctx->err = "no guess network set (hcc_set_guess_network)";
+    return HCC_E_STATE;
+  }
+  return HCC_OK;
+}
+
+int hcc_set_guess_network(hcc_ctx* ctx, const float* params, int count) {
+  if (ctx == nullptr || params == nullptr) return HCC_E_ARG;
+  if (count != HCC_MLP_PARAMS) {
+    ctx->err = "parameter count must be HCC_MLP_PARAMS";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  if (ctx->guess_net == nullptr) HCHECK(hipMalloc((void**)&ctx->guess_net, HCC_MLP_PARAMS * sizeof(float)));
+  HCHECK(hipMemcpy(ctx->guess_net, params, HCC_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+  API_END
+}
+
+int hcc_guess(hcc_ctx* ctx, int kind) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (int r = check_guess(ctx, kind)) return r;
   API_BEGIN
   Batch& b = *ctx->batch;
-  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind, ctx->stream);
+  apply_guess(ctx, kind);
   HCHECK(hipStreamSynchronize(ctx->stream));
   ctx->have_guess = true;
   API_END
@@ -1048,7 +1090,7 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventRecord(e0, s));
   if (build_integrals) precompute(ctx);
-  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, guess_kind, s);
+  apply_guess(ctx, guess_kind);
   Params prm = params(ctx, eps, maxiter, true);
   std::vector<int> act(b.nm);
   const int check_every = 4;
@@ -1093,7 +1135,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventRecord(e0, s));
   if (build_integrals) precompute(ctx);
-  guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, guess_kind, s);
+  apply_guess(ctx, guess_kind);
   Params prm = params(ctx, eps, maxiter, true);
   prm.kappa = ctx->kappa;
   prm.max_inner = ctx->max_inner;
@@ -1152,6 +1194,7 @@ int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double ener
               int* status) {
   if (ctx == nullptr) return HCC_E_ARG;
   if (int r = need_batch(ctx)) return r;
+  if (int r = check_guess(ctx, guess_kind)) return r;
   if (maxiter < 1) {
     ctx->err = "maxiter must be positive";
     return HCC_E_ARG;
@@ -1171,6 +1214,7 @@ int hcc_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, double ener
 int hcc_batch_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps) {
   if (ctx == nullptr) return HCC_E_ARG;
   if (int r = need_batch(ctx)) return r;
+  if (int r = check_guess(ctx, guess_kind)) return r;
   if (maxiter < 1) {
     ctx->err = "maxiter must be positive";
     return HCC_E_ARG;
@@ -1258,6 +1302,7 @@ void hcc_destroy(hcc_ctx* ctx) {
     (void)hipEventDestroy(ev.second);
   }
   delete ctx->batch;
+  if (ctx->guess_net) (void)hipFree(ctx->guess_net);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

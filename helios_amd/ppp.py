@@ -53,18 +53,32 @@ def _gauge(V):
     return V * s
 
 
-def rhf(core, G, huckel, nocc, tol=1e-11, maxiter=300, ndiis=8):
+def rhf(core, G, huckel, nocc, tol=1e-11, maxiter=300, ndiis=8, diis_start=1e-3, step=0.05):
+    """Closed-shell PPP RHF from the Hueckel guess.
+
+    Plain Roothaan steps on the commutator-damped density (the reference's
+    scheme, src/methods/mod_scf.f90:106-136 and :446-494) until the commutator
+    falls below diis_start, then Pulay DIIS on the AO Fock matrices.  DIIS
+    from the start can lock onto an oscillation in long polyenes (N >= 100).
+    """
     _, V = np.linalg.eigh(huckel)
     D = V[:, :nocc] @ V[:, :nocc].T
     Fs, Es = [], []
     for it in range(1, maxiter + 1):
         F = _fock(core, G, D)
         err = F @ D - D @ F
+        emax = np.abs(err).max()
+        if emax < tol:
+            break
+        if emax > diis_start and not Fs:
+            R = 2.0 * D - np.eye(len(D))
+            Fm = (F @ R - R @ F) @ R
+            _, V = np.linalg.eigh(_fock(core, G, D - step * Fm))
+            D = V[:, :nocc] @ V[:, :nocc].T
+            continue
         Fs.append(F)
         Es.append(err)
         Fs, Es = Fs[-ndiis:], Es[-ndiis:]
-        if np.abs(err).max() < tol:
-            break
         k = len(Fs)
         if k > 1:
             B = -np.ones((k + 1, k + 1))

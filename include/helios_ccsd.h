@@ -39,7 +39,7 @@ typedef struct hcc_ctx hcc_ctx;
 
 #define HCC_GUESS_REFERENCE 0 /* t2 = (2(ia|jb)-(ib|ja))/D, mod_cc.f90:634-650 */
 #define HCC_GUESS_MP2 1       /* t2 = (ia|jb)/D, mod_mbpt.f90:131-175 amplitudes */
-#define HCC_GUESS_LEARNED 2   /* MLP prediction from MP2 features */
+#define HCC_GUESS_LEARNED 2   /* MP2 amplitudes + learned MLP correction (hcc_set_guess_network) */
 
 /* Create a context on HIP device `device`.  precision: HCC_FP64 or HCC_MIXED.
  * diis_steps: DIIS depth (ccbd%diisSteps, src/input/definebd.f90:152), 2..32.
@@ -140,6 +140,57 @@ int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
 void hcc_destroy(hcc_ctx* ctx);
 const char* hcc_last_error(const hcc_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Learned T2 guess (no reference counterpart: it replaces the guess of
+ * guessCC, mod_cc.f90:634-650, and keeps the MP2 amplitudes of
+ * energyMBPT('mp2'), src/methods/mod_mbpt.f90:131-175, as its input and as
+ * the fallback).  A 3-layer MLP, 16 -> 64 -> 64 -> 1 with exact-erf GELU,
+ * BF16 weights and FP32 accumulation, maps 16 features of each amplitude
+ * t_ij^ab to y = Dp (t_ccsd - t_mp2), Dp = e_a + e_b - e_i - e_j; the guess
+ * is t2 = t_mp2 + y / Dp, t1 as for the MP2 guess.  Features, in order:
+ *   10 t_mp2, (ia|jb)/2, (ib|ja)/2, Dp/10, 2/Dp, (e_i+e_j-2mu)/10,
+ *   (e_a+e_b-2mu)/10, |e_i-e_j|/10, |e_a-e_b|/10, O_ij, O_ab,
+ *   (O_ia+O_jb)/2, |O_ia-O_jb|/2, (O_ib+O_ja)/2, |O_ib-O_ja|/2, 10/n
+ * with mu the HOMO-LUMO midpoint and O_pq = n sum_mu C_mu,p^2 C_mu,q^2.
+ * Parameters (FP32, HCC_MLP_PARAMS): W1[64][16], b1[64], W2[64][64], b2[64],
+ * W3[64], b3.
+ * ------------------------------------------------------------------------ */
+#define HCC_MLP_PARAMS 5313
+#define HCC_MLP_FEATURES 16
+#define HCC_MLP_ROW 18 /* training row: 16 features, target y, weight */
+
+/* Network used by HCC_GUESS_LEARNED in this context (copied to the device). */
+int hcc_set_guess_network(hcc_ctx* ctx, const float* params, int count);
+
+/* Trainer: a device-resident sample set of up to `capacity` rows, the FP32
+ * master parameters and Adam moments; training runs on the GPU. */
+typedef struct hcc_mlp hcc_mlp;
+int hcc_mlp_create(hcc_mlp** net, int device, long capacity);
+void hcc_mlp_destroy(hcc_mlp* net);
+const char* hcc_mlp_last_error(const hcc_mlp* net);
+int hcc_mlp_set_params(hcc_mlp* net, const float* params, int count);
+int hcc_mlp_get_params(hcc_mlp* net, float* params, int count);
+/* Append per_molecule hashed amplitudes of every molecule of ctx's last solve
+ * (features and target y from the converged t2; molecules that did not
+ * converge give rows of weight 0). */
+int hcc_mlp_collect(hcc_mlp* net, hcc_ctx* ctx, int per_molecule, unsigned seed, long* added);
+int hcc_mlp_samples(hcc_mlp* net, long* count);
+/* rows: count x HCC_MLP_ROW; where: count x 2 (molecule, amplitude index
+ * i*o*v*v + j*v*v + a*v + b), -1 for rows written by the host. */
+int hcc_mlp_read_samples(hcc_mlp* net, long first, long count, float* rows, int* where);
+int hcc_mlp_write_samples(hcc_mlp* net, const float* rows, long count);
+/* Mean weighted squared error and its gradient over rows [first, first+count),
+ * without an update. */
+int hcc_mlp_gradient(hcc_mlp* net, long first, int count, double* loss, float* grad);
+/* Adam (0.9, 0.999, 1e-8) over shuffled mini-batches, learning rate cosine
+ * from lr to lr_final; epoch_loss[e] = mean training loss of epoch e. */
+int hcc_mlp_train(hcc_mlp* net, int epochs, int batch, float lr, float lr_final, unsigned seed,
+                  double* epoch_loss);
+/* Mean weighted squared error over the whole sample set. */
+int hcc_mlp_evaluate(hcc_mlp* net, double* loss);
+/* y = network(features) for count rows of HCC_MLP_FEATURES values. */
+int hcc_mlp_predict(hcc_mlp* net, const float* features, long count, float* y);
 
 /* Self-test of the grouped GEMM engine on row-major A (m x k), B (k x n):
  * C = alpha A B + beta C.  No reference counterpart; used by the tests. */
