@@ -28,6 +28,11 @@ from helios_amd import capi, learned_guess, molecules, ppp  # noqa: E402
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
+# HBM bytes per gemm_f64_kernel launch in the FP64 solve of the default 4096-molecule
+# batch: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy), two rocprofv3
+# --pmc passes summarised in profiles/hbm_traffic_fp64_batch4096.md.  A counter pass
+# cannot run inside this process, so other workloads report null.
+GEMM_TRAFFIC_BATCH4096 = 4.932e9
 
 
 def _prep(k):
@@ -87,9 +92,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--ablation", action="store_true", help="also time FP64+learned and mixed+MP2")
+    ap.add_argument("--skip-reference", action="store_true",
+                    help="profiler runs only: no FP64 reference solve, solves count as converged by status alone")
     ap.add_argument("--kappa", type=float, default=0.03)
     ap.add_argument("--max-inner", type=int, default=40)
-    ap.add_argument("--max-outer", type=int, default=12)
+    ap.add_argument("--max-outer", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one MI355X; "
@@ -103,13 +110,17 @@ def main():
     v = np.array([m["n"] - m["nocc"] for m in mols])
 
     # FP64 reference energies (untimed, tight threshold)
-    ctx = capi.Context(0, capi.HCC_FP64, args.diis)
-    ctx.batch_load(mols)
-    ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
-    e_ref, it_ref, st_ref = ctx.batch_results()
-    ctx.close()
-    print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
-          f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
+    if args.skip_reference:
+        e_ref = np.zeros((len(mols), 5))
+        st_ref = np.ones(len(mols), dtype=np.int32)
+    else:
+        ctx = capi.Context(0, capi.HCC_FP64, args.diis)
+        ctx.batch_load(mols)
+        ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+        e_ref, it_ref, st_ref = ctx.batch_results()
+        ctx.close()
+        print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
+              f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
     net = learned_guess.load_params()
 
@@ -135,7 +146,7 @@ def main():
         if precision == capi.HCC_MIXED:
             outer, inner = ctx.batch_statistics()
         ctx.close()
-        de = np.abs(e[:, 0] - e_ref[:, 0])
+        de = np.abs(e[:, 0] - e_ref[:, 0]) if not args.skip_reference else np.zeros(len(mols))
         ok = (de < 1e-6) & (st == 1) & (st_ref == 1)
         step_s = float(np.mean(t_steps))
         return dict(e=e, iters=iters, st=st, de=de, ok=ok, step_s=step_s, gemm_ms=gemm_ms, launches=launches,
@@ -183,12 +194,14 @@ def main():
                                + ("mixed-precision defect-correction solver, learned T2 guess" if rmx is not None
                                   else "plain FP64 batched solver, MP2 guess"),
                    "molecules": len(mols), "residual_threshold": args.eps, "diis_steps": args.diis,
+                   "energies_checked_against_fp64_reference": not args.skip_reference,
                    "converged": int(head["ok"].sum()), "max_abs_dE_eV": float(head["de"].max()),
                    "fp64_solves_per_s": float(r64["solves"]), "fp64_ms_per_step": r64["step_s"] * 1e3,
                    "fp64_mean_iterations": float(r64["iters"].mean()),
                    "speedup_vs_fp64": float(head["solves"] / r64["solves"]) if rmx is not None else 1.0},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": GEMM_TRAFFIC_BATCH4096 if (args.nmol == 4096 and args.diis == 8) else None,
                      "kernel": "gemm_f64_kernel (FP64 solver)", "launches": int(r64["launches"]),
                      "kernel_ms": r64["gemm_ms"], "gemm_flops": gemm_flops},
         "f_iter_rate_tflops": falg / r64["step_s"] / 1e12,

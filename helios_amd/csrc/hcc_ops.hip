@@ -751,6 +751,12 @@ __global__ __launch_bounds__(kEwThreads) void k_zero_delta(const Mol* __restrict
   else dmols[mol].t1[e - n2] = 0.0;
 }
 
+// t1 = 0 for every molecule (a kernel, not per-molecule memsets)
+__global__ __launch_bounds__(kEwThreads) void k_zero_t1(const Mol* __restrict__ mols, Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v);
+  if (e < size_) M.t1[e] = 0.0;
+}
+
 // t0 += dt
 __global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
                                                             const State* __restrict__ st, Ragged rg, int nm) {
@@ -928,6 +934,9 @@ void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Rag
 }
 void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_delta, rgT2S, m0, md, st, rgT2S, nm);
+}
+void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_zero_t1, rgOV, mols, rgOV, nm);
 }
 void mixed_commit(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_commit_delta, rgT2S, m0, md, st, rgT2S, nm);

@@ -602,7 +602,7 @@ struct hcc_ctx {
   int mixed_bf16 = 1;
   double kappa = 0.03;
   int max_inner = 40;
-  int max_outer = 12;
+  int max_outer = 20;
   // live timing of the contraction GEMM launches (HIP events on `stream`)
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -743,8 +743,7 @@ static void precompute(hcc_ctx* c) {
   hipStream_t s = c->stream;
   reset_state(b.dst, b.dact, b.nm, s);
   // t1 = 0 -> undressed orbital products
-  for (int k = 0; k < b.nm; ++k)
-    HCHECK(hipMemsetAsync(b.hm[k].t1, 0, sizeof(double) * b.o[k] * b.v[k], s));
+  zero_t1(b.dm, b.rgOV, b.nm, s);
   gemm(c, b.pre[0], 60);
   gemm(c, b.pre[1], 61);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
@@ -797,22 +796,28 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   allocate(*b, keep_mp2);
   build_tables(*b);
   if (b->mixed) build_delta_tables(*b);
-  // upload, transposing the column-major inputs to row-major
-  long off = 0;
+  // upload, transposing the column-major inputs to row-major: one staging
+  // image per arena (molecule blocks at their arena offsets), one copy each
+  double* dsts0[3] = {b->hm[0].C, b->hm[0].F, b->hm[0].G};
+  const double* srcs[3] = {C, F, G};
+  const long nlast = (long)n[nmol - 1] * n[nmol - 1];
   std::vector<double> h;
-  for (int k = 0; k < nmol; ++k) {
-    const int N = n[k];
-    const Mol& M = b->hm[k];
-    h.resize((size_t)N * N);
-    const double* srcs[3] = {C + off, F + off, G + off};
-    double* dsts[3] = {M.C, M.F, M.G};
-    for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < 3; ++q) {
+    double* base = dsts0[q];
+    double* const* last = q == 0 ? &b->hm[nmol - 1].C : (q == 1 ? &b->hm[nmol - 1].F : &b->hm[nmol - 1].G);
+    const long span = (*last - base) + nlast;
+    h.assign((size_t)span, 0.0);
+    long off = 0;
+    for (int k = 0; k < nmol; ++k) {
+      const int N = n[k];
+      const Mol& M = b->hm[k];
+      double* dst = q == 0 ? M.C : (q == 1 ? M.F : M.G);
+      double* hp = h.data() + (dst - base);
       for (int r = 0; r < N; ++r)
-        for (int cc = 0; cc < N; ++cc) h[(size_t)r * N + cc] = srcs[q][r + (size_t)cc * N];
-      HCHECK(hipMemcpyAsync(dsts[q], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      HCHECK(hipStreamSynchronize(c->stream));
+        for (int cc = 0; cc < N; ++cc) hp[(size_t)r * N + cc] = srcs[q][off + r + (size_t)cc * N];
+      off += (long)N * N;
     }
-    off += (long)N * N;
+    HCHECK(hipMemcpy(base, h.data(), (size_t)span * sizeof(double), hipMemcpyHostToDevice));
   }
   c->have_guess = false;
 }

@@ -38,7 +38,36 @@ contains
         if (ctl%projType == '1') proj = 1
         call gpuCheck(hcc_set_options(ctx, real(ctl%t1Step, c_double), real(ctl%t2Step, c_double), proj, &
                                       merge(1_c_int, 0_c_int, ctl%diisEnabled)), 'hcc_set_options', ctx)
+        if (ctl%guess == 'learned') call loadGuessNetwork(ctl%guessNetwork)
     end subroutine setCCParameters
+
+    ! parameters of the learned guess: HCC_MLP_PARAMS little-endian float32 values
+    subroutine loadGuessNetwork(path)
+        character(len=*), intent(in) :: path
+        real(c_float) :: params(HCC_MLP_PARAMS)
+        integer :: u, ios
+
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='old', action='read', &
+              iostat=ios)
+        if (ios /= 0) then
+            write (*, '(A,A)') ' Error: cannot open the guess network ', trim(path)
+            error stop 'guess network missing'
+        end if
+        read (u, iostat=ios) params
+        close (u)
+        if (ios /= 0) then
+            write (*, '(A,A)') ' Error: short guess network file ', trim(path)
+            error stop 'guess network unreadable'
+        end if
+        call gpuCheck(hcc_set_guess_network(ctx, params, HCC_MLP_PARAMS), 'hcc_set_guess_network', ctx)
+    end subroutine loadGuessNetwork
+
+    integer(c_int) function guessKind(ctl)
+        type(hcontrol), intent(in) :: ctl
+        guessKind = HCC_GUESS_REFERENCE
+        if (ctl%guess == 'mp2') guessKind = HCC_GUESS_MP2
+        if (ctl%guess == 'learned') guessKind = HCC_GUESS_LEARNED
+    end function guessKind
 
     ! SCF, MO Fock and reference energy on the host, integrals and guess on the device
     subroutine initCC(ctl, mol)
@@ -55,8 +84,7 @@ contains
         call moFock(mol, hV, F, refeEnergy)
         call gpuCheck(hcc_set_molecule(ctx, int(N, c_int), int(Nocc, c_int), hV, F, mol%G, refeEnergy), &
                       'hcc_set_molecule', ctx)
-        kind = HCC_GUESS_REFERENCE
-        if (ctl%guess == 'mp2') kind = HCC_GUESS_MP2
+        kind = guessKind(ctl)
         call gpuCheck(hcc_guess(ctx, kind), 'hcc_guess', ctx)
     end subroutine initCC
 
@@ -78,8 +106,7 @@ contains
         real(kind=rglu),    intent(out) :: energy(5)
         integer(kind=iglu), intent(out) :: iterations, status
         integer(c_int) :: it, st, kind
-        kind = HCC_GUESS_REFERENCE
-        if (ctl%guess == 'mp2') kind = HCC_GUESS_MP2
+        kind = guessKind(ctl)
         call gpuCheck(hcc_solve(ctx, kind, int(ctl%ccMaxIters, c_int), real(ctl%ccAccuracy, c_double), energy, &
                                 it, st), 'hcc_solve', ctx)
         iterations = it

@@ -10,6 +10,7 @@ module ccsdGPU
 
     integer(c_int), parameter :: HCC_FP64 = 0, HCC_MIXED = 1
     integer(c_int), parameter :: HCC_GUESS_REFERENCE = 0, HCC_GUESS_MP2 = 1, HCC_GUESS_LEARNED = 2
+    integer(c_int), parameter :: HCC_MLP_PARAMS = 5313
 
     interface
         integer(c_int) function hcc_create(ctx, device, precision, diis_steps) bind(C, name='hcc_create')
@@ -25,6 +26,21 @@ module ccsdGPU
             real(c_double), value :: t1_step, t2_step
             integer(c_int), value :: projection_type, diis_enabled
         end function hcc_set_options
+
+        integer(c_int) function hcc_set_mixed(ctx, bf16, kappa, max_inner, max_outer) &
+                bind(C, name='hcc_set_mixed')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: bf16, max_inner, max_outer
+            real(c_double), value :: kappa
+        end function hcc_set_mixed
+
+        integer(c_int) function hcc_set_guess_network(ctx, params, count) bind(C, name='hcc_set_guess_network')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: params(*)
+            integer(c_int), value :: count
+        end function hcc_set_guess_network
 
         integer(c_int) function hcc_set_molecule(ctx, n, n_occ, C, F_mo, gamma, e_ref) &
                 bind(C, name='hcc_set_molecule')

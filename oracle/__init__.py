@@ -25,8 +25,15 @@ relative to the helios source tree):
                   two-electron case where CCSD is exact).
 * ``dressed``  -- the T1-dressed, ZDO-factorised residual that the GPU kernels
                   implement, written as numpy GEMMs; pinned against ``ccsd``.
-* ``ccsd_port.c`` -- a loop-for-loop C port of the reference projections with
-                  OpenMP, timed as the CPU baseline.
+* ``mlp``      -- the learned T2 guess (features, BF16-rounded MLP forward,
+                  analytic backward, Adam) restated in float64 numpy; no
+                  reference counterpart, pinned by finite differences.
+
+The reference itself (Fortran, Intel-compiler extensions, ~20 modules with
+generated includes) is not built here: it has no stand-alone source set for
+this path that AMD flang compiles without stand-in modules, so there is no
+``oracle/_ref``.  ``bench.py`` times ``ccsd.solve`` (numpy + BLAS) on the
+GPU box's host cores as the CPU baseline ("port").
 
 Pinning status of each function is recorded in the module docstrings.
 """

@@ -49,3 +49,18 @@ def test_host_rccsd_on_gpu(tmp_path, driver):
     assert "r-ccsd status 1" in out
     assert abs(energy_of(out) - ref["energy"][0]) < 1e-8
     assert abs(energy_of(out, "Reference energy") - p["eref"]) < 1e-9
+
+
+@pytest.mark.gpu
+def test_host_mixed_learned_on_gpu(tmp_path):
+    """precision=mixed with guess=learned through the Fortran host: the energy
+    stays within 1e-6 eV of the FP64 oracle."""
+    mol = M.batch_member(11)
+    net = os.path.join(os.path.dirname(EXE), "data", "guess_mlp.f32")
+    out = run(tmp_path, mol, extra=f"driver=device, precision=mixed, guess=learned, guess-network={net}, "
+                                   "diis-steps=8")
+    p = prepare(mol)
+    ref = ccsd.solve(p["C"], p["F"], p["G"], mol.nocc, p["eref"], eps=1e-10, diis_scale=True)
+    print(out[-600:])
+    assert "r-ccsd status 1" in out
+    assert abs(energy_of(out) - ref["energy"][0]) < 1e-6
