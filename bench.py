@@ -10,7 +10,8 @@ within 1e-6 eV of the FP64 reference energy of the same molecule (the FP64
 solver at a residual threshold of 1e-10, untimed).
 
 Prints one JSON line (see the keys below).  --gpus N>1 is not supported (one
-GPU per box); the multi-GPU plan is a molecule sharding described in DESIGN.md.
+GPU per box); across GPUs the batch would shard by molecule, with a final gather
+of the per-molecule results.
 """
 import argparse
 import json
@@ -100,7 +101,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one MI355X; "
-              "batches shard by molecule across GPUs (see DESIGN.md)", file=sys.stderr)
+              "across GPUs the batch would shard by molecule", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.time()
