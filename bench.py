@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--max-outer", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one MI355X; "
+        print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark runs on one MI355X; "
               "across GPUs the batch would shard by molecule", file=sys.stderr)
         sys.exit(2)
 

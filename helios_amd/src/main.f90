@@ -63,7 +63,7 @@ program helios_gpu
     do m = 1, max(ctl%nmethods, 1)
         if (ctl%nmethods > 0) then
             if (trim(ctl%methods(m)) /= 'r-ccsd') then
-                write (*, '(A,A,A)') ' method ', trim(ctl%methods(m)), ' is not part of this build; skipped'
+                write (*, '(A,A,A)') ' method ', trim(ctl%methods(m)), ' is not provided by helios_gpu; skipped'
                 cycle
             end if
         end if
