@@ -53,6 +53,9 @@ SIGNATURES = {
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
+    "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
+    "hcc_tune_gemm": (_i, [_vp, _i, _i, _i, _i, _P(_d)]),
+    "hcc_gemm_shape": (_i, [_vp, _i, _i, _P(_i)]),
     "hcc_mlp_create": (_i, [_P(_vp), _i, _l]),
     "hcc_mlp_destroy": (None, [_vp]),
     "hcc_mlp_last_error": (ctypes.c_char_p, [_vp]),
@@ -235,6 +238,22 @@ class Context:
         pre = np.zeros(nmol)
         self._check(self.lib.hcc_gemm_flops(self.h, _dp(it), _dp(pre)), "hcc_gemm_flops")
         return it, pre
+
+    def set_gemm_tuning(self, entries):
+        """entries: (op, n_occ, n_virt, tile, ksplit) rows; applies to molecules loaded afterwards."""
+        self._tune = np.ascontiguousarray(np.asarray(entries, dtype=np.int32).reshape(-1, 5))
+        self._check(self.lib.hcc_set_gemm_tuning(self.h, self._tune.shape[0], _ip(self._tune)),
+                    "hcc_set_gemm_tuning")
+
+    def tune_gemm(self, op, tile=0, ksplit=0, reps=5):
+        ms = ctypes.c_double()
+        self._check(self.lib.hcc_tune_gemm(self.h, op, tile, ksplit, reps, ctypes.byref(ms)), "hcc_tune_gemm")
+        return ms.value
+
+    def gemm_shape(self, op, mol=0):
+        mnk = np.zeros(3, dtype=np.int32)
+        self._check(self.lib.hcc_gemm_shape(self.h, op, mol, _ip(mnk)), "hcc_gemm_shape")
+        return tuple(int(x) for x in mnk)
 
     def set_guess_network(self, params):
         """Parameters (MLP_PARAMS float32) of the network used by GUESS_LEARNED."""

@@ -19,6 +19,29 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+// Operand pointers arrive as generic pointers inside GemmProb; loads through
+// them compile to FLAT instructions, which also count against the LDS
+// counter, so waiting for the LDS fragment reads would wait for the next
+// slab's prefetch as well.  Global-address-space views give GLOBAL loads that
+// stay in flight across the matrix-core work of the current slab.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ const gdouble* gptr(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+
+// Operand loads through buffer resources: a masked (padding) element uses a
+// byte offset beyond the resource's range and reads 0 in hardware, so no
+// select follows the load and nothing has to wait for it before the next
+// slab's store to LDS.  Valid offsets stay below 2 GiB (the largest operand is
+// far smaller).
+constexpr int kRsrcBytes = 0x7ffffff0;
+constexpr int kOobOffset = 0x7ffffff8;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const double* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
+}
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
 __device__ __forceinline__ int row_off(const Operand& o, int r) {
   return (r / o.rr) * o.srh + (r % o.rr) * o.srl;
 }
@@ -110,8 +133,7 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
     kqb[i] = BNF ? q / BN : q % kBK;
     fb[i] = n < P.N ? col_off(P.B, n) : -1;
   }
-  const double* __restrict__ A = P.A.p;
-  const double* __restrict__ B = P.B.p;
+  const __amdgpu_buffer_rsrc_t rA = rsrc(P.A.p), rB = rsrc(P.B.p);
   double ra[TL::kNA], rb[TL::kNB];
 
   d4 acc[2][2];
@@ -128,21 +150,20 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
       koffB[q] = row_off(P.B, c0 + q);
     }
     __syncthreads();
-    // branch-free staging: padded elements load the operand base and are zeroed
+    // staging loads: padded elements read 0 through the out-of-range offset
+    const int klast = cend - c0 - 1;
     auto load = [&](int k0) {
 #pragma unroll
       for (int i = 0; i < TL::kNA; ++i) {
         const int kl = k0 - c0 + kqa[i];
-        const bool ok = fa[i] >= 0 && kl < cend - c0;
-        const double x = A[ok ? fa[i] + koffA[ok ? kl : 0] : 0];
-        ra[i] = ok ? x : 0.0;
+        const int kv = koffA[min(kl, klast)];
+        ra[i] = bload(rA, fa[i] >= 0 && kl <= klast ? (fa[i] + kv) * 8 : kOobOffset);
       }
 #pragma unroll
       for (int i = 0; i < TL::kNB; ++i) {
         const int kl = k0 - c0 + kqb[i];
-        const bool ok = fb[i] >= 0 && kl < cend - c0;
-        const double x = B[ok ? fb[i] + koffB[ok ? kl : 0] : 0];
-        rb[i] = ok ? x : 0.0;
+        const int kv = koffB[min(kl, klast)];
+        rb[i] = bload(rB, fb[i] >= 0 && kl <= klast ? (fb[i] + kv) * 8 : kOobOffset);
       }
     };
     load(c0);
@@ -181,7 +202,7 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
   // epilogue: for the FP64 MFMA, D[row][col] of a 16x16 block sits in lane
   // (col + 16*(row%4)), element row/4
   if (P.ksplit > 1) {
-    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
@@ -196,7 +217,7 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
     }
     return;
   }
-  double* __restrict__ Cp = P.C.p;
+  gdouble* __restrict__ Cp = gptr(P.C.p);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
@@ -230,8 +251,8 @@ constexpr int kLd16 = kBK16 + 8;   // bf16 elements per LDS row (k contiguous)
 
 __device__ __forceinline__ void load_slab16(const GemmProb& P, int m0, int n0, int k0, int t, double (&a)[8],
                                             double (&b)[8]) {
-  const double* __restrict__ A = P.A.p;
-  const double* __restrict__ B = P.B.p;
+  const gdouble* __restrict__ A = gptr(P.A.p);
+  const gdouble* __restrict__ B = gptr(P.B.p);
   if (P.a_kfast) {
     const int m = m0 + (t >> 2);
     const int kb = k0 + (t & 3) * 8;
@@ -329,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __r
     __syncthreads();
   }
   if (P.ksplit > 1) {
-    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
@@ -344,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __r
     }
     return;
   }
-  double* __restrict__ Cp = P.C.p;
+  gdouble* __restrict__ Cp = gptr(P.C.p);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
@@ -426,10 +447,10 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
     kqb[i] = BNF ? q / BN : q % BK;
     fb[i] = n < P.N ? col_off(P.B, n) : -1;
   }
-  const double* __restrict__ A0 = P.A.p;
-  const double* __restrict__ dA = hasA ? P.dA : P.A.p;
-  const double* __restrict__ B0 = P.B.p;
-  const double* __restrict__ dB = hasB ? P.dB : P.B.p;
+  const gdouble* __restrict__ A0 = gptr(P.A.p);
+  const gdouble* __restrict__ dA = gptr(hasA ? P.dA : P.A.p);
+  const gdouble* __restrict__ B0 = gptr(P.B.p);
+  const gdouble* __restrict__ dB = gptr(hasB ? P.dB : P.B.p);
   double ra0[NE], rda[NE], rdb[NE], rbf[NE];
   auto load = [&](int k0) {
 #pragma unroll
@@ -489,7 +510,7 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
     __syncthreads();
   }
   if (P.ksplit > 1) {
-    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
     }
     return;
   }
-  double* __restrict__ Cp = P.C.p;
+  gdouble* __restrict__ Cp = gptr(P.C.p);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
@@ -563,12 +584,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     fb[i] = n0 + nl < P.N ? col_off(P.B, n0 + nl) : -1;
     sb[i] = nl * LD + kqb[i];
   }
-  const double* __restrict__ A0 = P.A.p;
-  const double* __restrict__ dA = P.dA;
-  const double* __restrict__ B0 = P.B.p;
-  const double* __restrict__ dB = P.dB;
-  // staged values: A0 and dB feed A0 dB (needs HB), dA and B0 + dB feed dA (B0 + dB) (needs HA)
-  float ra0[NE], rda[NE], rdb[NE], rbf[NE];
+  const __amdgpu_buffer_rsrc_t rA0 = rsrc(P.A.p), rdA = rsrc(HA ? P.dA : P.A.p);
+  const __amdgpu_buffer_rsrc_t rB0 = rsrc(P.B.p), rdB = rsrc(HB ? P.dB : P.B.p);
+  // raw FP64 staging registers (rounded to BF16 at the LDS store, so the
+  // loads stay in flight across the current slab's matrix-core work):
+  // A0 and dB feed A0 dB (needs HB), dA and B0 + dB feed dA (B0 + dB) (needs HA)
+  double ra0[NE], rda[NE], rdb[NE], rb0[NE];
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -582,21 +603,19 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
       koffB[q] = row_off(P.B, c0 + q);
     }
     __syncthreads();
-    // branch-free staging: padded elements read the operand base and are zeroed
+    // staging loads: padded elements read 0 through the out-of-range offset
+    const int klast = cend - c0 - 1;
     auto load = [&](int k0) {
 #pragma unroll
       for (int i = 0; i < NE; ++i) {
         int kl = k0 - c0 + kqa[i];
-        bool ok = fa[i] >= 0 && kl < cend - c0;
-        int o = ok ? fa[i] + koffA[ok ? kl : 0] : 0;
-        if (HB) ra0[i] = ok ? (float)A0[o] : 0.f;
-        if (HA) rda[i] = ok ? (float)dA[o] : 0.f;
+        int o = fa[i] >= 0 && kl <= klast ? (fa[i] + koffA[min(kl, klast)]) * 8 : kOobOffset;
+        if (HB) ra0[i] = bload(rA0, o);
+        if (HA) rda[i] = bload(rdA, o);
         kl = k0 - c0 + kqb[i];
-        ok = fb[i] >= 0 && kl < cend - c0;
-        o = ok ? fb[i] + koffB[ok ? kl : 0] : 0;
-        const double db = HB ? dB[o] : 0.0;
-        if (HB) rdb[i] = ok ? (float)db : 0.f;
-        if (HA) rbf[i] = ok ? (float)(B0[o] + db) : 0.f;
+        o = fb[i] >= 0 && kl <= klast ? (fb[i] + koffB[min(kl, klast)]) * 8 : kOobOffset;
+        if (HB) rdb[i] = bload(rdB, o);
+        if (HA) rb0[i] = bload(rB0, o);
       }
     };
     load(c0);
@@ -604,12 +623,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
 #pragma unroll
       for (int i = 0; i < NE; ++i) {
         if (HB) {
-          A0s[sa[i]] = (__bf16)ra0[i];
-          dBs[sb[i]] = (__bf16)rdb[i];
+          A0s[sa[i]] = (__bf16)(float)ra0[i];
+          dBs[sb[i]] = (__bf16)(float)rdb[i];
         }
         if (HA) {
-          dAs[sa[i]] = (__bf16)rda[i];
-          Bfs[sb[i]] = (__bf16)rbf[i];
+          dAs[sa[i]] = (__bf16)(float)rda[i];
+          Bfs[sb[i]] = (__bf16)(float)(rb0[i] + (HB ? rdb[i] : 0.0));
         }
       }
       __syncthreads();
@@ -640,7 +659,7 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     }
   }
   if (P.ksplit > 1) {
-    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
+    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
@@ -655,7 +674,7 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     }
     return;
   }
-  double* __restrict__ Cp = P.C.p;
+  gdouble* __restrict__ Cp = gptr(P.C.p);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);

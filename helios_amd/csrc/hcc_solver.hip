@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -51,6 +53,7 @@ struct GemmTable {
   int* red_bprob = nullptr;    // problem index of every reduction block
   int red_blocks = 0;
   int tile = 64;               // output tile edge (64 or 32)
+  int alt = -1;                // second part of the same contraction (other tile edge), index in Batch::alt
   int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3
 };
 
@@ -98,6 +101,9 @@ struct Batch {
   Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgT2L;
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
+  std::vector<GemmTable> alt;   // second (other tile edge) parts of split contraction tables
+  // measured tile/split choices per (contraction, n_occ, n_virt); null: cost heuristic
+  const std::map<std::array<int, 3>, std::array<int, 2>>* tuning = nullptr;
   double *d1out = nullptr, *d2out = nullptr;
   // mixed precision: correction buffers and correction-form GEMM tables
   bool mixed = false;
@@ -149,44 +155,43 @@ static Ragged make_ragged(Batch& b, const std::function<long(int, int, int)>& si
   return r;
 }
 
-// Tile edge for a table: 32x32 tiles (one wave) only when the padding of
+// Tile edge of one problem: 32x32 tiles (one wave) only when the padding of
 // 64x64 tiles would cost more than the (measured, about 2.5x) lower per-flop
 // efficiency of the small tile.
-static int choose_tile(const std::vector<GemmProb>& probs) {
-  double c64 = 0.0, c32 = 0.0;
-  for (const auto& p : probs) {
-    const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
-    const double t32 = (double)((p.M + 31) / 32) * ((p.N + 31) / 32) * 32 * 32;
-    c64 += t64 * p.K;
-    c32 += 2.5 * t32 * p.K;
-  }
-  return c32 < c64 ? 32 : 64;
+static int choose_tile(const GemmProb& p) {
+  const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
+  const double t32 = (double)((p.M + 31) / 32) * ((p.N + 31) / 32) * 32 * 32;
+  return 2.5 * t32 < t64 ? 32 : 64;
 }
 
-static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
-                            int force_tile = 0) {
-  std::vector<GemmProb> probs;
+// K split of a problem into `ks` chunks (multiples of 32, both K tiles)
+static void force_split(GemmProb& p, int ks) {
+  p.ksplit = 1;
+  p.kchunk = p.K;
+  if (ks <= 1) return;
+  int chunk = (p.K + ks - 1) / ks;
+  chunk = (chunk + 31) / 32 * 32;
+  p.kchunk = chunk;
+  p.ksplit = (p.K + chunk - 1) / chunk;
+  if (p.ksplit <= 1) {
+    p.ksplit = 1;
+    p.kchunk = p.K;
+  }
+}
+
+// One launchable table from problems that share the tile edge `ts` (their
+// ksplit / kchunk already chosen).
+static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
   int tiles = 0;
   long ws_total = 0;
-  std::vector<GemmProb> raw;
-  for (int k = 0; k < b.nm; ++k) {
-    GemmProb p = build(k);
-    if (p.M <= 0 || p.N <= 0) continue;
-    p.mol = k;
-    raw.push_back(p);
-  }
-  const int ts = force_tile ? force_tile : choose_tile(raw);
-  for (GemmProb p : raw) {
+  for (GemmProb& p : probs) {
     p.tiles_n = (p.N + ts - 1) / ts;
     const int tm = (p.M + ts - 1) / ts;
-    if (allow_split) choose_split(p, ts);
-    else { p.ksplit = 1; p.kchunk = p.K; }
     p.tile_begin = tiles;
     tiles += tm * p.tiles_n * p.ksplit;
     p.a_kfast = unit_cols(p.A) ? 1 : (unit_rows(p.A) ? 0 : 1);
     p.b_nfast = unit_cols(p.B) ? 1 : (unit_rows(p.B) ? 0 : 1);
     if (p.ksplit > 1) ws_total += (long)p.ksplit * p.M * p.N;
-    probs.push_back(p);
   }
   GemmTable t;
   t.nprob = (int)probs.size();
@@ -237,6 +242,44 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
   if (t.nprob) {
     t.d = b.dalloc<GemmProb>(probs.size());
     HCHECK(hipMemcpy(t.d, probs.data(), probs.size() * sizeof(GemmProb), hipMemcpyHostToDevice));
+  }
+  return t;
+}
+
+// Table of one contraction over every molecule.  Each problem gets its tile
+// edge and K split from the measured tuning of its (op, n_occ, n_virt) when
+// there is one, else from the cost heuristic (choose_tile / choose_split);
+// problems with 32x32 tiles go to a second launch (Batch::alt).
+// force_tile / force_ks (> 0) override both for every problem.
+static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
+                            int force_tile = 0, int op = -1, int force_ks = 0) {
+  std::vector<GemmProb> g64, g32;
+  for (int k = 0; k < b.nm; ++k) {
+    GemmProb p = build(k);
+    if (p.M <= 0 || p.N <= 0) continue;
+    p.mol = k;
+    int ts = 0, ks = -1;
+    if (force_tile) {
+      ts = force_tile;
+      if (force_ks > 0) ks = force_ks;
+    } else if (b.tuning && op >= 0) {
+      auto it = b.tuning->find({op, b.o[k], b.v[k]});
+      if (it != b.tuning->end()) {
+        ts = it->second[0];
+        ks = it->second[1];
+      }
+    }
+    if (ts != 32 && ts != 64) ts = choose_tile(p);
+    if (!allow_split) force_split(p, 1);
+    else if (ks >= 1) force_split(p, ks);
+    else choose_split(p, ts);
+    (ts == 32 ? g32 : g64).push_back(p);
+  }
+  if (g64.empty()) return build_group(b, g32, 32);
+  GemmTable t = build_group(b, g64, 64);
+  if (!g32.empty()) {
+    b.alt.push_back(build_group(b, g32, 32));
+    t.alt = (int)b.alt.size() - 1;
   }
   return t;
 }
@@ -414,7 +457,7 @@ static void build_tables(Batch& b) {
   b.ops.clear();
   auto add = [&](const std::function<GemmProb(const Mol&)>& f) {
     b.ops.push_back(f);
-    b.it.push_back(make_table(b, [&](int k) { return f(H[k]); }));
+    b.it.push_back(make_table(b, [&](int k) { return f(H[k]); }, true, 0, (int)b.it.size()));
   };
   // g1: Q = gamma P
   add([](const Mol& M) {
@@ -613,6 +656,8 @@ struct hcc_ctx {
   double prof_op_ms[64] = {0};
   // learned-guess network (FP32 parameters on the device), null until set
   float* guess_net = nullptr;
+  // measured GEMM tile / K-split choices, (op, n_occ, n_virt) -> (tile edge, ksplit)
+  std::map<std::array<int, 3>, std::array<int, 2>> tuning;
 };
 
 namespace hcc {
@@ -621,6 +666,7 @@ static void launch_table(hcc_ctx* c, const GemmTable& t) {
   HCHECK(launch_gemm_f64(t.d, t.tile_prob, t.tiles, c->batch->dact, c->stream, t.tile, t.flags));
   if (t.red_blocks > 0)
     HCHECK(launch_splitk_reduce(t.d, t.red_prefix, t.red_bprob, t.red_blocks, c->batch->dact, c->stream));
+  if (t.alt >= 0) launch_table(c, c->batch->alt[t.alt]);
 }
 
 static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
@@ -715,9 +761,11 @@ static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
     ev = &c->prof_events[c->prof_used++].first;
     HCHECK(hipEventRecord(ev[0], c->stream));
   }
-  HCHECK(launch_gemm_dual(t.d, t.tile_prob, t.tiles, c->batch->dact, c->stream, t.flags, c->mixed_bf16));
-  if (t.red_blocks > 0)
-    HCHECK(launch_splitk_reduce(t.d, t.red_prefix, t.red_bprob, t.red_blocks, c->batch->dact, c->stream));
+  for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr) {
+    HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16));
+    if (p->red_blocks > 0)
+      HCHECK(launch_splitk_reduce(p->d, p->red_prefix, p->red_bprob, p->red_blocks, c->batch->dact, c->stream));
+  }
   if (ev) HCHECK(hipEventRecord(c->prof_events[c->prof_used - 1].second, c->stream));
 }
 
@@ -786,6 +834,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   b->nm = nmol;
   b->m = c->diis_steps;
   b->mixed = c->precision == HCC_MIXED;
+  b->tuning = c->tuning.empty() ? nullptr : &c->tuning;
   for (int k = 0; k < nmol; ++k) {
     if (n[k] < 2 || nocc[k] < 1 || nocc[k] >= n[k]) throw std::string("invalid molecule dimensions");
     b->n.push_back(n[k]);
@@ -1276,18 +1325,85 @@ int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches) {
   return HCC_OK;
 }
 
+int hcc_set_gemm_tuning(hcc_ctx* ctx, int count, const int* entries) {
+  if (ctx == nullptr || count < 0 || (count > 0 && entries == nullptr)) return HCC_E_ARG;
+  std::map<std::array<int, 3>, std::array<int, 2>> m;
+  for (int q = 0; q < count; ++q) {
+    const int* e = entries + 5 * q;
+    if (e[0] < 0 || e[1] < 1 || e[2] < 1 || (e[3] != 32 && e[3] != 64) || e[4] < 1 || e[4] > 256) {
+      ctx->err = "invalid tuning entry (op, n_occ, n_virt, tile 32|64, ksplit 1..256)";
+      return HCC_E_ARG;
+    }
+    m[{e[0], e[1], e[2]}] = {e[3], e[4]};
+  }
+  ctx->tuning.swap(m);
+  return HCC_OK;
+}
+
+int hcc_gemm_shape(hcc_ctx* ctx, int op, int mol, int* mnk) {
+  if (ctx == nullptr || mnk == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  Batch& b = *ctx->batch;
+  if (op < 0 || op >= (int)b.ops.size() || mol < 0 || mol >= b.nm) {
+    ctx->err = "invalid op or molecule";
+    return HCC_E_ARG;
+  }
+  const GemmProb p = b.ops[op](b.hm[mol]);
+  mnk[0] = p.M;
+  mnk[1] = p.N;
+  mnk[2] = p.K;
+  return HCC_OK;
+}
+
+int hcc_tune_gemm(hcc_ctx* ctx, int op, int tile, int ksplit, int reps, double* ms) {
+  if (ctx == nullptr || ms == nullptr || reps < 1) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  Batch& b = *ctx->batch;
+  if (op < 0 || op >= (int)b.ops.size() || (tile != 0 && tile != 32 && tile != 64) || ksplit < 0) {
+    ctx->err = "invalid tuning request";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  const size_t nalloc = b.allocs.size(), nalt = b.alt.size();
+  GemmTable t = tile == 0 ? b.it[op]
+                          : make_table(b, [&](int k) { return b.ops[op](b.hm[k]); }, true, tile, -1,
+                                       ksplit > 0 ? ksplit : 1);
+  reset_state(b.dst, b.dact, b.nm, ctx->stream);   // every molecule active
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  launch_table(ctx, t);
+  HCHECK(hipEventRecord(e0, ctx->stream));
+  for (int q = 0; q < reps; ++q) launch_table(ctx, t);
+  HCHECK(hipEventRecord(e1, ctx->stream));
+  HCHECK(hipEventSynchronize(e1));
+  float el = 0.f;
+  HCHECK(hipEventElapsedTime(&el, e0, e1));
+  *ms = el / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t q = nalloc; q < b.allocs.size(); ++q) HCHECK(hipFree(b.allocs[q]));
+  b.allocs.resize(nalloc);
+  b.alt.resize(nalt);
+  ctx->have_guess = false;
+  API_END
+}
+
 int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup) {
   if (ctx == nullptr) return HCC_E_ARG;
   if (int r = need_batch(ctx)) return r;
   std::vector<GemmProb> h;
-  auto table_flops = [&](const GemmTable& t, std::vector<double>& acc) {
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  std::function<void(const GemmTable&, std::vector<double>&)> table_flops = [&](const GemmTable& t,
+                                                                                std::vector<double>& acc) {
     h.resize(t.nprob);
     if (t.nprob)
       HCHECK(hipMemcpy(h.data(), t.d, t.nprob * sizeof(GemmProb), hipMemcpyDeviceToHost));
     for (const auto& p : h) acc[p.mol] += 2.0 * p.M * (double)p.N * p.K;
+    if (t.alt >= 0) table_flops(b.alt[t.alt], acc);
   };
-  API_BEGIN
-  Batch& b = *ctx->batch;
   std::vector<double> it(b.nm, 0.0), pre(b.nm, 0.0);
   for (const auto& t : b.it) table_flops(t, it);
   for (const auto& t : b.pre) table_flops(t, pre);

@@ -138,6 +138,19 @@ int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup)
  * of the iteration (0..N-1 in launch order) and 60..63 for the set-up GEMMs. */
 int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
+/* GEMM autotuning (no reference counterpart).  A contraction `op` (0..23, the
+ * iteration's grouped GEMMs in launch order) of a molecule with (n_occ, n_virt)
+ * runs with the tile edge (32 or 64) and K split of a matching entry; molecules
+ * without one use the cost heuristic.  entries: count x 5 ints
+ * (op, n_occ, n_virt, tile, ksplit); applies to molecules loaded afterwards.
+ * hcc_tune_gemm times one contraction over the loaded batch (reps launches,
+ * mean ms per launch) with a forced tile and ksplit (tile 0: the current
+ * choice); it overwrites that contraction's output buffers. */
+int hcc_set_gemm_tuning(hcc_ctx* ctx, int count, const int* entries);
+int hcc_tune_gemm(hcc_ctx* ctx, int op, int tile, int ksplit, int reps, double* ms);
+/* M, N, K of contraction `op` for loaded molecule `mol`. */
+int hcc_gemm_shape(hcc_ctx* ctx, int op, int mol, int* mnk);
+
 void hcc_destroy(hcc_ctx* ctx);
 const char* hcc_last_error(const hcc_ctx* ctx);
 
