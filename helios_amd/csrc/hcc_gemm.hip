@@ -70,36 +70,45 @@ __device__ __forceinline__ int logical_tile(int total) {
 // LDS tables (chunks up to kKTab), so the slab loop does no integer division.
 // ---------------------------------------------------------------------------
 constexpr int kKTab = 2048;
-constexpr int kPk = kBK + 1;   // pitch of the [row][k] staging layout
+constexpr int kPk = kBK + 1;   // pitch of the [row][k] staging layout (16-deep slabs)
 
-template <int BM, int BN>
+// K-contiguous operands are staged [k / 16][row][k % 16] with a 17-double
+// row pitch (16-deep panels), whatever the slab depth BK.
+template <int BM, int BN, int BK>
 struct F64Tile {
   static constexpr int kWaves = (BM / 32) * (BN / 32);
   static constexpr int kT = 64 * kWaves;
   static constexpr int kPa = BM + 2;          // pitch of the [k][m] layout
   static constexpr int kPb = BN + 2;
-  static constexpr int kLdsA = (BM * kPk > kBK * kPa) ? BM * kPk : kBK * kPa;
-  static constexpr int kLdsB = (BN * kPk > kBK * kPb) ? BN * kPk : kBK * kPb;
-  static constexpr int kNA = BM * kBK / kT;   // A elements per thread per slab
-  static constexpr int kNB = BN * kBK / kT;
+  static constexpr int kLdsA = ((BK / 16) * BM * kPk > BK * kPa) ? (BK / 16) * BM * kPk : BK * kPa;
+  static constexpr int kLdsB = ((BK / 16) * BN * kPk > BK * kPb) ? (BK / 16) * BN * kPk : BK * kPb;
+  // offset of (row, k) in the K-contiguous layout of an R-row operand
+  template <int R>
+  static __device__ __forceinline__ int rk(int row, int k) { return (k >> 4) * (R * kPk) + row * kPk + (k & 15); }
+  static constexpr int kNA = BM * BK / kT;    // A elements per thread per slab
+  static constexpr int kNB = BN * BK / kT;
+  // K-offset table length (LDS) per chunk: small enough that the table does
+  // not limit the workgroups per CU
+  static constexpr int kKT = BM >= 64 ? 1024 : 512;
 };
 
 #define HCC_F64_THREADS(BM_, BN_) (64 * ((BM_) / 32) * ((BN_) / 32))
 
 // AKF: A's unit stride runs along K; BNF: B's unit stride runs along N.  Both
 // are template parameters so the slab loop is straight-line code and the
-// accumulators stay in AGPRs across iterations.
-template <int BM, int BN, bool AKF, bool BNF>
+// accumulators stay in AGPRs across iterations.  BK: slab depth (16 or 32;
+// 32 halves the barriers per K for long contractions).
+template <int BM, int BN, int BK, bool AKF, bool BNF>
 __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const GemmProb* __restrict__ probs,
                                                                            const int* __restrict__ tile_prob,
                                                                            int total_tiles,
                                                                            const int* __restrict__ active) {
-  using TL = F64Tile<BM, BN>;
+  using TL = F64Tile<BM, BN, BK>;
   constexpr int T = TL::kT;
   __shared__ double As[TL::kLdsA];
   __shared__ double Bs[TL::kLdsB];
-  __shared__ int koffA[kKTab];
-  __shared__ int koffB[kKTab];
+  __shared__ int koffA[TL::kKT];
+  __shared__ int koffB[TL::kKT];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
   const GemmProb P = probs[find_problem(tile_prob, tile)];
@@ -122,15 +131,15 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
 #pragma unroll
   for (int i = 0; i < TL::kNA; ++i) {
     const int q = t + i * T;
-    const int m = m0 + (AKF ? q / kBK : q % BM);
-    kqa[i] = AKF ? q % kBK : q / BM;
+    const int m = m0 + (AKF ? q / BK : q % BM);
+    kqa[i] = AKF ? q % BK : q / BM;
     fa[i] = m < P.M ? row_off(P.A, m) : -1;
   }
 #pragma unroll
   for (int i = 0; i < TL::kNB; ++i) {
     const int q = t + i * T;
-    const int n = n0 + (BNF ? q % BN : q / kBK);
-    kqb[i] = BNF ? q / BN : q % kBK;
+    const int n = n0 + (BNF ? q % BN : q / BK);
+    kqb[i] = BNF ? q / BN : q % BK;
     fb[i] = n < P.N ? col_off(P.B, n) : -1;
   }
   const __amdgpu_buffer_rsrc_t rA = rsrc(P.A.p), rB = rsrc(P.B.p);
@@ -142,8 +151,8 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  for (int c0 = kbeg; c0 < kend; c0 += kKTab) {
-    const int cend = min(kend, c0 + kKTab);
+  for (int c0 = kbeg; c0 < kend; c0 += TL::kKT) {
+    const int cend = min(kend, c0 + TL::kKT);
     __syncthreads();
     for (int q = t; q < cend - c0; q += T) {
       koffA[q] = col_off(P.A, c0 + q);
@@ -167,29 +176,29 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
       }
     };
     load(c0);
-    for (int k0 = c0; k0 < cend; k0 += kBK) {
+    for (int k0 = c0; k0 < cend; k0 += BK) {
 #pragma unroll
       for (int i = 0; i < TL::kNA; ++i) {
         const int q = t + i * T;
-        if (AKF) As[(q / kBK) * kPk + q % kBK] = ra[i];
+        if (AKF) As[TL::template rk<BM>(q / BK, q % BK)] = ra[i];
         else As[(q / BM) * TL::kPa + q % BM] = ra[i];
       }
 #pragma unroll
       for (int i = 0; i < TL::kNB; ++i) {
         const int q = t + i * T;
         if (BNF) Bs[(q / BN) * TL::kPb + q % BN] = rb[i];
-        else Bs[(q / kBK) * kPk + q % kBK] = rb[i];
+        else Bs[TL::template rk<BN>(q / BK, q % BK)] = rb[i];
       }
       __syncthreads();
-      load(min(k0 + kBK, cend));   // the last prefetch is fully masked
+      load(min(k0 + BK, cend));   // the last prefetch is fully masked
 #pragma unroll
-      for (int kk = 0; kk < kBK; kk += 4) {
+      for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4);
         const int ma = wm + (lane & 15), nb = wn + (lane & 15);
-        const double a0 = AKF ? As[ma * kPk + kr] : As[kr * TL::kPa + ma];
-        const double a1 = AKF ? As[(ma + 16) * kPk + kr] : As[kr * TL::kPa + ma + 16];
-        const double b0 = BNF ? Bs[kr * TL::kPb + nb] : Bs[nb * kPk + kr];
-        const double b1 = BNF ? Bs[kr * TL::kPb + nb + 16] : Bs[(nb + 16) * kPk + kr];
+        const double a0 = AKF ? As[TL::template rk<BM>(ma, kr)] : As[kr * TL::kPa + ma];
+        const double a1 = AKF ? As[TL::template rk<BM>(ma + 16, kr)] : As[kr * TL::kPa + ma + 16];
+        const double b0 = BNF ? Bs[kr * TL::kPb + nb] : Bs[TL::template rk<BN>(nb, kr)];
+        const double b1 = BNF ? Bs[kr * TL::kPb + nb + 16] : Bs[TL::template rk<BN>(nb + 16, kr)];
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -753,15 +762,15 @@ hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const 
   return hipGetLastError();
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int BK>
 static void launch_f64(const GemmProb* probs, const int* tp, int total_tiles, const int* active, hipStream_t s,
                        int grid, int flags) {
   constexpr int T = HCC_F64_THREADS(BM, BN);
   switch (flags & 3) {
-    case 0: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
-    case 1: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
-    case 2: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
-    default: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    case 0: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    case 1: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, false>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    case 2: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, false, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
+    default: hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, true, true>), dim3(grid), dim3(T), 0, s, probs, tp, total_tiles, active); break;
   }
 }
 
@@ -769,8 +778,8 @@ hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int tota
                            hipStream_t s, int tile, int flags) {
   if (total_tiles <= 0) return hipSuccess;
   int grid = ((total_tiles + 7) / 8) * 8;
-  if (tile == 32) launch_f64<32, 32>(probs, tile_prob, total_tiles, active, s, grid, flags);
-  else launch_f64<64, 64>(probs, tile_prob, total_tiles, active, s, grid, flags);
+  if (tile == 32) launch_f64<32, 32, 16>(probs, tile_prob, total_tiles, active, s, grid, flags);
+  else launch_f64<64, 64, 16>(probs, tile_prob, total_tiles, active, s, grid, flags);
   return hipGetLastError();
 }
 
