@@ -2,8 +2,8 @@
 
 For every size class of the batched workload (half filling, n_occ = n_virt = k)
 and every contraction of the iteration, times the grouped GEMM over a uniform
-batch of molecules of that size with each candidate tile edge (32, 64) and K
-split, and keeps the fastest when it beats the cost heuristic by more than 3 %:
+batch of molecules of that size with each candidate tile (32x32, 64x64,
+128x32) and K split, and keeps the fastest when it beats the cost heuristic by more than 3 %:
 
     python -m helios_amd.autotune            # on a GPU; writes data/gemm_tuning.json
 
@@ -38,8 +38,11 @@ def _prep(args):
 
 def candidates(M, N, K):
     out = []
-    for tile in (32, 64):
-        tiles = -(-M // tile) * -(-N // tile)
+    for tile in (32, 64, 128):
+        if tile == 128 and (N > 32 or M < 128):
+            continue
+        tm, tn = (128, 32) if tile == 128 else (tile, tile)
+        tiles = -(-M // tm) * -(-N // tn)
         for ks in KSPLITS:
             if ks > 1 and (K // ks < 64 or tiles * ks > 4096):
                 continue

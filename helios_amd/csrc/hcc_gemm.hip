@@ -779,6 +779,7 @@ hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int tota
   if (total_tiles <= 0) return hipSuccess;
   int grid = ((total_tiles + 7) / 8) * 8;
   if (tile == 32) launch_f64<32, 32, 16>(probs, tile_prob, total_tiles, active, s, grid, flags);
+  else if (tile == 128) launch_f64<128, 32, 16>(probs, tile_prob, total_tiles, active, s, grid, flags);
   else launch_f64<64, 64, 16>(probs, tile_prob, total_tiles, active, s, grid, flags);
   return hipGetLastError();
 }

@@ -59,7 +59,8 @@ struct GemmProb {
 constexpr int kBM = 64, kBN = 64, kBK = 16, kThreads = 256;
 
 // launches one grouped GEMM over all problems of a table (device memory)
-// tile: 64 (64x64 tiles, 256 threads) or 32 (32x32 tiles, 64 threads);
+// tile: 64 (64x64 tiles, 256 threads), 32 (32x32 tiles, 64 threads) or 128
+// (128x32 tiles, 256 threads, for outputs at most 32 columns wide);
 // flags: bit 0 = A's unit stride along K, bit 1 = B's unit stride along N
 // (the same for every problem of a table)
 // tile_prob: problem index of every global tile of the table

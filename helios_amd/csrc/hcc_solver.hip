@@ -57,10 +57,15 @@ struct GemmTable {
   int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3
 };
 
+// Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
+// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide).
+static int tile_m(int kind) { return kind == 128 ? 128 : kind; }
+static int tile_n(int kind) { return kind == 128 ? 32 : kind; }
+
 // Split K when a problem has few output tiles and a long K: aim for >= 16
 // work items per problem with K chunks of at least 128.
 static void choose_split(GemmProb& p, int ts) {
-  const int tmn = ((p.M + ts - 1) / ts) * ((p.N + ts - 1) / ts);
+  const int tmn = ((p.M + tile_m(ts) - 1) / tile_m(ts)) * ((p.N + tile_n(ts) - 1) / tile_n(ts));
   p.ksplit = 1;
   p.kchunk = p.K;
   if (tmn >= 16 || p.K < 512) return;
@@ -161,6 +166,8 @@ static Ragged make_ragged(Batch& b, const std::function<long(int, int, int)>& si
 static int choose_tile(const GemmProb& p) {
   const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
   const double t32 = (double)((p.M + 31) / 32) * ((p.N + 31) / 32) * 32 * 32;
+  const double t128 = (double)((p.M + 127) / 128) * ((p.N + 31) / 32) * 128 * 32;
+  if (p.N <= 32 && p.M >= 128 && t128 < t64 && t128 < 2.5 * t32) return 128;
   return 2.5 * t32 < t64 ? 32 : 64;
 }
 
@@ -185,8 +192,8 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
   int tiles = 0;
   long ws_total = 0;
   for (GemmProb& p : probs) {
-    p.tiles_n = (p.N + ts - 1) / ts;
-    const int tm = (p.M + ts - 1) / ts;
+    p.tiles_n = (p.N + tile_n(ts) - 1) / tile_n(ts);
+    const int tm = (p.M + tile_m(ts) - 1) / tile_m(ts);
     p.tile_begin = tiles;
     tiles += tm * p.tiles_n * p.ksplit;
     p.a_kfast = unit_cols(p.A) ? 1 : (unit_rows(p.A) ? 0 : 1);
@@ -253,7 +260,8 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
 // force_tile / force_ks (> 0) override both for every problem.
 static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
                             int force_tile = 0, int op = -1, int force_ks = 0) {
-  std::vector<GemmProb> g64, g32;
+  const int kinds[3] = {64, 32, 128};
+  std::vector<GemmProb> groups[3];
   for (int k = 0; k < b.nm; ++k) {
     GemmProb p = build(k);
     if (p.M <= 0 || p.N <= 0) continue;
@@ -269,19 +277,32 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
         ks = it->second[1];
       }
     }
-    if (ts != 32 && ts != 64) ts = choose_tile(p);
+    if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
     if (!allow_split) force_split(p, 1);
     else if (ks >= 1) force_split(p, ks);
     else choose_split(p, ts);
-    (ts == 32 ? g32 : g64).push_back(p);
+    groups[ts == 64 ? 0 : (ts == 32 ? 1 : 2)].push_back(p);
   }
-  if (g64.empty()) return build_group(b, g32, 32);
-  GemmTable t = build_group(b, g64, 64);
-  if (!g32.empty()) {
-    b.alt.push_back(build_group(b, g32, 32));
-    t.alt = (int)b.alt.size() - 1;
+  // the first non-empty group is the table; the others hang off it through alt links
+  GemmTable head;
+  bool have = false;
+  int last = -1;   // index in b.alt of the last linked part, -1: the head
+  for (int g = 0; g < 3; ++g) {
+    if (groups[g].empty()) continue;
+    GemmTable t = build_group(b, groups[g], kinds[g]);
+    if (!have) {
+      head = t;
+      have = true;
+      continue;
+    }
+    b.alt.push_back(t);
+    const int idx = (int)b.alt.size() - 1;
+    if (last < 0) head.alt = idx;
+    else b.alt[last].alt = idx;
+    last = idx;
   }
-  return t;
+  if (!have) head = build_group(b, {}, 64);
+  return head;
 }
 
 static GemmProb prob(Operand A, Operand B, Operand C, int M, int N, int K, double alpha, double beta) {
@@ -1330,8 +1351,8 @@ int hcc_set_gemm_tuning(hcc_ctx* ctx, int count, const int* entries) {
   std::map<std::array<int, 3>, std::array<int, 2>> m;
   for (int q = 0; q < count; ++q) {
     const int* e = entries + 5 * q;
-    if (e[0] < 0 || e[1] < 1 || e[2] < 1 || (e[3] != 32 && e[3] != 64) || e[4] < 1 || e[4] > 256) {
-      ctx->err = "invalid tuning entry (op, n_occ, n_virt, tile 32|64, ksplit 1..256)";
+    if (e[0] < 0 || e[1] < 1 || e[2] < 1 || (e[3] != 32 && e[3] != 64 && e[3] != 128) || e[4] < 1 || e[4] > 256) {
+      ctx->err = "invalid tuning entry (op, n_occ, n_virt, tile 32|64|128, ksplit 1..256)";
       return HCC_E_ARG;
     }
     m[{e[0], e[1], e[2]}] = {e[3], e[4]};
@@ -1359,7 +1380,7 @@ int hcc_tune_gemm(hcc_ctx* ctx, int op, int tile, int ksplit, int reps, double* 
   if (ctx == nullptr || ms == nullptr || reps < 1) return HCC_E_ARG;
   if (int r = need_batch(ctx)) return r;
   Batch& b = *ctx->batch;
-  if (op < 0 || op >= (int)b.ops.size() || (tile != 0 && tile != 32 && tile != 64) || ksplit < 0) {
+  if (op < 0 || op >= (int)b.ops.size() || (tile != 0 && tile != 32 && tile != 64 && tile != 128) || ksplit < 0) {
     ctx->err = "invalid tuning request";
     return HCC_E_ARG;
   }

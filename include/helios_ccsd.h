@@ -140,7 +140,8 @@ int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
 /* GEMM autotuning (no reference counterpart).  A contraction `op` (0..23, the
  * iteration's grouped GEMMs in launch order) of a molecule with (n_occ, n_virt)
- * runs with the tile edge (32 or 64) and K split of a matching entry; molecules
+ * runs with the tile kind (64: 64x64, 32: 32x32, 128: 128x32) and K split of a
+ * matching entry; molecules
  * without one use the cost heuristic.  entries: count x 5 ints
  * (op, n_occ, n_virt, tile, ksplit); applies to molecules loaded afterwards.
  * hcc_tune_gemm times one contraction over the loaded batch (reps launches,
