@@ -78,8 +78,10 @@ template <int BM, int BN, int BK>
 struct F64Tile {
   static constexpr int kWaves = (BM / 32) * (BN / 32);
   static constexpr int kT = 64 * kWaves;
-  static constexpr int kPa = BM + 2;          // pitch of the [k][m] layout
-  static constexpr int kPb = BN + 2;
+  // pitch of the [k][m] layout: == 16 (mod 32) doubles, so the fragment reads
+  // of rows k and k+1 (lanes 0-15 and 16-31) fall in disjoint bank halves
+  static constexpr int kPa = BM + 16;
+  static constexpr int kPb = BN + 16;
   static constexpr int kLdsA = ((BK / 16) * BM * kPk > BK * kPa) ? (BK / 16) * BM * kPk : BK * kPa;
   static constexpr int kLdsB = ((BK / 16) * BN * kPk > BK * kPb) ? (BK / 16) * BN * kPk : BK * kPb;
   // offset of (row, k) in the K-contiguous layout of an R-row operand
