@@ -1214,6 +1214,11 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   Params prm = params(ctx, eps, maxiter, true);
   prm.kappa = ctx->kappa;
   prm.max_inner = ctx->max_inner;
+  // The inner solve is a fixed-point iteration on the correction equation,
+  // not the reference's iteration history: it takes the Jacobi step dt = Omega/D
+  // (the '2-1' update (2 d2 + d2^T)/D is 3 Omega/D, hence 1/3 there).
+  prm.s1 = 1.0;
+  prm.s2 = ctx->projtype == 2 ? 1.0 / 3.0 : 1.0;
   Params prm0 = prm;   // residual-norm pass at the outer point: no step, no DIIS
   prm0.s1 = prm0.s2 = 0.0;
   prm0.diis = 0;

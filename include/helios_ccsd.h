@@ -60,7 +60,9 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
  * correction-form contractions C += A0 dB + dA (B0 + dB) whose operands are
  * rounded to BF16 (bf16 = 1; 0 keeps FP64, for testing), until
  * max|d| < kappa * max|d(t0)| or max_inner iterations; at most max_outer
- * outer points.  Defaults: 1, 0.03, 40, 20. */
+ * outer points.  The inner iterations take the Jacobi step dt = Omega/D with
+ * DIIS (the t1/t2 steps of hcc_set_options apply to FP64 solves only).
+ * Defaults: 1, 0.03, 40, 20. */
 int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer);
 
 /* Per molecule of the last mixed-precision batch solve: FP64 residual
