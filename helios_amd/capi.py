@@ -54,6 +54,7 @@ SIGNATURES = {
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
     "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
+    "hcc_set_graphs": (_i, [_vp, _i]),
     "hcc_tune_gemm": (_i, [_vp, _i, _i, _i, _i, _P(_d)]),
     "hcc_gemm_shape": (_i, [_vp, _i, _i, _P(_i)]),
     "hcc_mlp_create": (_i, [_P(_vp), _i, _l]),
@@ -238,6 +239,9 @@ class Context:
         pre = np.zeros(nmol)
         self._check(self.lib.hcc_gemm_flops(self.h, _dp(it), _dp(pre)), "hcc_gemm_flops")
         return it, pre
+
+    def set_graphs(self, enable=True):
+        self._check(self.lib.hcc_set_graphs(self.h, int(enable)), "hcc_set_graphs")
 
     def set_gemm_tuning(self, entries):
         """entries: (op, n_occ, n_virt, tile, ksplit) rows; applies to molecules loaded afterwards."""

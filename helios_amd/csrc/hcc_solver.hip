@@ -677,6 +677,8 @@ struct hcc_ctx {
   double prof_op_ms[64] = {0};
   // learned-guess network (FP32 parameters on the device), null until set
   float* guess_net = nullptr;
+  // replay blocks of iterations as captured HIP graphs (off while profiling)
+  int graphs = 1;
   // measured GEMM tile / K-split choices, (op, n_occ, n_virt) -> (tile edge, ksplit)
   std::map<std::array<int, 3>, std::array<int, 2>> tuning;
 };
@@ -1157,6 +1159,32 @@ int hcc_residual(hcc_ctx* ctx, double* d1, double* d2) {
   API_END
 }
 
+// Executable graph of the launches `body` enqueues on stream s (captured, not
+// run).  The body must not synchronize, copy to the host or memset.
+struct GraphExec {
+  hipGraphExec_t e = nullptr;
+  ~GraphExec() {
+    if (e) (void)hipGraphExecDestroy(e);
+  }
+  void launch(hipStream_t s) { HCHECK(hipGraphLaunch(e, s)); }
+};
+
+static void capture(GraphExec& ge, hipStream_t s, const std::function<void()>& body) {
+  hipGraph_t g = nullptr;
+  HCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HCHECK(hipStreamEndCapture(s, &g));
+  const hipError_t r = hipGraphInstantiate(&ge.e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HCHECK(r);
+}
+
 static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool build_integrals) {
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
@@ -1171,8 +1199,21 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   const int check_every = 4;
   int it = 0;
   double iters_total = 0.0;
+  // blocks of check_every iterations replay one captured graph: every kernel
+  // reads its molecule's state on the device, so the launches never change
+  GraphExec block;
+  const bool graphs = ctx->graphs && !ctx->prof && maxiter >= check_every;
+  if (graphs)
+    capture(block, s, [&]() {
+      for (int q = 0; q < check_every; ++q) run_iteration(ctx, prm, nullptr);
+    });
   while (it < maxiter) {
-    for (int q = 0; q < check_every && it < maxiter; ++q, ++it) run_iteration(ctx, prm, nullptr);
+    if (graphs && it + check_every <= maxiter) {
+      block.launch(s);
+      it += check_every;
+    } else {
+      for (int q = 0; q < check_every && it < maxiter; ++q, ++it) run_iteration(ctx, prm, nullptr);
+    }
     HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
     HCHECK(hipStreamSynchronize(s));
     if (ctx->prof) prof_collect(ctx);
@@ -1231,6 +1272,18 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     for (int a : act) n += a;
     return n;
   };
+  auto inner_iteration = [&]() {
+    residual_delta(ctx);
+    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
+    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    if (prm.diis) {
+      diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
+      mixed_inner_advance(b.dm, b.dst, b.nm, s);
+    }
+  };
+  // blocks of 4 inner iterations replay one captured graph (see run_solve)
+  GraphExec inner_block;
+  const bool graphs = ctx->graphs && !ctx->prof;
   for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
     set_phase(b.dst, b.dact, b.nm, 0, s);
     residual(ctx);
@@ -1239,15 +1292,17 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     if (alive() == 0) break;
     set_phase(b.dst, b.dact, b.nm, 1, s);
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
-    for (int inner = 0; inner < ctx->max_inner; ++inner) {
-      residual_delta(ctx);
-      mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
-      mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
-      if (prm.diis) {
-        diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
-        mixed_inner_advance(b.dm, b.dst, b.nm, s);
+    for (int inner = 0; inner < ctx->max_inner;) {
+      if (graphs && inner + 4 <= ctx->max_inner) {
+        if (!inner_block.e) capture(inner_block, s, [&]() {
+            for (int q = 0; q < 4; ++q) inner_iteration();
+          });
+        inner_block.launch(s);
+        inner += 4;
+      } else {
+        for (int q = 0; q < 4 && inner < ctx->max_inner; ++q, ++inner) inner_iteration();
       }
-      if ((inner & 3) == 3 && alive() == 0) break;
+      if (alive() == 0) break;
     }
     mixed_commit(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
   }
@@ -1348,6 +1403,12 @@ int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches) {
   if (ctx == nullptr) return HCC_E_ARG;
   if (gemm_ms) *gemm_ms = ctx->prof_ms;
   if (gemm_launches) *gemm_launches = ctx->prof_launches;
+  return HCC_OK;
+}
+
+int hcc_set_graphs(hcc_ctx* ctx, int enable) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  ctx->graphs = enable ? 1 : 0;
   return HCC_OK;
 }
 

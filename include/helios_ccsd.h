@@ -140,6 +140,13 @@ int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup)
  * of the iteration (0..N-1 in launch order) and 60..63 for the set-up GEMMs. */
 int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
+/* Iteration blocks (4 iterations of hcc_solve / hcc_batch_solve, 4 inner
+ * iterations of the mixed solver) are captured once per solve as HIP graphs
+ * and replayed; enable = 0 launches every kernel from the host instead (same
+ * kernels, same order, same results).  Graphs are not used while profiling.
+ * Default 1. */
+int hcc_set_graphs(hcc_ctx* ctx, int enable);
+
 /* GEMM autotuning (no reference counterpart).  A contraction `op` (0..23, the
  * iteration's grouped GEMMs in launch order) of a molecule with (n_occ, n_virt)
  * runs with the tile kind (64: 64x64, 32: 32x32, 128: 128x32) and K split of a

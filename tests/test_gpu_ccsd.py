@@ -98,3 +98,29 @@ def test_batch_matches_single_solves():
         ref = ccsd.solve(p["C"], p["F"], p["G"], p["nocc"], p["eref"], eps=1e-11, diis_scale=True)
         assert st[k] == 1
         assert abs(e[k, 0] - ref["energy"][0]) < 1e-8
+
+
+@pytest.mark.parametrize("precision", [capi.HCC_FP64, capi.HCC_MIXED])
+def test_graph_replay_equals_eager_launches(precision):
+    """Iteration blocks replayed from captured HIP graphs give bitwise the
+    results of launching every kernel from the host, solve after solve (each
+    solve replays its graph many times, with host synchronisations between)."""
+    ps = [prepare(M.batch_member(k)) for k in range(12)]
+    out = []
+    for graphs in (False, True):
+        ctx = capi.Context(0, precision, 8)
+        ctx.set_graphs(graphs)
+        ctx.batch_load(ps)
+        runs = []
+        for _ in range(2):
+            ctx.batch_solve(capi.GUESS_MP2, 500, 1e-9)
+            runs.append(ctx.batch_results())
+        ctx.close()
+        for r in runs[1:]:
+            assert all(np.array_equal(a, b) for a, b in zip(runs[0], r))
+        out.append(runs[0])
+    e0, it0, st0 = out[0]
+    e1, it1, st1 = out[1]
+    assert (st0 == 1).all()
+    assert np.array_equal(it0, it1) and np.array_equal(st0, st1)
+    assert np.array_equal(e0, e1)
