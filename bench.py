@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from helios_amd import capi, learned_guess, molecules, ppp  # noqa: E402
+from helios_amd import autotune, capi, learned_guess, molecules, ppp  # noqa: E402
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--ablation", action="store_true", help="also time FP64+learned and mixed+MP2")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore the measured GEMM tile/split table")
     ap.add_argument("--skip-reference", action="store_true",
                     help="profiler runs only: no FP64 reference solve, solves count as converged by status alone")
     ap.add_argument("--kappa", type=float, default=0.03)
@@ -124,9 +125,12 @@ def main():
               f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
     net = learned_guess.load_params()
+    tuning = None if args.no_tuning else autotune.load()
 
     def timed(precision, guess):
         ctx = capi.Context(0, precision, args.diis)
+        if tuning:
+            ctx.set_gemm_tuning(tuning)
         if precision == capi.HCC_MIXED:
             ctx.set_mixed(True, args.kappa, args.max_inner, args.max_outer)
         ctx.set_guess_network(net)
@@ -196,6 +200,7 @@ def main():
                                   else "plain FP64 batched solver, MP2 guess"),
                    "molecules": len(mols), "residual_threshold": args.eps, "diis_steps": args.diis,
                    "energies_checked_against_fp64_reference": not args.skip_reference,
+                   "gemm_tuning": bool(tuning),
                    "converged": int(head["ok"].sum()), "max_abs_dE_eV": float(head["de"].max()),
                    "fp64_solves_per_s": float(r64["solves"]), "fp64_ms_per_step": r64["step_s"] * 1e3,
                    "fp64_mean_iterations": float(r64["iters"].mean()),
