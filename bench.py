@@ -29,11 +29,11 @@ from helios_amd import autotune, capi, learned_guess, molecules, ppp  # noqa: E4
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
-# HBM bytes per gemm_f64_kernel launch in the FP64 solve of the default 4096-molecule
+# HBM bytes per gemm_f64_kernel dispatch in the FP64 solve of the default 4096-molecule
 # batch: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy), two rocprofv3
 # --pmc passes summarised in profiles/hbm_traffic_fp64_batch4096.md.  A counter pass
 # cannot run inside this process, so other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 4.932e9
+GEMM_TRAFFIC_BATCH4096 = 2.879e9
 
 
 def _prep(k):

@@ -692,9 +692,17 @@ static void launch_table(hcc_ctx* c, const GemmTable& t) {
   if (t.alt >= 0) launch_table(c, c->batch->alt[t.alt]);
 }
 
+// GEMM kernel dispatches of a table (its parts of different tile kinds)
+static int table_parts(const hcc_ctx* c, const GemmTable& t) {
+  int n = 1;
+  for (int a = t.alt; a >= 0; a = c->batch->alt[a].alt) ++n;
+  return n;
+}
+
 static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
   if (t.nprob == 0) return;
   if (c->prof) {
+    c->prof_launches += table_parts(c, t);
     if (c->prof_used == c->prof_events.size()) {
       hipEvent_t a, b;
       HCHECK(hipEventCreate(&a));
@@ -720,7 +728,6 @@ static void prof_collect(hcc_ctx* c) {
     c->prof_ms += ms;
     c->prof_op_ms[c->prof_ids[q]] += ms;
   }
-  c->prof_launches += (long)c->prof_used;
   c->prof_used = 0;
 }
 
@@ -773,6 +780,7 @@ static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
   if (t.nprob == 0) return;
   hipEvent_t* ev = nullptr;
   if (c->prof) {
+    c->prof_launches += table_parts(c, t);
     if (c->prof_used == c->prof_events.size()) {
       hipEvent_t a, b2;
       HCHECK(hipEventCreate(&a));

@@ -128,8 +128,10 @@ int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total);
 
 /* Live timing of the contraction GEMM launches (the dominant kernel): when
  * enabled, every grouped-GEMM launch is bracketed by HIP events on the
- * context's stream; hcc_profile_read returns the summed launch time and the
- * launch count since hcc_profile(ctx, 1).  hcc_gemm_flops gives, per loaded
+ * context's stream; hcc_profile_read returns the summed time of those brackets
+ * (GEMM kernels and their split-K reductions) and the number of GEMM kernel
+ * dispatches (a contraction with problems of several tile kinds dispatches
+ * once per kind) since hcc_profile(ctx, 1).  hcc_gemm_flops gives, per loaded
  * molecule, the GEMM flops (2*M*N*K over its problems) of one iteration and of
  * the once-per-solve integral set-up.  No reference counterpart (the
  * reference has only wall-clock timers, src/modules/mod_global.f90:2025-2039). */
