@@ -41,6 +41,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const double* p) {
 __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, int byte_off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc16(const __bf16* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
+}
+// BF16 element at an FP64 byte offset of the same array (element index = off / 8)
+__device__ __forceinline__ unsigned short bload16(__amdgpu_buffer_rsrc_t r, int byte_off64) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, byte_off64 == kOobOffset ? kOobOffset : byte_off64 >> 2, 0, 0);
+}
 
 __device__ __forceinline__ int row_off(const Operand& o, int r) {
   return (r / o.rr) * o.srh + (r % o.rr) * o.srl;
@@ -595,12 +602,13 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     fb[i] = n0 + nl < P.N ? col_off(P.B, n0 + nl) : -1;
     sb[i] = nl * LD + kqb[i];
   }
-  const __amdgpu_buffer_rsrc_t rA0 = rsrc(P.A.p), rdA = rsrc(HA ? P.dA : P.A.p);
-  const __amdgpu_buffer_rsrc_t rB0 = rsrc(P.B.p), rdB = rsrc(HB ? P.dB : P.B.p);
-  // raw FP64 staging registers (rounded to BF16 at the LDS store, so the
-  // loads stay in flight across the current slab's matrix-core work):
-  // A0 and dB feed A0 dB (needs HB), dA and B0 + dB feed dA (B0 + dB) (needs HA)
-  double ra0[NE], rda[NE], rdb[NE], rb0[NE];
+  const __amdgpu_buffer_rsrc_t rA0 = rsrc16(P.hA), rdA = rsrc(HA ? P.dA : P.A.p);
+  const __amdgpu_buffer_rsrc_t rB0 = rsrc16(P.hB), rdB = rsrc(HB ? P.dB : P.B.p);
+  // staging registers, converted at the LDS store so that the loads stay in
+  // flight across the current slab's matrix-core work: A0 (BF16 copy) and dB
+  // feed A0 dB (needs HB), dA and B0 (BF16 copy) + dB feed dA (B0 + dB) (needs HA)
+  unsigned short ra0[NE], rb0[NE];
+  double rda[NE], rdb[NE];
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -621,12 +629,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
       for (int i = 0; i < NE; ++i) {
         int kl = k0 - c0 + kqa[i];
         int o = fa[i] >= 0 && kl <= klast ? (fa[i] + koffA[min(kl, klast)]) * 8 : kOobOffset;
-        if (HB) ra0[i] = bload(rA0, o);
+        if (HB) ra0[i] = bload16(rA0, o);
         if (HA) rda[i] = bload(rdA, o);
         kl = k0 - c0 + kqb[i];
         o = fb[i] >= 0 && kl <= klast ? (fb[i] + koffB[min(kl, klast)]) * 8 : kOobOffset;
         if (HB) rdb[i] = bload(rdB, o);
-        if (HA) rb0[i] = bload(rB0, o);
+        if (HA) rb0[i] = bload16(rB0, o);
       }
     };
     load(c0);
@@ -634,12 +642,13 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
 #pragma unroll
       for (int i = 0; i < NE; ++i) {
         if (HB) {
-          A0s[sa[i]] = (__bf16)(float)ra0[i];
+          A0s[sa[i]] = __builtin_bit_cast(__bf16, ra0[i]);
           dBs[sb[i]] = (__bf16)(float)rdb[i];
         }
         if (HA) {
           dAs[sa[i]] = (__bf16)(float)rda[i];
-          Bfs[sb[i]] = (__bf16)(float)(rb0[i] + (HB ? rdb[i] : 0.0));
+          const float b0 = __builtin_bit_cast(float, (unsigned)rb0[i] << 16);
+          Bfs[sb[i]] = (__bf16)(b0 + (HB ? (float)rdb[i] : 0.f));
         }
       }
       __syncthreads();

@@ -53,6 +53,10 @@ struct GemmProb {
   // dB marks an operand that does not change with the amplitudes
   double* dA;
   double* dB;
+  // BF16 copies of A0 and B0 (same element offsets), refreshed at every outer
+  // point; the BF16 correction kernel reads A0 and B0 from them
+  const __bf16* hA;
+  const __bf16* hB;
 };
 
 // Tile shape of the FP64 MFMA GEMM.

@@ -28,6 +28,7 @@ void make_L(const Mol* mols, Ragged rgT2, int nm, hipStream_t s);
 void guess(const Mol* mols, State* st, int* act, Ragged rgT2S, int nm, int kind, hipStream_t s);
 void reset_state(State* st, int* act, int nm, hipStream_t s);
 void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s);
+void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 
 // mixed-precision solver (correction form)

@@ -8,6 +8,7 @@
 // launch covers all molecules and no workgroup straddles two.  Reductions are
 // two-pass (per-workgroup partials, then one fixed-order pass per molecule):
 // results are bitwise reproducible run to run.
+#include <algorithm>
 #include <cmath>
 
 #include "hcc_batch.h"
@@ -935,6 +936,19 @@ void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Rag
 void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_delta, rgT2S, m0, md, st, rgT2S, nm);
 }
+// BF16 copy of an FP64 array (round to nearest even)
+__global__ __launch_bounds__(kEwThreads) void k_to_bf16(const double* __restrict__ src, __bf16* __restrict__ dst,
+                                                       long n) {
+  for (long e = (long)blockIdx.x * kEwThreads + threadIdx.x; e < n; e += (long)gridDim.x * kEwThreads)
+    dst[e] = (__bf16)(float)src[e];
+}
+
+void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s) {
+  if (n <= 0) return;
+  const long blocks = std::min<long>((n + kEwThreads - 1) / kEwThreads, 65536);
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)blocks), dim3(kEwThreads), 0, s, src, dst, n);
+}
+
 void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_t1, rgOV, mols, rgOV, nm);
 }

@@ -107,6 +107,14 @@ struct Batch {
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   std::vector<GemmTable> alt;   // second (other tile edge) parts of split contraction tables
+  // mixed precision: BF16 copies of the outer-point operands (one per arena)
+  struct Shadow {
+    double* base;      // FP64 arena (molecule 0's block)
+    long count;        // elements spanned by the arena
+    __bf16* copy;      // BF16 copy, same element offsets
+    bool variable;     // changes at every outer point
+  };
+  std::vector<Shadow> shadows;
   // measured tile/split choices per (contraction, n_occ, n_virt); null: cost heuristic
   const std::map<std::array<int, 3>, std::array<int, 2>>* tuning = nullptr;
   double *d1out = nullptr, *d2out = nullptr;
@@ -342,6 +350,22 @@ static const std::vector<FieldDef>& variable_fields() {
   return f;
 }
 
+// Operand arenas that never change during a solve (inputs, undressed integrals).
+static const std::vector<FieldDef>& constant_fields() {
+  static const std::vector<FieldDef> f = {
+      {&Mol::C, sz_nn}, {&Mol::G, sz_nn}, {&Mol::F, sz_nn}, {&Mol::Fao, sz_nn},
+      {&Mol::Gklcd, sz_t2}, {&Mol::Lldkc, sz_t2}, {&Mol::Lt, sz_t2},
+  };
+  return f;
+}
+
+// BF16 copy of an outer-point operand element, or null when p lies in no arena
+static const __bf16* shadow_of(const Batch& b, const double* p) {
+  for (const auto& sh : b.shadows)
+    if (p >= sh.base && p < sh.base + sh.count) return sh.copy + (p - sh.base);
+  return nullptr;
+}
+
 // Correction twin of an operand pointer of molecule k, or null for a constant
 // input (MO coefficients, gamma, Fock, undressed integrals).
 static double* delta_of(const Batch& b, int k, double* p) {
@@ -423,6 +447,15 @@ static void allocate(Batch& b, bool keep_mp2) {
     }
     b.ddm = b.dalloc<Mol>(b.nm);
     HCHECK(hipMemcpy(b.ddm, b.hd.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
+    auto add_shadow = [&](const FieldDef& f, bool variable) {
+      const Mol& first = b.hm[0];
+      const Mol& last = b.hm[b.nm - 1];
+      double* base = first.*(f.member);
+      const long count = (last.*(f.member) - base) + f.size(last.n, last.o, last.v);
+      b.shadows.push_back({base, count, b.dalloc<__bf16>(count), variable});
+    };
+    for (const auto& f : variable_fields()) add_shadow(f, true);
+    for (const auto& f : constant_fields()) add_shadow(f, false);
   }
   b.dst = b.dalloc<State>(b.nm);
   HCHECK(hipMemset(b.dst, 0, b.nm * sizeof(State)));
@@ -634,6 +667,9 @@ static void build_delta_tables(Batch& b) {
       double* dc = delta_of(b, k, p.C.p);
       if (dc == nullptr) throw std::string("correction table: constant output operand");
       if (p.dA == nullptr && p.dB == nullptr) throw std::string("correction table: both operands constant");
+      p.hA = shadow_of(b, p.A.p);
+      p.hB = shadow_of(b, p.B.p);
+      if (p.hA == nullptr || p.hB == nullptr) throw std::string("correction table: operand without a BF16 copy");
       p.C.p = dc;
       return p;
     }, true, 64));
@@ -1246,6 +1282,13 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   return HCC_OK;
 }
 
+// BF16 copies of the outer-point operands (variable arenas; constants too when asked)
+static void refresh_shadows(hcc_ctx* c, bool constants) {
+  for (const auto& sh : c->batch->shadows)
+    if (sh.variable || constants) to_bf16(sh.base, sh.copy, sh.count, c->stream);
+  HCHECK(hipGetLastError());
+}
+
 // Defect-correction solve: FP64 residual at the outer point t0, then inner
 // iterations on the correction dt whose residual is Omega(t0) + dOmega with
 // dOmega from correction-form GEMMs (BF16 operands, FP32 accumulation), until
@@ -1299,6 +1342,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2, b.part2, s);
     if (alive() == 0) break;
     set_phase(b.dst, b.dact, b.nm, 1, s);
+    if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
     for (int inner = 0; inner < ctx->max_inner;) {
       if (graphs && inner + 4 <= ctx->max_inner) {
