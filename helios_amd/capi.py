@@ -143,7 +143,7 @@ class Context:
         self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
                     "hcc_set_options")
 
-    def set_mixed(self, bf16=True, kappa=0.03, max_inner=40, max_outer=20):
+    def set_mixed(self, bf16=True, kappa=0.06, max_inner=40, max_outer=20):
         self._check(self.lib.hcc_set_mixed(self.h, int(bf16), kappa, max_inner, max_outer), "hcc_set_mixed")
 
     def batch_statistics(self):

@@ -5,8 +5,12 @@
 
 namespace hcc {
 
-// Device pointers of one molecule's blocks in the per-buffer arenas.
-struct Mol {
+// Device pointers of one molecule's blocks in the per-buffer arenas.  V is
+// the element type of the buffers that change with the amplitudes: double for
+// the solver state (Mol), float for the corrections of the mixed-precision
+// solver (MolD), whose inputs, DIIS storage and constants stay FP64.
+template <class V>
+struct MolT {
   int n, o, v, m;          // sites, occupied, virtual, DIIS depth
   int nd;                  // packed DIIS vector length: ov + ov(ov+1)/2
   int pad0;
@@ -14,20 +18,24 @@ struct Mol {
   // inputs (row-major)
   double *C, *G, *F, *Fao;
   // dressing
-  double *Xf, *Yf, *Mf, *P, *Q, *Ft, *tmp;
+  V *Xf, *Yf, *Mf, *P, *Q, *Ft, *tmp;
   // amplitudes and residuals
-  double *t1, *t2, *u, *w1, *w2, *x, *Zc, *WC, *WD;
+  V *t1, *t2, *u, *w1, *w2, *x, *Zc, *WC, *WD;
   // integrals and intermediates
-  double *G4, *Wkl, *Gklcd, *Lldkc, *Lt, *gvvov, *gooov, *Xvv, *Xoo;
+  V *G4, *Wkl;
+  double *Gklcd, *Lldkc, *Lt;
+  V *gvvov, *gooov, *Xvv, *Xoo;
   // DIIS history: m slots of nd (amplitudes st, errors sd); B matrix m x m
   double *st, *sd, *Bm;
   // guess features / MP2 amplitudes (o^2 v^2)
   double *t2mp2;
   // dressed-density diagonal and its Coulomb field (2n)
-  double *dvec;
+  V *dvec;
   // site-localisation overlaps of the MOs (n x n), features of the learned guess
   double *Ov;
 };
+using Mol = MolT<double>;
+using MolD = MolT<float>;
 
 // Per-molecule iteration state (reference iterator semantics,
 // src/calculation/gIterator.f90:88-210, statuses mod_helios.f90:119-120).

@@ -44,6 +44,15 @@ __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, int byte_off) 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc16(const __bf16* p) {
   return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
 }
+// FP32 element at an FP64 byte offset of the same array (element index = off / 8)
+__device__ __forceinline__ float bload32(__amdgpu_buffer_rsrc_t r, int byte_off64) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off64 == kOobOffset ? kOobOffset
+                                                                                                 : byte_off64 >> 1,
+                                                                         0, 0));
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrcf(const float* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
+}
 // BF16 element at an FP64 byte offset of the same array (element index = off / 8)
 __device__ __forceinline__ unsigned short bload16(__amdgpu_buffer_rsrc_t r, int byte_off64) {
   return __builtin_amdgcn_raw_buffer_load_b16(r, byte_off64 == kOobOffset ? kOobOffset : byte_off64 >> 2, 0, 0);
@@ -466,9 +475,9 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
     fb[i] = n < P.N ? col_off(P.B, n) : -1;
   }
   const gdouble* __restrict__ A0 = gptr(P.A.p);
-  const gdouble* __restrict__ dA = gptr(hasA ? P.dA : P.A.p);
+  const float* __restrict__ dA = P.dA;
   const gdouble* __restrict__ B0 = gptr(P.B.p);
-  const gdouble* __restrict__ dB = gptr(hasB ? P.dB : P.B.p);
+  const float* __restrict__ dB = P.dB;
   double ra0[NE], rda[NE], rdb[NE], rbf[NE];
   auto load = [&](int k0) {
 #pragma unroll
@@ -476,13 +485,13 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
       int k = k0 + kqa[i];
       bool ok = fa[i] >= 0 && k < kend;
       int o = ok ? fa[i] + col_off(P.A, k) : 0;
-      const double a0 = A0[o], da = dA[o];
+      const double a0 = A0[o], da = hasA ? (double)dA[o] : 0.0;
       ra0[i] = ok ? a0 : 0.0;
       rda[i] = (ok && hasA) ? da : 0.0;
       k = k0 + kqb[i];
       ok = fb[i] >= 0 && k < kend;
       o = ok ? fb[i] + row_off(P.B, k) : 0;
-      const double b0 = B0[o], db = dB[o];
+      const double b0 = B0[o], db = hasB ? (double)dB[o] : 0.0;
       rdb[i] = (ok && hasB) ? db : 0.0;
       rbf[i] = ok ? b0 + rdb[i] : 0.0;
     }
@@ -543,7 +552,7 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
     }
     return;
   }
-  gdouble* __restrict__ Cp = gptr(P.C.p);
+  float* __restrict__ Cp = P.Cd;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
@@ -602,13 +611,13 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     fb[i] = n0 + nl < P.N ? col_off(P.B, n0 + nl) : -1;
     sb[i] = nl * LD + kqb[i];
   }
-  const __amdgpu_buffer_rsrc_t rA0 = rsrc16(P.hA), rdA = rsrc(HA ? P.dA : P.A.p);
-  const __amdgpu_buffer_rsrc_t rB0 = rsrc16(P.hB), rdB = rsrc(HB ? P.dB : P.B.p);
+  const __amdgpu_buffer_rsrc_t rA0 = rsrc16(P.hA), rdA = rsrcf(HA ? P.dA : nullptr);
+  const __amdgpu_buffer_rsrc_t rB0 = rsrc16(P.hB), rdB = rsrcf(HB ? P.dB : nullptr);
   // staging registers, converted at the LDS store so that the loads stay in
   // flight across the current slab's matrix-core work: A0 (BF16 copy) and dB
   // feed A0 dB (needs HB), dA and B0 (BF16 copy) + dB feed dA (B0 + dB) (needs HA)
   unsigned short ra0[NE], rb0[NE];
-  double rda[NE], rdb[NE];
+  float rda[NE], rdb[NE];
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -630,10 +639,10 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
         int kl = k0 - c0 + kqa[i];
         int o = fa[i] >= 0 && kl <= klast ? (fa[i] + koffA[min(kl, klast)]) * 8 : kOobOffset;
         if (HB) ra0[i] = bload16(rA0, o);
-        if (HA) rda[i] = bload(rdA, o);
+        if (HA) rda[i] = bload32(rdA, o);
         kl = k0 - c0 + kqb[i];
         o = fb[i] >= 0 && kl <= klast ? (fb[i] + koffB[min(kl, klast)]) * 8 : kOobOffset;
-        if (HB) rdb[i] = bload(rdB, o);
+        if (HB) rdb[i] = bload32(rdB, o);
         if (HA) rb0[i] = bload16(rB0, o);
       }
     };
@@ -643,12 +652,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
       for (int i = 0; i < NE; ++i) {
         if (HB) {
           A0s[sa[i]] = __builtin_bit_cast(__bf16, ra0[i]);
-          dBs[sb[i]] = (__bf16)(float)rdb[i];
+          dBs[sb[i]] = (__bf16)rdb[i];
         }
         if (HA) {
-          dAs[sa[i]] = (__bf16)(float)rda[i];
+          dAs[sa[i]] = (__bf16)rda[i];
           const float b0 = __builtin_bit_cast(float, (unsigned)rb0[i] << 16);
-          Bfs[sb[i]] = (__bf16)(b0 + (HB ? (float)rdb[i] : 0.f));
+          Bfs[sb[i]] = (__bf16)(b0 + (HB ? rdb[i] : 0.f));
         }
       }
       __syncthreads();
@@ -694,7 +703,7 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     }
     return;
   }
-  gdouble* __restrict__ Cp = gptr(P.C.p);
+  float* __restrict__ Cp = P.Cd;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
@@ -762,6 +771,11 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmProb* __re
   for (int k = 0; k < P.ksplit; ++k) s += P.ws[k * mn + e];
   const int o = row_off(P.C, row) + col_off(P.C, col);
   double v = P.alpha * s;
+  if (P.Cd != nullptr) {   // FP32 correction output
+    if (P.beta != 0.0) v += P.beta * P.Cd[o];
+    P.Cd[o] = (float)v;
+    return;
+  }
   if (P.beta != 0.0) v += P.beta * P.C.p[o];
   P.C.p[o] = v;
 }

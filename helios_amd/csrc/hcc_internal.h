@@ -49,10 +49,12 @@ struct GemmProb {
   int kchunk;       // K range of one chunk (multiple of the K tile)
   double* ws;       // split-K partials: ksplit x M x N, row-major (ksplit > 1)
   // correction ("delta") form, used by the mixed-precision inner iterations:
-  // C += alpha * (A0 dB + dA (B0 + dB)) with A0 = A.p, B0 = B.p; a null dA or
-  // dB marks an operand that does not change with the amplitudes
-  double* dA;
-  double* dB;
+  // Cd += alpha * (A0 dB + dA (B0 + dB)) with A0 = A.p, B0 = B.p and the FP32
+  // corrections dA, dB, Cd (Cd at C's offsets); a null dA or dB marks an
+  // operand that does not change with the amplitudes
+  const float* dA;
+  const float* dB;
+  float* Cd;
   // BF16 copies of A0 and B0 (same element offsets), refreshed at every outer
   // point; the BF16 correction kernel reads A0 and B0 from them
   const __bf16* hA;

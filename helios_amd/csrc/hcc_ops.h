@@ -12,14 +12,19 @@ void ew_dress(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t
 void ew_fock_m(const Mol* mols, const State* st, Ragged rgN, Ragged rgNN, int nm, hipStream_t s);
 void ew_pouter(const Mol* mols, const State* st, Ragged rgN3, int nm, hipStream_t s);
 void ew_u(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
+void ew_u(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
 void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
+void ew_init_xx(const MolD* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
 void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
+void ew_assemble(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
 void ew_update(const Mol* mols, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                double* part1, double* part2, hipStream_t s);
 void ew_contravariant(const Mol* mols, Ragged rgT2S, int nm, double* d1, double* d2, hipStream_t s);
 void ctl_after_update(const Mol* mols, State* st, int nm, const Params& prm, Ragged rgOV,
                       const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
 void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part,
+               hipStream_t s);
+void diis_step(const MolD* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part,
                hipStream_t s);
 void ctl_after_diis(const Mol* mols, State* st, int* act, int nm, const Params& prm, const int* stop,
                     hipStream_t s);
@@ -32,11 +37,11 @@ void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 
 // mixed-precision solver (correction form)
-void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
+void mixed_dress(const Mol* m0, const MolD* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
                  hipStream_t s);
-void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
-void mixed_commit(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
-void mixed_update(const Mol* m0, const Mol* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
+void mixed_zero(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
+void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
+void mixed_update(const Mol* m0, const MolD* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                   double* part1, double* part2, hipStream_t s);
 void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, int max_outer, Ragged rgOV,
                          const double* part1, Ragged rgT2, const double* part2, hipStream_t s);

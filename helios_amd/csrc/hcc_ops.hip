@@ -28,7 +28,7 @@ __device__ __forceinline__ int block_mol(const int* __restrict__ prefix, int nm,
 #define RAGGED_PROLOGUE(SIZE)                                         \
   const int mol = rg.bmol[blockIdx.x];                               \
   const int lb = blockIdx.x - rg.prefix[mol];                        \
-  const Mol& M = mols[mol];                                          \
+  const auto& M = mols[mol];                                         \
   const long e = (long)lb * kEwThreads + threadIdx.x;                \
   const long size_ = (SIZE);                                         \
   (void)size_;
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ m
 }
 
 // u = 2 t2 - t2(a<->b)
-__global__ __launch_bounds__(kEwThreads) void k_u(const Mol* __restrict__ mols, const State* __restrict__ st,
+template <class MT>
+__global__ __launch_bounds__(kEwThreads) void k_u(const MT* __restrict__ mols, const State* __restrict__ st,
                                                  Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
@@ -113,11 +114,12 @@ __global__ __launch_bounds__(kEwThreads) void k_u(const Mol* __restrict__ mols, 
   const long ab = e % ((long)v * v);
   const long base = e - ab;
   const int a = ab / v, b = ab % v;
-  M.u[e] = 2.0 * M.t2[e] - M.t2[base + (long)b * v + a];
+  M.u[e] = 2.0 * M.t2[e] - (double)M.t2[base + (long)b * v + a];
 }
 
 // Xvv = Ft_vv, Xoo = Ft_oo (the Fock parts of the E2 intermediates)
-__global__ __launch_bounds__(kEwThreads) void k_init_xx(const Mol* __restrict__ mols, const State* __restrict__ st,
+template <class MT>
+__global__ __launch_bounds__(kEwThreads) void k_init_xx(const MT* __restrict__ mols, const State* __restrict__ st,
                                                        Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].o);
   if (e >= size_ || !st[mol].active) return;
@@ -134,7 +136,8 @@ __global__ __launch_bounds__(kEwThreads) void k_init_xx(const Mol* __restrict__ 
 
 // w2 += x + x^T - 1/2 (Z[ai,bj] + Z[bj,ai]) - (Z[aj,bi] + Z[bi,aj]), with
 // Zs[i][j][a][b] = Z[(a,i),(b,j)]
-__global__ __launch_bounds__(kEwThreads) void k_assemble(const Mol* __restrict__ mols, const State* __restrict__ st,
+template <class MT>
+__global__ __launch_bounds__(kEwThreads) void k_assemble(const MT* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
@@ -145,8 +148,8 @@ __global__ __launch_bounds__(kEwThreads) void k_assemble(const Mol* __restrict__
   const int i = e / ((long)o * v * v);
   const long ij = ((long)i * o + j) * v * v, ji = ((long)j * o + i) * v * v;
   const long ab = (long)a * v + b, ba = (long)b * v + a;
-  const double* Z = M.Zc;
-  double val = M.x[e] + M.x[ji + ba];
+  const auto* Z = M.Zc;
+  double val = (double)M.x[e] + M.x[ji + ba];
   val -= 0.5 * (Z[ij + ab] + Z[ji + ba]);
   val -= Z[ji + ab] + Z[ij + ba];
   M.w2[e] += val;
@@ -345,12 +348,13 @@ __global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st,
 // DIIS
 // ---------------------------------------------------------------------------
 // partial dots <sd_k, sd_new> over this workgroup's chunk, for every valid slot k
-__global__ __launch_bounds__(kEwThreads) void k_diis_dot(const Mol* __restrict__ mols, const State* __restrict__ st,
+template <class MT>
+__global__ __launch_bounds__(kEwThreads) void k_diis_dot(const MT* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm, double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
   const int mol = rg.bmol[blockIdx.x];
   const int lb = blockIdx.x - rg.prefix[mol];
-  const Mol& M = mols[mol];
+  const auto& M = mols[mol];
   const State& S = st[mol];
   const int m = M.m;
   if (!S.active || !S.staged) {
@@ -373,10 +377,11 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const Mol* __restrict__
 
 // finalise dots into B, build the bordered system, solve it (partial pivoting),
 // scaled by the largest diagonal of B so that it stays accurate near convergence
-__global__ __launch_bounds__(64) void k_diis_solve(const Mol* __restrict__ mols, State* __restrict__ st, int nm,
+template <class MT>
+__global__ __launch_bounds__(64) void k_diis_solve(const MT* __restrict__ mols, State* __restrict__ st, int nm,
                                                    const int* __restrict__ prefix, const double* __restrict__ part) {
   const int mol = blockIdx.x;
-  const Mol& M = mols[mol];
+  const auto& M = mols[mol];
   State& S = st[mol];
   if (!S.active || !S.staged) return;
   const int m = M.m, cnt = S.diis_count, snew = S.diis_next;
@@ -440,7 +445,8 @@ __global__ __launch_bounds__(64) void k_diis_solve(const Mol* __restrict__ mols,
 // t = sum_k c_k st_k.  Grid: singles, then the (mm, nn) square of excitation
 // pairs; each upper-triangle element reads its packed entry once (coalesced
 // along nn) and writes both mirror elements of t2.
-__global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const Mol* __restrict__ mols, const State* __restrict__ st,
+template <class MT>
+__global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict__ mols, const State* __restrict__ st,
                                                            Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v * (1L + (long)mols[mol].o * mols[mol].v));
   const State& S = st[mol];
@@ -622,14 +628,31 @@ void ew_fock_m(const Mol* mols, const State* st, Ragged rgN, Ragged rgNN, int nm
 void ew_pouter(const Mol* mols, const State* st, Ragged rgN3, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_pouter, rgN3, mols, st, rgN3, nm);
 }
-void ew_u(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
-  RAGGED_LAUNCH(k_u, rgT2, mols, st, rgT2, nm);
+template <class MT>
+static void ew_u_t(const MT* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_u<MT>, rgT2, mols, st, rgT2, nm);
 }
+template <class MT>
+static void ew_init_xx_t(const MT* mols, const State* st, Ragged rgXX, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_init_xx<MT>, rgXX, mols, st, rgXX, nm);
+}
+template <class MT>
+static void ew_assemble_t(const MT* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_assemble<MT>, rgT2, mols, st, rgT2, nm);
+}
+void ew_u(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) { ew_u_t(mols, st, rgT2, nm, s); }
+void ew_u(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) { ew_u_t(mols, st, rgT2, nm, s); }
 void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream_t s) {
-  RAGGED_LAUNCH(k_init_xx, rgXX, mols, st, rgXX, nm);
+  ew_init_xx_t(mols, st, rgXX, nm, s);
+}
+void ew_init_xx(const MolD* mols, const State* st, Ragged rgXX, int nm, hipStream_t s) {
+  ew_init_xx_t(mols, st, rgXX, nm, s);
 }
 void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
-  RAGGED_LAUNCH(k_assemble, rgT2, mols, st, rgT2, nm);
+  ew_assemble_t(mols, st, rgT2, nm, s);
+}
+void ew_assemble(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s) {
+  ew_assemble_t(mols, st, rgT2, nm, s);
 }
 void ew_update(const Mol* mols, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                double* part1, double* part2, hipStream_t s) {
@@ -644,11 +667,17 @@ void ctl_after_update(const Mol* mols, State* st, int nm, const Params& prm, Rag
   hipLaunchKernelGGL(k_control1, dim3(nm), dim3(kEwThreads), 0, s, mols, st, nm, prm, rgOV.prefix,
                      part1, rgT2.prefix, part2);
 }
-void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part,
-               hipStream_t s) {
-  RAGGED_LAUNCH(k_diis_dot, rgND, mols, st, rgND, nm, part);
-  hipLaunchKernelGGL(k_diis_solve, dim3(nm), dim3(64), 0, s, mols, st, nm, rgND.prefix, part);
-  RAGGED_LAUNCH(k_diis_extrap, rgP2, mols, st, rgP2, nm);
+template <class MT>
+static void diis_step_t(const MT* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part, hipStream_t s) {
+  RAGGED_LAUNCH(k_diis_dot<MT>, rgND, mols, st, rgND, nm, part);
+  hipLaunchKernelGGL(k_diis_solve<MT>, dim3(nm), dim3(64), 0, s, mols, st, nm, rgND.prefix, part);
+  RAGGED_LAUNCH(k_diis_extrap<MT>, rgP2, mols, st, rgP2, nm);
+}
+void diis_step(const Mol* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part, hipStream_t s) {
+  diis_step_t(mols, st, nm, rgND, rgP2, part, s);
+}
+void diis_step(const MolD* mols, State* st, int nm, Ragged rgND, Ragged rgP2, double* part, hipStream_t s) {
+  diis_step_t(mols, st, nm, rgND, rgP2, part, s);
 }
 void ctl_after_diis(const Mol* mols, State* st, int* act, int nm, const Params& prm, const int* stop,
                     hipStream_t s) {
@@ -679,11 +708,11 @@ void activate(State* st, int* act, int nm, hipStream_t s) {
 // exact change of its plain counterpart, linear (dressing, dressed density,
 // Fock matrix) or bilinear (orbital products).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kEwThreads) void k_dress_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_dress_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                        const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
   if (e >= size_ || !st[mol].active) return;
-  const Mol& D = dmols[mol];
+  const MolD& D = dmols[mol];
   const int N = M.n, o = M.o, v = M.v;
   const int mu = e / N, p = e % N;
   const double* C = M.C + (long)mu * N;
@@ -698,26 +727,26 @@ __global__ __launch_bounds__(kEwThreads) void k_dress_d(const Mol* __restrict__ 
   D.Yf[e] = dy;
 }
 
-__device__ __forceinline__ double dressed_density_d(const Mol& M, const Mol& D, int mu, int nu) {
-  const double* dY = D.Yf + (long)mu * M.n;
+__device__ __forceinline__ double dressed_density_d(const Mol& M, const MolD& D, int mu, int nu) {
+  const float* dY = D.Yf + (long)mu * M.n;
   const double* Cn = M.C + (long)nu * M.n;
   double s = 0.0;
   for (int k = 0; k < M.o; ++k) s += dY[k] * Cn[k];
   return s;
 }
 
-__global__ __launch_bounds__(kEwThreads) void k_fock_diag_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_fock_diag_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                            const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n);
   if (e >= size_ || !st[mol].active) return;
   dmols[mol].dvec[e] = dressed_density_d(M, dmols[mol], (int)e, (int)e);
 }
 
-__global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                         const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n);
   if (e >= size_ || !st[mol].active) return;
-  const Mol& D = dmols[mol];
+  const MolD& D = dmols[mol];
   const int N = M.n;
   const int mu = e / N, nu = e % N;
   double val = -M.G[e] * dressed_density_d(M, D, mu, nu);
@@ -730,20 +759,20 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__
 }
 
 // dP = X0 (x) dY + dX (x) (Y0 + dY)
-__global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                         const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
   if (e >= size_ || !st[mol].active) return;
-  const Mol& D = dmols[mol];
+  const MolD& D = dmols[mol];
   const long N = M.n, N2 = N * N;
   const long mu = e / N2, pq = e % N2;
   const long p = pq / N, q = pq % N;
   const long ip = mu * N + p, iq = mu * N + q;
-  D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + D.Yf[iq]);
+  D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + (double)D.Yf[iq]);
 }
 
 // t (x) 0 for the corrections of active molecules
-__global__ __launch_bounds__(kEwThreads) void k_zero_delta(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_zero_delta(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                           const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
@@ -759,7 +788,7 @@ __global__ __launch_bounds__(kEwThreads) void k_zero_t1(const Mol* __restrict__ 
 }
 
 // t0 += dt
-__global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                             const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
   if (e >= size_ || st[mol].status != 0) return;
@@ -769,7 +798,7 @@ __global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restri
 }
 
 // mixed update, singles: Omega1 = (w1_0 incl. F~_ai) + dw1 + dF~_ai; T = t1_0 + dt1
-__global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                              const State* __restrict__ st, Ragged rg, int nm, Params prm,
                                                              double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
@@ -777,18 +806,19 @@ __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restr
   if (!st[mol].active) return;
   double amax = 0.0;
   if (e < size_) {
-    const Mol& D = dmols[mol];
+    const MolD& D = dmols[mol];
     const int N = M.n, o = M.o, v = M.v;
     const int i = e / v, a = e % v;
     const double d1 = M.w1[e] + D.w1[e] + D.Ft[(long)(o + a) * N + i];
     const double Dn = M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a];
     const double tn = M.t1[e] + D.t1[e] + prm.s1 * d1 / Dn;
-    D.t1[e] = tn - M.t1[e];
+    const float dt = (float)(tn - M.t1[e]);
+    D.t1[e] = dt;
     amax = fabs(d1);
     if (prm.diis) {
       const State& S = st[mol];
       M.sd[(long)S.diis_next * M.nd + e] = d1;
-      M.st[(long)S.diis_next * M.nd + e] = D.t1[e];
+      M.st[(long)S.diis_next * M.nd + e] = dt;
     }
   }
   const double r = block_max(amax, sh);
@@ -796,7 +826,7 @@ __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restr
 }
 
 // mixed update, doubles (pair representatives as in k_update2); DIIS stores corrections
-__global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restrict__ mols, const Mol* __restrict__ dmols,
+__global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                              const State* __restrict__ st, Ragged rg, int nm, Params prm,
                                                              double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
@@ -804,7 +834,7 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
   if (!st[mol].active) return;
   double amax = 0.0;
   if (e < size_) {
-    const Mol& Dm = dmols[mol];
+    const MolD& Dm = dmols[mol];
     const int N = M.n, o = M.o, v = M.v;
     const long ne = (long)o * v;
     const int b = e % v;
@@ -817,15 +847,15 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
     const long r = (((long)I * o + J) * v + A) * v + B;
     const long mb = (long)I * v + B, na = (long)J * v + A;
     const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
-    const double om = M.w2[r] + Dm.w2[r];
-    const double d2 = 2.0 * om - (M.w2[rx] + Dm.w2[rx]);
+    const double om = M.w2[r] + (double)Dm.w2[r];
+    const double d2 = 2.0 * om - (M.w2[rx] + (double)Dm.w2[rx]);
     amax = fabs(d2);
     const double Dn = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
                       M.F[(long)(o + B) * N + o + B];
     const double num = prm.projtype == 2 ? 3.0 * om : d2;
     if (mm <= nn) {
       const double tn = M.t2[e] + Dm.t2[e] + prm.s2 * num / Dn;
-      const double dt = tn - M.t2[e];
+      const float dt = (float)(tn - M.t2[e]);
       Dm.t2[e] = dt;
       Dm.t2[(((long)j * o + i) * v + b) * v + a] = dt;
       if (prm.diis) {
@@ -926,14 +956,14 @@ __global__ void k_set_phase(State* __restrict__ st, int* __restrict__ act, int n
   act[mol] = a;
 }
 
-void mixed_dress(const Mol* m0, const Mol* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
+void mixed_dress(const Mol* m0, const MolD* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
                  hipStream_t s) {
   RAGGED_LAUNCH(k_dress_d, rgNN, m0, md, st, rgNN, nm);
   RAGGED_LAUNCH(k_fock_diag_d, rgN, m0, md, st, rgN, nm);
   RAGGED_LAUNCH(k_fock_m_d, rgNN, m0, md, st, rgNN, nm);
   RAGGED_LAUNCH(k_pouter_d, rgN3, m0, md, st, rgN3, nm);
 }
-void mixed_zero(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
+void mixed_zero(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_delta, rgT2S, m0, md, st, rgT2S, nm);
 }
 // BF16 copy of an FP64 array (round to nearest even)
@@ -952,10 +982,10 @@ void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s) {
 void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_t1, rgOV, mols, rgOV, nm);
 }
-void mixed_commit(const Mol* m0, const Mol* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
+void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_commit_delta, rgT2S, m0, md, st, rgT2S, nm);
 }
-void mixed_update(const Mol* m0, const Mol* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
+void mixed_update(const Mol* m0, const MolD* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                   double* part1, double* part2, hipStream_t s) {
   RAGGED_LAUNCH(k_update1_mixed, rgOV, m0, md, st, rgOV, nm, prm, part1);
   RAGGED_LAUNCH(k_update2_mixed, rgT2, m0, md, st, rgT2, nm, prm, part2);

@@ -120,8 +120,8 @@ struct Batch {
   double *d1out = nullptr, *d2out = nullptr;
   // mixed precision: correction buffers and correction-form GEMM tables
   bool mixed = false;
-  std::vector<Mol> hd;
-  Mol* ddm = nullptr;
+  std::vector<MolD> hd;   // FP32 corrections (variable buffers) + shared FP64 inputs
+  MolD* ddm = nullptr;
   std::vector<std::function<GemmProb(const Mol&)>> ops;
   std::vector<GemmTable> dit;
 
@@ -330,31 +330,35 @@ static GemmProb prob(Operand A, Operand B, Operand C, int M, int N, int K, doubl
 // in the mixed-precision solver), with their per-molecule sizes.
 struct FieldDef {
   double* Mol::*member;
+  float* MolD::*dmember;   // its FP32 correction twin (variable fields)
   std::function<long(int, int, int)> size;
 };
 
+#define HCC_VF(f, sz) {&Mol::f, &MolD::f, sz}
 static const std::vector<FieldDef>& variable_fields() {
   static const std::vector<FieldDef> f = {
-      {&Mol::Xf, sz_nn}, {&Mol::Yf, sz_nn}, {&Mol::Mf, sz_nn}, {&Mol::Ft, sz_nn}, {&Mol::tmp, sz_nn},
-      {&Mol::P, sz_n3}, {&Mol::Q, sz_n3}, {&Mol::t1, sz_ov}, {&Mol::w1, sz_ov}, {&Mol::t2, sz_t2},
-      {&Mol::u, sz_t2}, {&Mol::w2, sz_t2}, {&Mol::x, sz_t2}, {&Mol::Zc, sz_t2}, {&Mol::WC, sz_t2},
-      {&Mol::WD, sz_t2},
-      {&Mol::G4, [](int, int, int v) { return (long)v * v * v * v; }},
-      {&Mol::Wkl, [](int, int o, int) { return (long)o * o * o * o; }},
-      {&Mol::gvvov, [](int, int o, int v) { return (long)o * v * v * v; }},
-      {&Mol::gooov, [](int, int o, int v) { return (long)o * o * o * v; }},
-      {&Mol::Xvv, [](int, int, int v) { return (long)v * v; }},
-      {&Mol::Xoo, [](int, int o, int) { return (long)o * o; }},
-      {&Mol::dvec, [](int n, int, int) { return 2L * n; }},
+      HCC_VF(Xf, sz_nn), HCC_VF(Yf, sz_nn), HCC_VF(Mf, sz_nn), HCC_VF(Ft, sz_nn), HCC_VF(tmp, sz_nn),
+      HCC_VF(P, sz_n3), HCC_VF(Q, sz_n3), HCC_VF(t1, sz_ov), HCC_VF(w1, sz_ov), HCC_VF(t2, sz_t2),
+      HCC_VF(u, sz_t2), HCC_VF(w2, sz_t2), HCC_VF(x, sz_t2), HCC_VF(Zc, sz_t2), HCC_VF(WC, sz_t2),
+      HCC_VF(WD, sz_t2),
+      HCC_VF(G4, [](int, int, int v) { return (long)v * v * v * v; }),
+      HCC_VF(Wkl, [](int, int o, int) { return (long)o * o * o * o; }),
+      HCC_VF(gvvov, [](int, int o, int v) { return (long)o * v * v * v; }),
+      HCC_VF(gooov, [](int, int o, int v) { return (long)o * o * o * v; }),
+      HCC_VF(Xvv, [](int, int, int v) { return (long)v * v; }),
+      HCC_VF(Xoo, [](int, int o, int) { return (long)o * o; }),
+      HCC_VF(dvec, [](int n, int, int) { return 2L * n; }),
   };
   return f;
 }
+#undef HCC_VF
 
 // Operand arenas that never change during a solve (inputs, undressed integrals).
 static const std::vector<FieldDef>& constant_fields() {
   static const std::vector<FieldDef> f = {
-      {&Mol::C, sz_nn}, {&Mol::G, sz_nn}, {&Mol::F, sz_nn}, {&Mol::Fao, sz_nn},
-      {&Mol::Gklcd, sz_t2}, {&Mol::Lldkc, sz_t2}, {&Mol::Lt, sz_t2},
+      {&Mol::C, nullptr, sz_nn}, {&Mol::G, nullptr, sz_nn}, {&Mol::F, nullptr, sz_nn},
+      {&Mol::Fao, nullptr, sz_nn}, {&Mol::Gklcd, nullptr, sz_t2}, {&Mol::Lldkc, nullptr, sz_t2},
+      {&Mol::Lt, nullptr, sz_t2},
   };
   return f;
 }
@@ -368,13 +372,13 @@ static const __bf16* shadow_of(const Batch& b, const double* p) {
 
 // Correction twin of an operand pointer of molecule k, or null for a constant
 // input (MO coefficients, gamma, Fock, undressed integrals).
-static double* delta_of(const Batch& b, int k, double* p) {
+static float* delta_of(const Batch& b, int k, const double* p) {
   const Mol& H = b.hm[k];
-  const Mol& D = b.hd[k];
+  const MolD& D = b.hd[k];
   for (const auto& f : variable_fields()) {
-    double* base = H.*(f.member);
+    const double* base = H.*(f.member);
     const long size = f.size(H.n, H.o, H.v);
-    if (p >= base && p < base + size) return D.*(f.member) + (p - base);
+    if (p >= base && p < base + size) return D.*(f.dmember) + (p - base);
   }
   return nullptr;
 }
@@ -440,13 +444,30 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.dm = b.dalloc<Mol>(b.nm);
   HCHECK(hipMemcpy(b.dm, b.hm.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
   if (b.mixed) {
-    b.hd = b.hm;   // constant inputs, DIIS storage: shared with the outer point
-    for (const auto& f : variable_fields()) {
-      auto ptr = arena([&](int n, int o, int v) { return f.size(n, o, v); });
-      for (int k = 0; k < b.nm; ++k) b.hd[k].*(f.member) = ptr[k];
+    // corrections: FP32 twins of the variable buffers; inputs, constants and the
+    // DIIS storage are shared with the outer point
+    b.hd.resize(b.nm);
+    for (int k = 0; k < b.nm; ++k) {
+      const Mol& H = b.hm[k];
+      MolD& D = b.hd[k];
+      memset(&D, 0, sizeof(D));
+      D.n = H.n; D.o = H.o; D.v = H.v; D.m = H.m; D.nd = H.nd; D.eref = H.eref;
+      D.C = H.C; D.G = H.G; D.F = H.F; D.Fao = H.Fao;
+      D.Gklcd = H.Gklcd; D.Lldkc = H.Lldkc; D.Lt = H.Lt;
+      D.st = H.st; D.sd = H.sd; D.Bm = H.Bm; D.t2mp2 = H.t2mp2; D.Ov = H.Ov;
     }
-    b.ddm = b.dalloc<Mol>(b.nm);
-    HCHECK(hipMemcpy(b.ddm, b.hd.data(), b.nm * sizeof(Mol), hipMemcpyHostToDevice));
+    for (const auto& f : variable_fields()) {
+      long total = 0;
+      std::vector<long> off(b.nm);
+      for (int k = 0; k < b.nm; ++k) {
+        off[k] = total;
+        total += (f.size(b.n[k], b.o[k], b.v[k]) + 63) / 64 * 64;   // 256-byte aligned blocks
+      }
+      float* base = b.dalloc<float>(total);
+      for (int k = 0; k < b.nm; ++k) b.hd[k].*(f.dmember) = base + off[k];
+    }
+    b.ddm = b.dalloc<MolD>(b.nm);
+    HCHECK(hipMemcpy(b.ddm, b.hd.data(), b.nm * sizeof(MolD), hipMemcpyHostToDevice));
     auto add_shadow = [&](const FieldDef& f, bool variable) {
       const Mol& first = b.hm[0];
       const Mol& last = b.hm[b.nm - 1];
@@ -664,13 +685,12 @@ static void build_delta_tables(Batch& b) {
       GemmProb p = f(b.hm[k]);
       p.dA = delta_of(b, k, p.A.p);
       p.dB = delta_of(b, k, p.B.p);
-      double* dc = delta_of(b, k, p.C.p);
-      if (dc == nullptr) throw std::string("correction table: constant output operand");
+      p.Cd = delta_of(b, k, p.C.p);
+      if (p.Cd == nullptr) throw std::string("correction table: constant output operand");
       if (p.dA == nullptr && p.dB == nullptr) throw std::string("correction table: both operands constant");
       p.hA = shadow_of(b, p.A.p);
       p.hB = shadow_of(b, p.B.p);
       if (p.hA == nullptr || p.hB == nullptr) throw std::string("correction table: operand without a BF16 copy");
-      p.C.p = dc;
       return p;
     }, true, 64));
   }
@@ -700,7 +720,7 @@ struct hcc_ctx {
   double last_iters = 0.0;
   // mixed-precision solver options
   int mixed_bf16 = 1;
-  double kappa = 0.03;
+  double kappa = 0.06;
   int max_inner = 40;
   int max_outer = 20;
   // live timing of the contraction GEMM launches (HIP events on `stream`)
