@@ -48,6 +48,7 @@ SIGNATURES = {
     "hcc_profile": (_i, [_vp, _i]),
     "hcc_profile_read": (_i, [_vp, _P(_d), _P(ctypes.c_long)]),
     "hcc_gemm_flops": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_gemm_flops_correction": (_i, [_vp, _P(_d)]),
     "hcc_profile_ops": (_i, [_vp, _P(_d)]),
     "hcc_destroy": (None, [_vp]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
@@ -263,6 +264,11 @@ class Context:
         """Parameters (MLP_PARAMS float32) of the network used by GUESS_LEARNED."""
         self._w = np.ascontiguousarray(params, dtype=np.float32).ravel()
         self._check(self.lib.hcc_set_guess_network(self.h, _fp(self._w), self._w.size), "hcc_set_guess_network")
+
+    def gemm_flops_correction(self, nmol):
+        fl = np.zeros(nmol)
+        self._check(self.lib.hcc_gemm_flops_correction(self.h, _dp(fl)), "hcc_gemm_flops_correction")
+        return fl
 
     def last_timing(self):
         ms = ctypes.c_double()

@@ -1478,6 +1478,24 @@ int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches) {
   return HCC_OK;
 }
 
+int hcc_gemm_flops_correction(hcc_ctx* ctx, double* per_inner_iteration) {
+  if (ctx == nullptr || per_inner_iteration == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  for (int k = 0; k < b.nm; ++k) per_inner_iteration[k] = 0.0;
+  std::vector<GemmProb> h;
+  std::function<void(const GemmTable&)> add = [&](const GemmTable& t) {
+    h.resize(t.nprob);
+    if (t.nprob) HCHECK(hipMemcpy(h.data(), t.d, t.nprob * sizeof(GemmProb), hipMemcpyDeviceToHost));
+    for (const auto& p : h)   // A0 dB and dA (B0 + dB): one product per changing operand
+      per_inner_iteration[p.mol] += 2.0 * p.M * (double)p.N * p.K * ((p.dA != nullptr) + (p.dB != nullptr));
+    if (t.alt >= 0) add(b.alt[t.alt]);
+  };
+  for (const auto& t : b.dit) add(t);
+  API_END
+}
+
 int hcc_set_graphs(hcc_ctx* ctx, int enable) {
   if (ctx == nullptr) return HCC_E_ARG;
   ctx->graphs = enable ? 1 : 0;

@@ -138,8 +138,12 @@ int hcc_last_timing(hcc_ctx* ctx, double* solve_ms, double* iterations_total);
 int hcc_profile(hcc_ctx* ctx, int enable);
 int hcc_profile_read(hcc_ctx* ctx, double* gemm_ms, long* gemm_launches);
 int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup);
+/* Matrix-core flops of one inner iteration of the mixed solver per loaded
+ * molecule (HCC_MIXED contexts; 0 otherwise): 2*M*N*K per correction product. */
+int hcc_gemm_flops_correction(hcc_ctx* ctx, double* per_inner_iteration);
 /* Per-contraction split of the profiled GEMM time: ms64[k] for GEMM table k
- * of the iteration (0..N-1 in launch order) and 60..63 for the set-up GEMMs. */
+ * of the iteration (0..N-1 in launch order), 32+k for its correction form
+ * (mixed solver) and 60..63 for the set-up GEMMs. */
 int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
 
 /* Iteration blocks (4 iterations of hcc_solve / hcc_batch_solve, 4 inner
