@@ -574,17 +574,20 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
 
 // HA / HB: the operand has a correction (dA / dB non-null for every problem
 // of the table).  A constant A needs only A0 dB, a constant B only dA B0.
-template <bool AKF, bool BNF, bool HA, bool HB>
+// S: tile scale -- 64S x 64S output tiles, 4 waves of 32S x 32S (S = 1, 2);
+// larger tiles halve the operand traffic per product, which bounds this kernel.
+template <int S, bool AKF, bool BNF, bool HA, bool HB>
 __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict__ probs,
                                                       const int* __restrict__ tile_prob, int total_tiles,
                                                       const int* __restrict__ active) {
-  constexpr int BM = 64, BN = 64, T = 256, BK = 32, NE = 8, LD = BK + 8;
+  constexpr int BM = 64 * S, BN = 64 * S, T = 256, BK = 32, NE = 8 * S, LD = BK + 8, F = 2 * S;
+  constexpr int KT = 512;   // K-offset table length per chunk
   __shared__ __attribute__((aligned(16))) __bf16 A0s[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 dAs[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 dBs[BN * LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bfs[BN * LD];
-  __shared__ int koffA[kKTab];
-  __shared__ int koffB[kKTab];
+  __shared__ int koffA[KT];
+  __shared__ int koffB[KT];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
   const GemmProb P = probs[find_problem(tile_prob, tile)];
@@ -597,19 +600,18 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   const int m0 = (lt / tiles_n) * BM, n0 = (lt % tiles_n) * BN;
   const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  int fa[NE], fb[NE], kqa[NE], kqb[NE], sa[NE], sb[NE];
+  const int wm = (wave >> 1) * (32 * S), wn = (wave & 1) * (32 * S);
+  // staged element i of this thread: row ml(i), k kq(i) of the slab (element t + i T of
+  // the [row][k] or [k][row] order of the operand's unit stride)
+  auto mlA = [&](int i) { return AKF ? t / BK + i * (T / BK) : t % BM; };
+  auto kqA = [&](int i) { return AKF ? t % BK : t / BM + i * (T / BM); };
+  auto nlB = [&](int i) { return BNF ? t % BN : t / BK + i * (T / BK); };
+  auto kqB = [&](int i) { return BNF ? t / BN + i * (T / BN) : t % BK; };
+  int fa[NE], fb[NE];
 #pragma unroll
   for (int i = 0; i < NE; ++i) {
-    const int q = t + i * T;
-    const int ml = AKF ? q / BK : q % BM;
-    kqa[i] = AKF ? q % BK : q / BM;
-    fa[i] = m0 + ml < P.M ? row_off(P.A, m0 + ml) : -1;
-    sa[i] = ml * LD + kqa[i];
-    const int nl = BNF ? q % BN : q / BK;
-    kqb[i] = BNF ? q / BN : q % BK;
-    fb[i] = n0 + nl < P.N ? col_off(P.B, n0 + nl) : -1;
-    sb[i] = nl * LD + kqb[i];
+    fa[i] = m0 + mlA(i) < P.M ? row_off(P.A, m0 + mlA(i)) : -1;
+    fb[i] = n0 + nlB(i) < P.N ? col_off(P.B, n0 + nlB(i)) : -1;
   }
   const __amdgpu_buffer_rsrc_t rA0 = rsrc16(P.hA), rdA = rsrcf(HA ? P.dA : nullptr);
   const __amdgpu_buffer_rsrc_t rB0 = rsrc16(P.hB), rdB = rsrcf(HB ? P.dB : nullptr);
@@ -618,13 +620,13 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   // feed A0 dB (needs HB), dA and B0 (BF16 copy) + dB feed dA (B0 + dB) (needs HA)
   unsigned short ra0[NE], rb0[NE];
   float rda[NE], rdb[NE];
-  f4 acc[2][2];
+  f4 acc[F][F];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < F; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int c0 = kbeg; c0 < kend; c0 += kKTab) {
-    const int cend = min(kend, c0 + kKTab);
+    for (int j = 0; j < F; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = kbeg; c0 < kend; c0 += KT) {
+    const int cend = min(kend, c0 + KT);
     __syncthreads();
     for (int q = t; q < cend - c0; q += T) {
       koffA[q] = col_off(P.A, c0 + q);
@@ -636,11 +638,11 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     auto load = [&](int k0) {
 #pragma unroll
       for (int i = 0; i < NE; ++i) {
-        int kl = k0 - c0 + kqa[i];
+        int kl = k0 - c0 + kqA(i);
         int o = fa[i] >= 0 && kl <= klast ? (fa[i] + koffA[min(kl, klast)]) * 8 : kOobOffset;
         if (HB) ra0[i] = bload16(rA0, o);
         if (HA) rda[i] = bload32(rdA, o);
-        kl = k0 - c0 + kqb[i];
+        kl = k0 - c0 + kqB(i);
         o = fb[i] >= 0 && kl <= klast ? (fb[i] + koffB[min(kl, klast)]) * 8 : kOobOffset;
         if (HB) rdb[i] = bload32(rdB, o);
         if (HA) rb0[i] = bload16(rB0, o);
@@ -650,51 +652,57 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
     for (int k0 = c0; k0 < cend; k0 += BK) {
 #pragma unroll
       for (int i = 0; i < NE; ++i) {
+        const int sa = mlA(i) * LD + kqA(i), sb = nlB(i) * LD + kqB(i);
         if (HB) {
-          A0s[sa[i]] = __builtin_bit_cast(__bf16, ra0[i]);
-          dBs[sb[i]] = (__bf16)rdb[i];
+          A0s[sa] = __builtin_bit_cast(__bf16, ra0[i]);
+          dBs[sb] = (__bf16)rdb[i];
         }
         if (HA) {
-          dAs[sa[i]] = (__bf16)rda[i];
+          dAs[sa] = (__bf16)rda[i];
           const float b0 = __builtin_bit_cast(float, (unsigned)rb0[i] << 16);
-          Bfs[sb[i]] = (__bf16)(b0 + (HB ? rdb[i] : 0.f));
+          Bfs[sb] = (__bf16)(b0 + (HB ? rdb[i] : 0.f));
         }
       }
       __syncthreads();
       load(min(k0 + BK, cend));   // the last prefetch is fully masked
       const int kq = 8 * (lane >> 4);
-      const int ra = (wm + (lane & 15)) * LD + kq, rb = (wn + (lane & 15)) * LD + kq;
       if (HB) {
-        const bf8 a0 = *reinterpret_cast<const bf8*>(&A0s[ra]);
-        const bf8 a1 = *reinterpret_cast<const bf8*>(&A0s[ra + 16 * LD]);
-        const bf8 b0 = *reinterpret_cast<const bf8*>(&dBs[rb]);
-        const bf8 b1 = *reinterpret_cast<const bf8*>(&dBs[rb + 16 * LD]);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        bf8 a[F], b[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          a[f] = *reinterpret_cast<const bf8*>(&A0s[(wm + 16 * f + (lane & 15)) * LD + kq]);
+          b[f] = *reinterpret_cast<const bf8*>(&dBs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+#pragma unroll
+          for (int j = 0; j < F; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
       }
       if (HA) {
-        const bf8 e0 = *reinterpret_cast<const bf8*>(&dAs[ra]);
-        const bf8 e1 = *reinterpret_cast<const bf8*>(&dAs[ra + 16 * LD]);
-        const bf8 f0 = *reinterpret_cast<const bf8*>(&Bfs[rb]);
-        const bf8 f1 = *reinterpret_cast<const bf8*>(&Bfs[rb + 16 * LD]);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e0, f1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(e1, f1, acc[1][1], 0, 0, 0);
+        bf8 a[F], b[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          a[f] = *reinterpret_cast<const bf8*>(&dAs[(wm + 16 * f + (lane & 15)) * LD + kq]);
+          b[f] = *reinterpret_cast<const bf8*>(&Bfs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+#pragma unroll
+          for (int j = 0; j < F; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
       }
       __syncthreads();
     }
   }
+  // epilogue: V_MFMA_F32_16X16X32_BF16 leaves D[row][col] of a 16x16 block in
+  // lane col + 16 (row / 4), element row % 4
   if (P.ksplit > 1) {
-    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
+    double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < F; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
       if (col >= P.N) continue;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < F; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -705,12 +713,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   }
   float* __restrict__ Cp = P.Cd;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < F; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
     if (col >= P.N) continue;
     const int co = col_off(P.C, col);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < F; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -724,25 +732,30 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
 }
 
 hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tiles, const int* active,
-                            hipStream_t s, int flags, int bf16) {
+                            hipStream_t s, int flags, int bf16, int tile) {
   if (total_tiles <= 0) return hipSuccess;
   const int grid = ((total_tiles + 7) / 8) * 8;
   if (bf16) {
-#define HCC_DUAL16(AK, BN)                                                                              \
+#define HCC_DUAL16(SC, AK, BN)                                                                          \
   switch ((flags >> 2) & 3) {                                                                           \
-    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 1: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 2: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 3: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
     default: return hipErrorInvalidValue; /* both operands constant: not a correction product */     \
   }
-    switch (flags & 3) {
-      case 0: HCC_DUAL16(false, false) break;
-      case 1: HCC_DUAL16(true, false) break;
-      case 2: HCC_DUAL16(false, true) break;
-      default: HCC_DUAL16(true, true) break;
-    }
+#define HCC_DUAL16_S(SC)                        \
+  switch (flags & 3) {                          \
+    case 0: HCC_DUAL16(SC, false, false) break; \
+    case 1: HCC_DUAL16(SC, true, false) break;  \
+    case 2: HCC_DUAL16(SC, false, true) break;  \
+    default: HCC_DUAL16(SC, true, true) break;  \
+  }
+    if (tile == 256) { HCC_DUAL16_S(2) }
+    else { HCC_DUAL16_S(1) }
+#undef HCC_DUAL16_S
 #undef HCC_DUAL16
   } else {
+    if (tile != 64) return hipErrorInvalidValue;
     switch (flags & 3) {
       case 0: hipLaunchKernelGGL((gemm_dual_f64<false, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;
       case 1: hipLaunchKernelGGL((gemm_dual_f64<true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break;

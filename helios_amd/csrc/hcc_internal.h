@@ -75,9 +75,10 @@ hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int tota
 hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int total_tiles,
                             const int* active, hipStream_t s);
 // correction-form GEMM (see GemmProb::dA/dB); bf16 = 1 rounds the operands to
-// BF16 for V_MFMA_F32_16X16X32_BF16 (FP32 accumulation), 0 keeps FP64
+// BF16 for V_MFMA_F32_16X16X32_BF16 (FP32 accumulation), 0 keeps FP64;
+// tile kind 64 (64x64) or, with bf16, 256 (128x128)
 hipError_t launch_gemm_dual(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
-                            hipStream_t s, int flags, int bf16);
+                            hipStream_t s, int flags, int bf16, int tile);
 // C = alpha * sum_k ws_k + beta * C for the split problems of a table;
 // prefix: block prefix over the table's problems (256 elements per block),
 // block_prob: problem index of every reduction block

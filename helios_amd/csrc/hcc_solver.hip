@@ -58,9 +58,23 @@ struct GemmTable {
 };
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
-// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide).
-static int tile_m(int kind) { return kind == 128 ? 128 : kind; }
-static int tile_n(int kind) { return kind == 128 ? 32 : kind; }
+// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide); of the
+// BF16 correction GEMM: 64 (64x64) and 256 (128x128).
+static int tile_m(int kind) { return kind == 128 || kind == 256 ? 128 : kind; }
+static int tile_n(int kind) { return kind == 128 ? 32 : (kind == 256 ? 128 : kind); }
+
+// 128x128 correction tiles need ~260 registers per lane (one wave per SIMD);
+// measured slower than 64x64 tiles at four waves per SIMD, so off.
+constexpr bool kDualBigTiles = false;
+
+// Tile kind of a correction-form problem: 128x128 tiles carry half the operand
+// traffic per product (the bound of that kernel) and win unless the padding
+// of the larger tile costs more.
+static int choose_tile_dual(const GemmProb& p) {
+  const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
+  const double t128 = (double)((p.M + 127) / 128) * ((p.N + 127) / 128) * 128 * 128;
+  return kDualBigTiles && p.M >= 128 && p.N >= 128 && t128 < 1.5 * t64 ? 256 : 64;
+}
 
 // Split K when a problem has few output tiles and a long K: aim for >= 16
 // work items per problem with K chunks of at least 128.
@@ -120,6 +134,7 @@ struct Batch {
   double *d1out = nullptr, *d2out = nullptr;
   // mixed precision: correction buffers and correction-form GEMM tables
   bool mixed = false;
+  bool mixed_bf16 = true;   // BF16 correction GEMMs (false: FP64 exact mode, 64x64 tiles only)
   std::vector<MolD> hd;   // FP32 corrections (variable buffers) + shared FP64 inputs
   MolD* ddm = nullptr;
   std::vector<std::function<GemmProb(const Mol&)>> ops;
@@ -267,9 +282,9 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
 // problems with 32x32 tiles go to a second launch (Batch::alt).
 // force_tile / force_ks (> 0) override both for every problem.
 static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
-                            int force_tile = 0, int op = -1, int force_ks = 0) {
-  const int kinds[3] = {64, 32, 128};
-  std::vector<GemmProb> groups[3];
+                            int force_tile = 0, int op = -1, int force_ks = 0, bool dual = false) {
+  const int kinds[4] = {64, 32, 128, 256};
+  std::vector<GemmProb> groups[4];
   for (int k = 0; k < b.nm; ++k) {
     GemmProb p = build(k);
     if (p.M <= 0 || p.N <= 0) continue;
@@ -285,17 +300,18 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
         ks = it->second[1];
       }
     }
-    if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
+    if (dual) ts = choose_tile_dual(p);
+    else if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
     if (!allow_split) force_split(p, 1);
     else if (ks >= 1) force_split(p, ks);
     else choose_split(p, ts);
-    groups[ts == 64 ? 0 : (ts == 32 ? 1 : 2)].push_back(p);
+    groups[ts == 64 ? 0 : (ts == 32 ? 1 : (ts == 128 ? 2 : 3))].push_back(p);
   }
   // the first non-empty group is the table; the others hang off it through alt links
   GemmTable head;
   bool have = false;
   int last = -1;   // index in b.alt of the last linked part, -1: the head
-  for (int g = 0; g < 3; ++g) {
+  for (int g = 0; g < 4; ++g) {
     if (groups[g].empty()) continue;
     GemmTable t = build_group(b, groups[g], kinds[g]);
     if (!have) {
@@ -692,7 +708,7 @@ static void build_delta_tables(Batch& b) {
       p.hB = shadow_of(b, p.B.p);
       if (p.hA == nullptr || p.hB == nullptr) throw std::string("correction table: operand without a BF16 copy");
       return p;
-    }, true, 64));
+    }, true, b.mixed_bf16 ? 0 : 64, -1, 0, b.mixed_bf16));
   }
 }
 
@@ -849,7 +865,8 @@ static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
     HCHECK(hipEventRecord(ev[0], c->stream));
   }
   for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr) {
-    HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16));
+    HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16,
+                            p->tile));
     if (p->red_blocks > 0)
       HCHECK(launch_splitk_reduce(p->d, p->red_prefix, p->red_bprob, p->red_blocks, c->batch->dact, c->stream));
   }
@@ -921,6 +938,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   b->nm = nmol;
   b->m = c->diis_steps;
   b->mixed = c->precision == HCC_MIXED;
+  b->mixed_bf16 = c->mixed_bf16 != 0;
   b->tuning = c->tuning.empty() ? nullptr : &c->tuning;
   for (int k = 0; k < nmol; ++k) {
     if (n[k] < 2 || nocc[k] < 1 || nocc[k] >= n[k]) throw std::string("invalid molecule dimensions");
@@ -1021,11 +1039,19 @@ int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_o
     ctx->err = "invalid mixed-precision options";
     return HCC_E_ARG;
   }
-  ctx->mixed_bf16 = bf16 ? 1 : 0;
   ctx->kappa = kappa;
   ctx->max_inner = max_inner;
   ctx->max_outer = max_outer;
-  return HCC_OK;
+  const int want = bf16 ? 1 : 0;
+  if (want == ctx->mixed_bf16) return HCC_OK;
+  ctx->mixed_bf16 = want;
+  // a loaded batch's correction tables depend on the choice (tile kinds)
+  if (ctx->batch == nullptr || !ctx->batch->mixed) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  ctx->batch->mixed_bf16 = want != 0;
+  build_delta_tables(*ctx->batch);
+  API_END
 }
 
 int hcc_batch_statistics(hcc_ctx* ctx, int* outer_evaluations, int* inner_iterations) {
