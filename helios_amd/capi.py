@@ -53,6 +53,8 @@ SIGNATURES = {
     "hcc_destroy": (None, [_vp]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
+    "hcc_selftest_gemm_correction": (_i, [_i, _i, _i, _i, _i, _P(_d), _P(_f), _P(_d), _P(_f), _P(_f), _d, _d, _i,
+                                          _P(_d)]),
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
     "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
     "hcc_set_graphs": (_i, [_vp, _i]),
@@ -356,6 +358,29 @@ class Trainer:
         y = np.zeros(X.shape[0], dtype=np.float32)
         self._check(self.lib.hcc_mlp_predict(self.h, _fp(X), X.shape[0], _fp(y)), "hcc_mlp_predict")
         return y
+
+
+def selftest_gemm_correction(A0, dA, B0, dB, Cd, alpha=1.0, beta=0.0, a_kfast=True, b_nfast=True, reps=0):
+    """Cd' = alpha (A0 dB + dA (B0 + dB)) + beta Cd on the correction GEMM kernel.
+    A0, dA: m x k; B0, dB: k x n (logical shapes; dA or dB may be None).
+    Returns (Cd', mean ms of reps extra launches)."""
+    lib = load()
+    m, k = A0.shape
+    n = B0.shape[1]
+    lay_a = (lambda x: x) if a_kfast else (lambda x: x.T)
+    lay_b = (lambda x: x) if b_nfast else (lambda x: x.T)
+    a0 = np.ascontiguousarray(lay_a(A0), dtype=np.float64)
+    b0 = np.ascontiguousarray(lay_b(B0), dtype=np.float64)
+    da = None if dA is None else np.ascontiguousarray(lay_a(dA), dtype=np.float32)
+    db = None if dB is None else np.ascontiguousarray(lay_b(dB), dtype=np.float32)
+    cd = np.ascontiguousarray(Cd, dtype=np.float32).copy()
+    fp = lambda x: None if x is None else x.ctypes.data_as(_P(_f))
+    ms = ctypes.c_double(0.0)
+    rc = lib.hcc_selftest_gemm_correction(m, n, k, int(a_kfast), int(b_nfast), _dp(a0), fp(da), _dp(b0), fp(db),
+                                          fp(cd), alpha, beta, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise HccError(f"hcc_selftest_gemm_correction failed ({rc})")
+    return cd, ms.value
 
 
 def selftest_gemm(A, B, C=None, alpha=1.0, beta=0.0, precision=HCC_FP64):

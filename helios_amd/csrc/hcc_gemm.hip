@@ -41,23 +41,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const double* p) {
 __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, int byte_off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc16(const __bf16* p) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
-}
-// FP32 element at an FP64 byte offset of the same array (element index = off / 8)
-__device__ __forceinline__ float bload32(__amdgpu_buffer_rsrc_t r, int byte_off64) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off64 == kOobOffset ? kOobOffset
-                                                                                                 : byte_off64 >> 1,
-                                                                         0, 0));
-}
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrcf(const float* p) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, kRsrcBytes, 0x00020000);
-}
-// BF16 element at an FP64 byte offset of the same array (element index = off / 8)
-__device__ __forceinline__ unsigned short bload16(__amdgpu_buffer_rsrc_t r, int byte_off64) {
-  return __builtin_amdgcn_raw_buffer_load_b16(r, byte_off64 == kOobOffset ? kOobOffset : byte_off64 >> 2, 0, 0);
-}
-
 __device__ __forceinline__ int row_off(const Operand& o, int r) {
   return (r / o.rr) * o.srh + (r % o.rr) * o.srl;
 }
@@ -265,161 +248,6 @@ __global__ __launch_bounds__(HCC_F64_THREADS(BM, BN)) void gemm_f64_kernel(const
   }
 }
 
-// ---------------------------------------------------------------------------
-// BF16 variant: FP64 operands are rounded to BF16 while being staged into LDS,
-// multiplied on V_MFMA_F32_16X16X32_BF16 with FP32 accumulation, and the FP32
-// result is combined with C in FP64 (C = alpha*acc + beta*C).  BF16 keeps
-// FP32's exponent range, so no per-tile scaling is needed against overflow;
-// its 8-bit significand bounds the accuracy, which the solver restores with
-// FP64 residuals.  Tiles: 64x64, BK=32, one 16x16x32 MFMA step per slab.
-// ---------------------------------------------------------------------------
-constexpr int kBK16 = 32;
-constexpr int kLd16 = kBK16 + 8;   // bf16 elements per LDS row (k contiguous)
-
-__device__ __forceinline__ void load_slab16(const GemmProb& P, int m0, int n0, int k0, int t, double (&a)[8],
-                                            double (&b)[8]) {
-  const gdouble* __restrict__ A = gptr(P.A.p);
-  const gdouble* __restrict__ B = gptr(P.B.p);
-  if (P.a_kfast) {
-    const int m = m0 + (t >> 2);
-    const int kb = k0 + (t & 3) * 8;
-    const int ro = m < P.M ? row_off(P.A, m) : 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int k = kb + q;
-      a[q] = (m < P.M && k < P.K) ? A[ro + col_off(P.A, k)] : 0.0;
-    }
-  } else {
-    const int m = m0 + (t & 63);
-    const int kb = k0 + (t >> 6) * 8;
-    const int ro = m < P.M ? row_off(P.A, m) : 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int k = kb + q;
-      a[q] = (m < P.M && k < P.K) ? A[ro + col_off(P.A, k)] : 0.0;
-    }
-  }
-  if (P.b_nfast) {
-    const int n = n0 + (t & 63);
-    const int kb = k0 + (t >> 6) * 8;
-    const int co = n < P.N ? col_off(P.B, n) : 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int k = kb + q;
-      b[q] = (n < P.N && k < P.K) ? B[row_off(P.B, k) + co] : 0.0;
-    }
-  } else {
-    const int n = n0 + (t >> 2);
-    const int kb = k0 + (t & 3) * 8;
-    const int co = n < P.N ? col_off(P.B, n) : 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int k = kb + q;
-      b[q] = (n < P.N && k < P.K) ? B[row_off(P.B, k) + co] : 0.0;
-    }
-  }
-}
-
-__device__ __forceinline__ void store_slab16(const GemmProb& P, int t, const double (&a)[8], const double (&b)[8],
-                                             __bf16* As, __bf16* Bs) {
-  // A as [m][k], B as [n][k]; k contiguous so that a lane reads its 8 k-values at once
-  const int am = P.a_kfast ? (t >> 2) : (t & 63);
-  const int ak = P.a_kfast ? (t & 3) * 8 : (t >> 6) * 8;
-  const int bn = P.b_nfast ? (t & 63) : (t >> 2);
-  const int bk = P.b_nfast ? (t >> 6) * 8 : (t & 3) * 8;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    As[am * kLd16 + ak + q] = (__bf16)(float)a[q];
-    Bs[bn * kLd16 + bk + q] = (__bf16)(float)b[q];
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmProb* __restrict__ probs,
-                                                             const int* __restrict__ tile_prob,
-                                                             int total_tiles, const int* __restrict__ active) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[kBM * kLd16];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN * kLd16];
-  const int tile = logical_tile(total_tiles);
-  if (tile >= total_tiles) return;
-  GemmProb P = probs[find_problem(tile_prob, tile)];
-  if (active != nullptr && active[P.mol] == 0) return;
-  int lt = tile - P.tile_begin;
-  const int tiles_mn = ((P.M + kBM - 1) / kBM) * P.tiles_n;
-  const int ks = lt / tiles_mn;
-  lt -= ks * tiles_mn;
-  const int m0 = (lt / P.tiles_n) * kBM;
-  const int n0 = (lt % P.tiles_n) * kBN;
-  const int kbeg = ks * P.kchunk;
-  P.K = min(P.K, kbeg + P.kchunk);
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  double a[8], b[8];
-  load_slab16(P, m0, n0, kbeg, t, a, b);
-  for (int k0 = kbeg; k0 < P.K; k0 += kBK16) {
-    store_slab16(P, t, a, b, As, Bs);
-    __syncthreads();
-    if (k0 + kBK16 < P.K) load_slab16(P, m0, n0, k0 + kBK16, t, a, b);
-    const int kq = 8 * (lane >> 4);
-    bf8 a0 = *reinterpret_cast<const bf8*>(&As[(wm + (lane & 15)) * kLd16 + kq]);
-    bf8 a1 = *reinterpret_cast<const bf8*>(&As[(wm + 16 + (lane & 15)) * kLd16 + kq]);
-    bf8 b0 = *reinterpret_cast<const bf8*>(&Bs[(wn + (lane & 15)) * kLd16 + kq]);
-    bf8 b1 = *reinterpret_cast<const bf8*>(&Bs[(wn + 16 + (lane & 15)) * kLd16 + kq]);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
-    __syncthreads();
-  }
-  if (P.ksplit > 1) {
-    gdouble* __restrict__ W = gptr(P.ws + (long)ks * P.M * P.N);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = n0 + wn + 16 * j + (lane & 15);
-      if (col >= P.N) continue;
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-          if (row < P.M) W[(long)row * P.N + col] = (double)acc[i][j][r];
-        }
-    }
-    return;
-  }
-  gdouble* __restrict__ Cp = gptr(P.C.p);
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int col = n0 + wn + 16 * j + (lane & 15);
-    if (col >= P.N) continue;
-    const int co = col_off(P.C, col);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-        if (row >= P.M) continue;
-        const int o = row_off(P.C, row) + co;
-        double v = P.alpha * (double)acc[i][j][r];
-        if (P.beta != 0.0) v += P.beta * Cp[o];
-        Cp[o] = v;
-      }
-    }
-  }
-}
-
-hipError_t launch_gemm_bf16(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
-                            hipStream_t s) {
-  if (total_tiles <= 0) return hipSuccess;
-  int grid = ((total_tiles + 7) / 8) * 8;
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(grid), dim3(kThreads), 0, s, probs, tile_prob, total_tiles, active);
-  return hipGetLastError();
-}
 
 
 // ---------------------------------------------------------------------------
@@ -574,20 +402,48 @@ __global__ __launch_bounds__(256) void gemm_dual_f64(const GemmProb* __restrict_
 
 // HA / HB: the operand has a correction (dA / dB non-null for every problem
 // of the table).  A constant A needs only A0 dB, a constant B only dA B0.
-// S: tile scale -- 64S x 64S output tiles, 4 waves of 32S x 32S (S = 1, 2);
-// larger tiles halve the operand traffic per product, which bounds this kernel.
-template <int S, bool AKF, bool BNF, bool HA, bool HB>
+//
+// 64x64 output tiles, 4 waves of 32x32, 32-deep K slabs.  Staging is the
+// bound of this kernel (the matrix cores need 8 instructions per slab and
+// wave), so it is kept to a few instructions per element:
+//  * offsets are formed as row + K-table entry with no compare or select:
+//    a padding row or K carries the sentinel kPad, and any sum that includes
+//    it lands past the end of the 16- and 32-bit resources (which read 0);
+//  * K-contiguous operands are staged one element per lane with the lanes
+//    along K (32 lanes per row), so every load instruction is coalesced;
+//  * row-contiguous operands are staged with the lanes along the rows and two
+//    adjacent K per thread, so one V_CVT_PK_BF16_F32 and one 32-bit LDS store
+//    serve two elements.
+// Both go to LDS as [row][k] with a 40-element pitch and are read back as 8
+// consecutive K per lane.
+constexpr int kPad = 1 << 28;   // 2 x (element index) of a padding row or K
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_h(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, bytes, 0x00020000);
+}
+// staged element j (0 .. BK/4 - 1) of thread t (of 256) in a 64-row x BK slab
+template <bool KC, int BK>
+struct Stage {
+  static __device__ __forceinline__ int row(int t, int j) { return KC ? t / BK + (256 / BK) * j : t & 63; }
+  static __device__ __forceinline__ int k(int t, int j) { return KC ? t % BK : 2 * (t >> 6) + 8 * (j >> 1) + (j & 1); }
+};
+constexpr int kDualBK = 32;   // 64-deep slabs measured slower (K padding of the short contractions, fewer waves)
+template <bool AKF, bool BNF, bool HA, bool HB, int BK = kDualBK>
 __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict__ probs,
                                                       const int* __restrict__ tile_prob, int total_tiles,
                                                       const int* __restrict__ active) {
-  constexpr int BM = 64 * S, BN = 64 * S, T = 256, BK = 32, NE = 8 * S, LD = BK + 8, F = 2 * S;
-  constexpr int KT = 512;   // K-offset table length per chunk
+  constexpr int BM = 64, BN = 64, T = 256, LD = BK + 8, NE = BK / 4;
+  constexpr int KT = 512;   // K-table chunk
+  constexpr bool KA = AKF, KB = !BNF;   // operand K-contiguous in memory
+  using SA = Stage<KA, BK>;
+  using SB = Stage<KB, BK>;
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) __bf16 A0s[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 dAs[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 dBs[BN * LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bfs[BN * LD];
-  __shared__ int koffA[KT];
-  __shared__ int koffB[KT];
+  __shared__ int koffA[KT + BK];
+  __shared__ int koffB[KT + BK];
   const int tile = logical_tile(total_tiles);
   if (tile >= total_tiles) return;
   const GemmProb P = probs[find_problem(tile_prob, tile)];
@@ -600,95 +456,110 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   const int m0 = (lt / tiles_n) * BM, n0 = (lt % tiles_n) * BN;
   const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * (32 * S), wn = (wave & 1) * (32 * S);
-  // staged element i of this thread: row ml(i), k kq(i) of the slab (element t + i T of
-  // the [row][k] or [k][row] order of the operand's unit stride)
-  auto mlA = [&](int i) { return AKF ? t / BK + i * (T / BK) : t % BM; };
-  auto kqA = [&](int i) { return AKF ? t % BK : t / BM + i * (T / BM); };
-  auto nlB = [&](int i) { return BNF ? t % BN : t / BK + i * (T / BK); };
-  auto kqB = [&](int i) { return BNF ? t / BN + i * (T / BN) : t % BK; };
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // 2 x row offsets (kPad for rows past the problem)
   int fa[NE], fb[NE];
 #pragma unroll
-  for (int i = 0; i < NE; ++i) {
-    fa[i] = m0 + mlA(i) < P.M ? row_off(P.A, m0 + mlA(i)) : -1;
-    fb[i] = n0 + nlB(i) < P.N ? col_off(P.B, n0 + nlB(i)) : -1;
+  for (int j = 0; j < NE; ++j) {
+    fa[j] = m0 + SA::row(t, j) < P.M ? 2 * row_off(P.A, m0 + SA::row(t, j)) : kPad;
+    fb[j] = n0 + SB::row(t, j) < P.N ? 2 * col_off(P.B, n0 + SB::row(t, j)) : kPad;
   }
-  const __amdgpu_buffer_rsrc_t rA0 = rsrc16(P.hA), rdA = rsrcf(HA ? P.dA : nullptr);
-  const __amdgpu_buffer_rsrc_t rB0 = rsrc16(P.hB), rdB = rsrcf(HB ? P.dB : nullptr);
+  // BF16 resources: byte offset 2e, range 2^28 bytes; FP32: 4e, 2^29 bytes
+  // (valid element indices stay below 2^27, checked by the host)
+  const __amdgpu_buffer_rsrc_t rA0 = rsrc_h(P.hA, kPad), rdA = rsrc_h(HA ? P.dA : nullptr, 2 * kPad);
+  const __amdgpu_buffer_rsrc_t rB0 = rsrc_h(P.hB, kPad), rdB = rsrc_h(HB ? P.dB : nullptr, 2 * kPad);
   // staging registers, converted at the LDS store so that the loads stay in
   // flight across the current slab's matrix-core work: A0 (BF16 copy) and dB
   // feed A0 dB (needs HB), dA and B0 (BF16 copy) + dB feed dA (B0 + dB) (needs HA)
   unsigned short ra0[NE], rb0[NE];
   float rda[NE], rdb[NE];
-  f4 acc[F][F];
+  f4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < F; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < F; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = kbeg; c0 < kend; c0 += KT) {
-    const int cend = min(kend, c0 + KT);
+    const int clen = min(kend - c0, KT);
     __syncthreads();
-    for (int q = t; q < cend - c0; q += T) {
-      koffA[q] = col_off(P.A, c0 + q);
-      koffB[q] = row_off(P.B, c0 + q);
+    // 2 x K offsets of the chunk; entries past its end (read by the last,
+    // partial slab and the final prefetch) carry kPad
+    for (int q = t; q < KT + BK; q += T) {
+      koffA[q] = q < clen ? 2 * col_off(P.A, c0 + q) : kPad;
+      koffB[q] = q < clen ? 2 * row_off(P.B, c0 + q) : kPad;
     }
     __syncthreads();
-    // staging loads: padded elements read 0 through the out-of-range offset
-    const int klast = cend - c0 - 1;
-    auto load = [&](int k0) {
+    auto load = [&](int kl0) {
 #pragma unroll
-      for (int i = 0; i < NE; ++i) {
-        int kl = k0 - c0 + kqA(i);
-        int o = fa[i] >= 0 && kl <= klast ? (fa[i] + koffA[min(kl, klast)]) * 8 : kOobOffset;
-        if (HB) ra0[i] = bload16(rA0, o);
-        if (HA) rda[i] = bload32(rdA, o);
-        kl = k0 - c0 + kqB(i);
-        o = fb[i] >= 0 && kl <= klast ? (fb[i] + koffB[min(kl, klast)]) * 8 : kOobOffset;
-        if (HB) rdb[i] = bload32(rdB, o);
-        if (HA) rb0[i] = bload16(rB0, o);
+      for (int j = 0; j < NE; ++j) {
+        const int ea = fa[j] + koffA[kl0 + SA::k(t, j)];
+        if (HB) ra0[j] = __builtin_amdgcn_raw_buffer_load_b16(rA0, ea, 0, 0);
+        if (HA) rda[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdA, 2 * ea, 0, 0));
+        const int eb = fb[j] + koffB[kl0 + SB::k(t, j)];
+        if (HB) rdb[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdB, 2 * eb, 0, 0));
+        if (HA) rb0[j] = __builtin_amdgcn_raw_buffer_load_b16(rB0, eb, 0, 0);
       }
     };
-    load(c0);
-    for (int k0 = c0; k0 < cend; k0 += BK) {
+    // B0 + dB as FP32 (B0 is a BF16 bit pattern)
+    auto bsum = [&](int j) {
+      const float b0 = __builtin_bit_cast(float, (unsigned)rb0[j] << 16);
+      return HB ? b0 + rdb[j] : b0;
+    };
+    load(0);
+    for (int kl = 0; kl < clen; kl += BK) {
 #pragma unroll
-      for (int i = 0; i < NE; ++i) {
-        const int sa = mlA(i) * LD + kqA(i), sb = nlB(i) * LD + kqB(i);
-        if (HB) {
-          A0s[sa] = __builtin_bit_cast(__bf16, ra0[i]);
-          dBs[sb] = (__bf16)rdb[i];
+      for (int j = 0; j < NE; ++j) {
+        // A side
+        if (KA) {
+          const int sa = SA::row(t, j) * LD + SA::k(t, j);
+          if (HB) A0s[sa] = __builtin_bit_cast(__bf16, ra0[j]);
+          if (HA) dAs[sa] = (__bf16)rda[j];
+        } else if ((j & 1) == 0) {
+          const int sa = (SA::row(t, j) * LD + SA::k(t, j)) >> 1;
+          if (HB) reinterpret_cast<unsigned*>(A0s)[sa] = (unsigned)ra0[j] | ((unsigned)ra0[j + 1] << 16);
+          if (HA) reinterpret_cast<bf2*>(dAs)[sa] = __builtin_convertvector((f2{rda[j], rda[j + 1]}), bf2);
         }
-        if (HA) {
-          dAs[sa] = (__bf16)rda[i];
-          const float b0 = __builtin_bit_cast(float, (unsigned)rb0[i] << 16);
-          Bfs[sb] = (__bf16)(b0 + (HB ? rdb[i] : 0.f));
+        // B side
+        if (KB) {
+          const int sb = SB::row(t, j) * LD + SB::k(t, j);
+          if (HB) dBs[sb] = (__bf16)rdb[j];
+          if (HA) Bfs[sb] = (__bf16)bsum(j);
+        } else if ((j & 1) == 0) {
+          const int sb = (SB::row(t, j) * LD + SB::k(t, j)) >> 1;
+          if (HB) reinterpret_cast<bf2*>(dBs)[sb] = __builtin_convertvector((f2{rdb[j], rdb[j + 1]}), bf2);
+          if (HA) reinterpret_cast<bf2*>(Bfs)[sb] = __builtin_convertvector((f2{bsum(j), bsum(j + 1)}), bf2);
         }
       }
       __syncthreads();
-      load(min(k0 + BK, cend));   // the last prefetch is fully masked
-      const int kq = 8 * (lane >> 4);
-      if (HB) {
-        bf8 a[F], b[F];
+      load(min(kl + BK, clen));   // the last prefetch reads only padding
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-          a[f] = *reinterpret_cast<const bf8*>(&A0s[(wm + 16 * f + (lane & 15)) * LD + kq]);
-          b[f] = *reinterpret_cast<const bf8*>(&dBs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+      for (int kk = 0; kk < BK; kk += 32) {
+        const int kq = kk + 8 * (lane >> 4);
+        if (HB) {
+          bf8 a[2], b[2];
+#pragma unroll
+          for (int f = 0; f < 2; ++f) {
+            a[f] = *reinterpret_cast<const bf8*>(&A0s[(wm + 16 * f + (lane & 15)) * LD + kq]);
+            b[f] = *reinterpret_cast<const bf8*>(&dBs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+          }
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
+        if (HA) {
+          bf8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < F; ++i)
+          for (int f = 0; f < 2; ++f) {
+            a[f] = *reinterpret_cast<const bf8*>(&dAs[(wm + 16 * f + (lane & 15)) * LD + kq]);
+            b[f] = *reinterpret_cast<const bf8*>(&Bfs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+          }
 #pragma unroll
-          for (int j = 0; j < F; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-      }
-      if (HA) {
-        bf8 a[F], b[F];
+          for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-          a[f] = *reinterpret_cast<const bf8*>(&dAs[(wm + 16 * f + (lane & 15)) * LD + kq]);
-          b[f] = *reinterpret_cast<const bf8*>(&Bfs[(wn + 16 * f + (lane & 15)) * LD + kq]);
+            for (int j = 0; j < 2; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < F; ++i)
-#pragma unroll
-          for (int j = 0; j < F; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
       }
       __syncthreads();
     }
@@ -698,11 +569,11 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   if (P.ksplit > 1) {
     double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
 #pragma unroll
-    for (int j = 0; j < F; ++j) {
+    for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
       if (col >= P.N) continue;
 #pragma unroll
-      for (int i = 0; i < F; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -713,12 +584,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   }
   float* __restrict__ Cp = P.Cd;
 #pragma unroll
-  for (int j = 0; j < F; ++j) {
+  for (int j = 0; j < 2; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
     if (col >= P.N) continue;
     const int co = col_off(P.C, col);
 #pragma unroll
-    for (int i = 0; i < F; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -736,23 +607,20 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tile
   if (total_tiles <= 0) return hipSuccess;
   const int grid = ((total_tiles + 7) / 8) * 8;
   if (bf16) {
-#define HCC_DUAL16(SC, AK, BN)                                                                          \
+#define HCC_DUAL16(AK, BN)                                                                              \
   switch ((flags >> 2) & 3) {                                                                           \
-    case 1: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 2: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 3: hipLaunchKernelGGL((gemm_dual_bf16<SC, AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
     default: return hipErrorInvalidValue; /* both operands constant: not a correction product */     \
   }
-#define HCC_DUAL16_S(SC)                        \
-  switch (flags & 3) {                          \
-    case 0: HCC_DUAL16(SC, false, false) break; \
-    case 1: HCC_DUAL16(SC, true, false) break;  \
-    case 2: HCC_DUAL16(SC, false, true) break;  \
-    default: HCC_DUAL16(SC, true, true) break;  \
-  }
-    if (tile == 256) { HCC_DUAL16_S(2) }
-    else { HCC_DUAL16_S(1) }
-#undef HCC_DUAL16_S
+    if (tile != 64) return hipErrorInvalidValue;
+    switch (flags & 3) {
+      case 0: HCC_DUAL16(false, false) break;
+      case 1: HCC_DUAL16(true, false) break;
+      case 2: HCC_DUAL16(false, true) break;
+      default: HCC_DUAL16(true, true) break;
+    }
 #undef HCC_DUAL16
   } else {
     if (tile != 64) return hipErrorInvalidValue;

@@ -58,23 +58,10 @@ struct GemmTable {
 };
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
-// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide); of the
-// BF16 correction GEMM: 64 (64x64) and 256 (128x128).
-static int tile_m(int kind) { return kind == 128 || kind == 256 ? 128 : kind; }
-static int tile_n(int kind) { return kind == 128 ? 32 : (kind == 256 ? 128 : kind); }
-
-// 128x128 correction tiles need ~260 registers per lane (one wave per SIMD);
-// measured slower than 64x64 tiles at four waves per SIMD, so off.
-constexpr bool kDualBigTiles = false;
-
-// Tile kind of a correction-form problem: 128x128 tiles carry half the operand
-// traffic per product (the bound of that kernel) and win unless the padding
-// of the larger tile costs more.
-static int choose_tile_dual(const GemmProb& p) {
-  const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
-  const double t128 = (double)((p.M + 127) / 128) * ((p.N + 127) / 128) * 128 * 128;
-  return kDualBigTiles && p.M >= 128 && p.N >= 128 && t128 < 1.5 * t64 ? 256 : 64;
-}
+// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide).  The
+// BF16 correction GEMM uses 64x64 tiles only.
+static int tile_m(int kind) { return kind == 128 ? 128 : kind; }
+static int tile_n(int kind) { return kind == 128 ? 32 : kind; }
 
 // Split K when a problem has few output tiles and a long K: aim for >= 16
 // work items per problem with K chunks of at least 128.
@@ -283,8 +270,8 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
 // force_tile / force_ks (> 0) override both for every problem.
 static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
                             int force_tile = 0, int op = -1, int force_ks = 0, bool dual = false) {
-  const int kinds[4] = {64, 32, 128, 256};
-  std::vector<GemmProb> groups[4];
+  const int kinds[3] = {64, 32, 128};
+  std::vector<GemmProb> groups[3];
   for (int k = 0; k < b.nm; ++k) {
     GemmProb p = build(k);
     if (p.M <= 0 || p.N <= 0) continue;
@@ -300,18 +287,18 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
         ks = it->second[1];
       }
     }
-    if (dual) ts = choose_tile_dual(p);
+    if (dual) ts = 64;
     else if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
     if (!allow_split) force_split(p, 1);
     else if (ks >= 1) force_split(p, ks);
     else choose_split(p, ts);
-    groups[ts == 64 ? 0 : (ts == 32 ? 1 : (ts == 128 ? 2 : 3))].push_back(p);
+    groups[ts == 64 ? 0 : (ts == 32 ? 1 : 2)].push_back(p);
   }
   // the first non-empty group is the table; the others hang off it through alt links
   GemmTable head;
   bool have = false;
   int last = -1;   // index in b.alt of the last linked part, -1: the head
-  for (int g = 0; g < 4; ++g) {
+  for (int g = 0; g < 3; ++g) {
     if (groups[g].empty()) continue;
     GemmTable t = build_group(b, groups[g], kinds[g]);
     if (!have) {
@@ -942,6 +929,14 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   b->tuning = c->tuning.empty() ? nullptr : &c->tuning;
   for (int k = 0; k < nmol; ++k) {
     if (n[k] < 2 || nocc[k] < 1 || nocc[k] >= n[k]) throw std::string("invalid molecule dimensions");
+    // GEMM operands are addressed through buffer resources with 32-bit byte
+    // offsets: every per-molecule array stays below 2^28 elements
+    const long nn = n[k], oo = nocc[k], vv = n[k] - nocc[k];
+    const long largest = std::max({vv * vv * vv * vv, oo * oo * oo * oo, nn * nn * nn, oo * oo * vv * vv, oo * vv * vv * vv});
+    if (largest >= (1L << 28)) throw std::string("molecule too large: a per-molecule array exceeds 2^28 elements");
+    // the correction GEMM forms 2 x (element index) in 29 bits
+    if (b->mixed && largest >= (1L << 27))
+      throw std::string("molecule too large for the mixed solver: a per-molecule array exceeds 2^27 elements");
     b->n.push_back(n[k]);
     b->o.push_back(nocc[k]);
     b->v.push_back(n[k] - nocc[k]);
@@ -1639,7 +1634,7 @@ const char* hcc_last_error(const hcc_ctx* ctx) {
 int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C, double alpha, double beta,
                       int precision) {
   if (m < 1 || n < 1 || k < 1 || !A || !B || !C) return HCC_E_ARG;
-  if (precision != HCC_FP64 && precision != HCC_MIXED) return HCC_E_ARG;
+  if (precision != HCC_FP64) return HCC_E_ARG;
   double *dA = nullptr, *dB = nullptr, *dC = nullptr;
   GemmProb* dp = nullptr;
   int* dtp = nullptr;
@@ -1663,8 +1658,7 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
     HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
     HCHECK(hipMalloc(&dtp, sizeof(int) * tiles));
     HCHECK(hipMemset(dtp, 0, sizeof(int) * tiles));
-    if (precision == HCC_FP64) HCHECK(launch_gemm_f64(dp, dtp, tiles, nullptr, nullptr, 64, 3));
-    else HCHECK(launch_gemm_bf16(dp, dtp, tiles, nullptr, nullptr));
+    HCHECK(launch_gemm_f64(dp, dtp, tiles, nullptr, nullptr, 64, 3));
     HCHECK(hipDeviceSynchronize());
     HCHECK(hipMemcpy(C, dC, sizeof(double) * m * n, hipMemcpyDeviceToHost));
   } catch (const HipError&) {
@@ -1675,6 +1669,84 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
   (void)hipFree(dC);
   (void)hipFree(dp);
   (void)hipFree(dtp);
+  return rc;
+}
+
+int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, const double* A0, const float* dA,
+                                 const double* B0, const float* dB, float* Cd, double alpha, double beta, int reps,
+                                 double* ms) {
+  if (m < 1 || n < 1 || k < 1 || !A0 || !B0 || !Cd || (!dA && !dB) || reps < 0) return HCC_E_ARG;
+  if ((long)m * k >= (1L << 27) || (long)k * n >= (1L << 27) || (long)m * n >= (1L << 27)) return HCC_E_ARG;
+  double *a0 = nullptr, *b0 = nullptr;
+  __bf16 *ha = nullptr, *hb = nullptr;
+  float *da = nullptr, *db = nullptr, *cd = nullptr;
+  GemmProb* dp = nullptr;
+  int* dtp = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HCC_OK;
+  try {
+    const long na = (long)m * k, nb = (long)k * n, nc = (long)m * n;
+    HCHECK(hipMalloc(&a0, sizeof(double) * na));
+    HCHECK(hipMalloc(&b0, sizeof(double) * nb));
+    HCHECK(hipMalloc(&ha, sizeof(__bf16) * na));
+    HCHECK(hipMalloc(&hb, sizeof(__bf16) * nb));
+    HCHECK(hipMalloc(&cd, sizeof(float) * nc));
+    HCHECK(hipMemcpy(a0, A0, sizeof(double) * na, hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(b0, B0, sizeof(double) * nb, hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(cd, Cd, sizeof(float) * nc, hipMemcpyHostToDevice));
+    if (dA) {
+      HCHECK(hipMalloc(&da, sizeof(float) * na));
+      HCHECK(hipMemcpy(da, dA, sizeof(float) * na, hipMemcpyHostToDevice));
+    }
+    if (dB) {
+      HCHECK(hipMalloc(&db, sizeof(float) * nb));
+      HCHECK(hipMemcpy(db, dB, sizeof(float) * nb, hipMemcpyHostToDevice));
+    }
+    to_bf16(a0, ha, na, nullptr);
+    to_bf16(b0, hb, nb, nullptr);
+    // A: m x k row-major (K contiguous) or its transpose stored k x m; B: k x n
+    // row-major (N contiguous) or its transpose stored n x k
+    GemmProb p = prob(a_kfast ? rm(a0, k) : trn(a0, m), b_nfast ? rm(b0, n) : trn(b0, k), rm(nullptr, n), m, n, k,
+                      alpha, beta);
+    p.dA = da;
+    p.dB = db;
+    p.Cd = cd;
+    p.hA = ha;
+    p.hB = hb;
+    p.tiles_n = (n + 63) / 64;
+    p.tile_begin = 0;
+    p.a_kfast = a_kfast ? 1 : 0;
+    p.b_nfast = b_nfast ? 1 : 0;
+    p.ksplit = 1;
+    p.kchunk = k;
+    const int tiles = ((m + 63) / 64) * p.tiles_n;
+    const int flags = (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (dA ? 4 : 0) | (dB ? 8 : 0);
+    HCHECK(hipMalloc(&dp, sizeof(GemmProb)));
+    HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
+    HCHECK(hipMalloc(&dtp, sizeof(int) * tiles));
+    HCHECK(hipMemset(dtp, 0, sizeof(int) * tiles));
+    HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, 64));
+    HCHECK(hipDeviceSynchronize());
+    HCHECK(hipMemcpy(Cd, cd, sizeof(float) * nc, hipMemcpyDeviceToHost));
+    if (ms) *ms = 0.0;
+    if (reps > 0) {
+      HCHECK(hipEventCreate(&e0));
+      HCHECK(hipEventCreate(&e1));
+      HCHECK(hipEventRecord(e0, nullptr));
+      for (int r = 0; r < reps; ++r) HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, 64));
+      HCHECK(hipEventRecord(e1, nullptr));
+      HCHECK(hipEventSynchronize(e1));
+      float t = 0.f;
+      HCHECK(hipEventElapsedTime(&t, e0, e1));
+      if (ms) *ms = t / reps;
+    }
+  } catch (const HipError&) {
+    rc = HCC_E_HIP;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* q : {(void*)a0, (void*)b0, (void*)ha, (void*)hb, (void*)da, (void*)db, (void*)cd, (void*)dp, (void*)dtp})
+    (void)hipFree(q);
   return rc;
 }
 

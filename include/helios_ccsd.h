@@ -222,9 +222,23 @@ int hcc_mlp_evaluate(hcc_mlp* net, double* loss);
 int hcc_mlp_predict(hcc_mlp* net, const float* features, long count, float* y);
 
 /* Self-test of the grouped GEMM engine on row-major A (m x k), B (k x n):
- * C = alpha A B + beta C.  No reference counterpart; used by the tests. */
+ * C = alpha A B + beta C with the FP64 matrix-core kernel (precision must be
+ * HCC_FP64).  No reference counterpart; used by the tests. */
 int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C,
                       double alpha, double beta, int precision);
+
+/* Self-test of the mixed solver's correction-form GEMM (BF16 matrix cores,
+ * FP32 accumulation): Cd = alpha (A0 dB + dA (B0 + dB)) + beta Cd on one
+ * problem.  A0 (double, rounded to BF16 on the device) is m x k row-major
+ * when a_kfast, else its transpose stored k x m; B0 is k x n row-major when
+ * b_nfast, else its transpose stored n x k; dA / dB (float) share those
+ * layouts and one of them may be null (constant operand).  Cd is m x n
+ * row-major.  With reps > 0 the launch is repeated reps more times and *ms
+ * receives the mean kernel time.  Every matrix must stay below 2^27
+ * elements.  No reference counterpart; used by the tests. */
+int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, const double* A0, const float* dA,
+                                 const double* B0, const float* dB, float* Cd, double alpha, double beta, int reps,
+                                 double* ms);
 
 #ifdef __cplusplus
 }
