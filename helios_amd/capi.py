@@ -54,7 +54,7 @@ SIGNATURES = {
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
     "hcc_selftest_gemm_correction": (_i, [_i, _i, _i, _i, _i, _P(_d), _P(_f), _P(_d), _P(_f), _P(_f), _d, _d, _i,
-                                          _P(_d)]),
+                                          _i, _P(_d)]),
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
     "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
     "hcc_set_graphs": (_i, [_vp, _i]),
@@ -360,7 +360,7 @@ class Trainer:
         return y
 
 
-def selftest_gemm_correction(A0, dA, B0, dB, Cd, alpha=1.0, beta=0.0, a_kfast=True, b_nfast=True, reps=0):
+def selftest_gemm_correction(A0, dA, B0, dB, Cd, alpha=1.0, beta=0.0, a_kfast=True, b_nfast=True, tile=64, reps=0):
     """Cd' = alpha (A0 dB + dA (B0 + dB)) + beta Cd on the correction GEMM kernel.
     A0, dA: m x k; B0, dB: k x n (logical shapes; dA or dB may be None).
     Returns (Cd', mean ms of reps extra launches)."""
@@ -377,7 +377,7 @@ def selftest_gemm_correction(A0, dA, B0, dB, Cd, alpha=1.0, beta=0.0, a_kfast=Tr
     fp = lambda x: None if x is None else x.ctypes.data_as(_P(_f))
     ms = ctypes.c_double(0.0)
     rc = lib.hcc_selftest_gemm_correction(m, n, k, int(a_kfast), int(b_nfast), _dp(a0), fp(da), _dp(b0), fp(db),
-                                          fp(cd), alpha, beta, reps, ctypes.byref(ms))
+                                          fp(cd), alpha, beta, tile, reps, ctypes.byref(ms))
     if rc != 0:
         raise HccError(f"hcc_selftest_gemm_correction failed ({rc})")
     return cd, ms.value

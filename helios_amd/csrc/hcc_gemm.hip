@@ -41,11 +41,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const double* p) {
 __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, int byte_off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
+// r / radix for 0 <= r < kMaxIndex, 1 <= radix < kMaxRadix (exact, see Operand)
+__device__ __forceinline__ int quot(int r, double inv) { return (int)(((double)r + 0.5) * inv); }
 __device__ __forceinline__ int row_off(const Operand& o, int r) {
-  return (r / o.rr) * o.srh + (r % o.rr) * o.srl;
+  const int q = quot(r, o.ir);
+  return q * o.srh + (r - q * o.rr) * o.srl;
 }
 __device__ __forceinline__ int col_off(const Operand& o, int c) {
-  return (c / o.rc) * o.sch + (c % o.rc) * o.scl;
+  const int q = quot(c, o.ic);
+  return q * o.sch + (c - q * o.rc) * o.scl;
 }
 
 // Problem owning global tile `tile`: one load from the table's tile map
@@ -420,22 +424,29 @@ constexpr int kPad = 1 << 28;   // 2 x (element index) of a padding row or K
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_h(const void* p, int bytes) {
   return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, bytes, 0x00020000);
 }
-// staged element j (0 .. BK/4 - 1) of thread t (of 256) in a 64-row x BK slab
-template <bool KC, int BK>
+// staged element j (0 .. 7) of thread t (of T) in an R-row x BK slab
+// (R * BK = 8 T): K-contiguous operands with the lanes along K,
+// row-contiguous ones with the lanes along the rows and K in adjacent pairs
+template <bool KC, int BK, int R, int T>
 struct Stage {
-  static __device__ __forceinline__ int row(int t, int j) { return KC ? t / BK + (256 / BK) * j : t & 63; }
-  static __device__ __forceinline__ int k(int t, int j) { return KC ? t % BK : 2 * (t >> 6) + 8 * (j >> 1) + (j & 1); }
+  static __device__ __forceinline__ int row(int t, int j) { return KC ? t / BK + (T / BK) * j : t % R; }
+  static __device__ __forceinline__ int k(int t, int j) {
+    return KC ? t % BK : 2 * (t / R) + 2 * (T / R) * (j >> 1) + (j & 1);
+  }
 };
 constexpr int kDualBK = 32;   // 64-deep slabs measured slower (K padding of the short contractions, fewer waves)
-template <bool AKF, bool BNF, bool HA, bool HB, int BK = kDualBK>
-__global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict__ probs,
-                                                      const int* __restrict__ tile_prob, int total_tiles,
-                                                      const int* __restrict__ active) {
-  constexpr int BM = 64, BN = 64, T = 256, LD = BK + 8, NE = BK / 4;
-  constexpr int KT = 512;   // K-table chunk
+// TS: output tile edge, 64 (4 waves of 32x32) or 128 (8 waves of 32x64)
+template <bool AKF, bool BNF, bool HA, bool HB, int TS>
+__global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(4, 8))) void gemm_dual_bf16(
+    const GemmProb* __restrict__ probs, const int* __restrict__ tile_prob, int total_tiles,
+    const int* __restrict__ active) {
+  constexpr int BK = kDualBK, BM = TS, BN = TS, T = TS * 4, LD = BK + 8, NE = 8;
+  constexpr int FM = 2, FN = TS / 32;   // 16x16 blocks per wave along M and N
+  constexpr int KT = 512;               // K-table chunk
   constexpr bool KA = AKF, KB = !BNF;   // operand K-contiguous in memory
-  using SA = Stage<KA, BK>;
-  using SB = Stage<KB, BK>;
+  static_assert(BM * BK == NE * T, "staging covers the slab");
+  using SA = Stage<KA, BK, BM, T>;
+  using SB = Stage<KB, BK, BN, T>;
   typedef float f2 __attribute__((ext_vector_type(2)));
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) __bf16 A0s[BM * LD];
@@ -456,7 +467,8 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   const int m0 = (lt / tiles_n) * BM, n0 = (lt % tiles_n) * BN;
   const int kbeg = ks * P.kchunk, kend = min(P.K, kbeg + P.kchunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // waves: 2 along N (32 FN columns each), T / 128 along M (32 rows each)
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * (16 * FN);
   // 2 x row offsets (kPad for rows past the problem)
   int fa[NE], fb[NE];
 #pragma unroll
@@ -473,11 +485,11 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   // feed A0 dB (needs HB), dA and B0 (BF16 copy) + dB feed dA (B0 + dB) (needs HA)
   unsigned short ra0[NE], rb0[NE];
   float rda[NE], rdb[NE];
-  f4 acc[2][2];
+  f4 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = kbeg; c0 < kend; c0 += KT) {
     const int clen = min(kend - c0, KT);
     __syncthreads();
@@ -534,32 +546,21 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 32) {
         const int kq = kk + 8 * (lane >> 4);
-        if (HB) {
-          bf8 a[2], b[2];
+        // product with A from Xa, B from Xb (LDS [row][k])
+        auto product = [&](const __bf16* Xa, const __bf16* Xb) {
+          bf8 a[FM], b[FN];
 #pragma unroll
-          for (int f = 0; f < 2; ++f) {
-            a[f] = *reinterpret_cast<const bf8*>(&A0s[(wm + 16 * f + (lane & 15)) * LD + kq]);
-            b[f] = *reinterpret_cast<const bf8*>(&dBs[(wn + 16 * f + (lane & 15)) * LD + kq]);
-          }
+          for (int f = 0; f < FM; ++f) a[f] = *reinterpret_cast<const bf8*>(&Xa[(wm + 16 * f + (lane & 15)) * LD + kq]);
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
+          for (int f = 0; f < FN; ++f) b[f] = *reinterpret_cast<const bf8*>(&Xb[(wn + 16 * f + (lane & 15)) * LD + kq]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
               acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        if (HA) {
-          bf8 a[2], b[2];
-#pragma unroll
-          for (int f = 0; f < 2; ++f) {
-            a[f] = *reinterpret_cast<const bf8*>(&dAs[(wm + 16 * f + (lane & 15)) * LD + kq]);
-            b[f] = *reinterpret_cast<const bf8*>(&Bfs[(wn + 16 * f + (lane & 15)) * LD + kq]);
-          }
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+        };
+        if (HB) product(A0s, dBs);
+        if (HA) product(dAs, Bfs);
       }
       __syncthreads();
     }
@@ -569,11 +570,11 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   if (P.ksplit > 1) {
     double* __restrict__ W = P.ws + (long)ks * P.M * P.N;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < FN; ++j) {
       const int col = n0 + wn + 16 * j + (lane & 15);
       if (col >= P.N) continue;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -584,12 +585,12 @@ __global__ __launch_bounds__(256) void gemm_dual_bf16(const GemmProb* __restrict
   }
   float* __restrict__ Cp = P.Cd;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < FN; ++j) {
     const int col = n0 + wn + 16 * j + (lane & 15);
     if (col >= P.N) continue;
     const int co = col_off(P.C, col);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
@@ -607,20 +608,24 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tile
   if (total_tiles <= 0) return hipSuccess;
   const int grid = ((total_tiles + 7) / 8) * 8;
   if (bf16) {
-#define HCC_DUAL16(AK, BN)                                                                              \
+#define HCC_DUAL16(AK, BN, TS)                                                                          \
   switch ((flags >> 2) & 3) {                                                                           \
-    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
-    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true>), dim3(grid), dim3(256), 0, s, probs, tp, total_tiles, active); break; \
+    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
+    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
+    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
     default: return hipErrorInvalidValue; /* both operands constant: not a correction product */     \
   }
-    if (tile != 64) return hipErrorInvalidValue;
-    switch (flags & 3) {
-      case 0: HCC_DUAL16(false, false) break;
-      case 1: HCC_DUAL16(true, false) break;
-      case 2: HCC_DUAL16(false, true) break;
-      default: HCC_DUAL16(true, true) break;
+#define HCC_DUAL16_T(TS)                           \
+    switch (flags & 3) {                           \
+      case 0: HCC_DUAL16(false, false, TS) break;  \
+      case 1: HCC_DUAL16(true, false, TS) break;   \
+      case 2: HCC_DUAL16(false, true, TS) break;   \
+      default: HCC_DUAL16(true, true, TS) break;   \
     }
+    if (tile == 64) { HCC_DUAL16_T(64) }
+    else if (tile == 256) { HCC_DUAL16_T(128) }
+    else return hipErrorInvalidValue;
+#undef HCC_DUAL16_T
 #undef HCC_DUAL16
   } else {
     if (tile != 64) return hipErrorInvalidValue;

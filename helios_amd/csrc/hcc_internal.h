@@ -31,10 +31,19 @@ struct HipError {
 
 struct Operand {
   double* p;        // base of this molecule's block
-  int rr, rc;       // radices of the row and column composite indices (>= 1)
+  int rr, rc;       // radices of the row and column composite indices (1 .. kMaxRadix)
   int srh, srl;     // row offset = (r / rr) * srh + (r % rr) * srl
   int sch, scl;     // col offset = (c / rc) * sch + (c % rc) * scl
+  double ir, ic;    // 1 / rr and 1 / rc (FP64, round to nearest)
 };
+// Composite indices stay below kMaxIndex and radices below kMaxRadix, so the
+// kernels' quotient floor((r + 1/2) * (1 / radix)) is exact: (r + 1/2) / radix
+// lies at least 2^-17 from an integer, the FP64 rounding error is below 2^-25.
+constexpr int kMaxRadix = 1 << 16;
+constexpr int kMaxIndex = 1 << 27;
+inline Operand make_operand(double* p, int rr, int rc, int srh, int srl, int sch, int scl) {
+  return Operand{p, rr, rc, srh, srl, sch, scl, 1.0 / rr, 1.0 / rc};
+}
 
 struct GemmProb {
   Operand A, B, C;  // A: M x K, B: K x N, C: M x N

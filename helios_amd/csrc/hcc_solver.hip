@@ -34,11 +34,12 @@ namespace hcc {
 // ---------------------------------------------------------------------------
 // operand helpers
 // ---------------------------------------------------------------------------
-static Operand rm(double* p, int ld) { return Operand{p, 1, 1, ld, 0, 1, 0}; }
-static Operand trn(double* p, int ld) { return Operand{p, 1, 1, 1, 0, ld, 0}; }
+static Operand rm(double* p, int ld) { return make_operand(p, 1, 1, ld, 0, 1, 0); }
+static Operand trn(double* p, int ld) { return make_operand(p, 1, 1, 1, 0, ld, 0); }
 // composite: row = (r / rr, r % rr) with strides (srh, srl), col likewise
 static Operand cmp(double* p, int rr, int srh, int srl, int rc, int sch, int scl) {
-  return Operand{p, rr, rc, srh, srl, sch, scl};
+  if (rr < 1 || rc < 1 || rr >= kMaxRadix || rc >= kMaxRadix) throw std::string("composite radix out of range");
+  return make_operand(p, rr, rc, srh, srl, sch, scl);
 }
 
 static bool unit_cols(const Operand& o) { return (o.rc == 1 ? o.sch : o.scl) == 1; }
@@ -58,10 +59,20 @@ struct GemmTable {
 };
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
-// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide).  The
-// BF16 correction GEMM uses 64x64 tiles only.
-static int tile_m(int kind) { return kind == 128 ? 128 : kind; }
-static int tile_n(int kind) { return kind == 128 ? 32 : kind; }
+// 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide); of the
+// BF16 correction GEMM: 64 (64x64, 4 waves) and 256 (128x128, 8 waves).
+static int tile_m(int kind) { return kind == 128 || kind == 256 ? 128 : kind; }
+static int tile_n(int kind) { return kind == 128 ? 32 : (kind == 256 ? 128 : kind); }
+
+// Tile kind of a correction-form problem.  128x128 tiles halve the operand
+// staging per product and win on long contractions wherever their padding
+// stays close to that of 64x64 tiles; short contractions (K < 128) are bound
+// by their output stores and stay on 64x64 tiles, in one launch.
+static int choose_tile_dual(const GemmProb& p) {
+  const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
+  const double t128 = (double)((p.M + 127) / 128) * ((p.N + 127) / 128) * 128 * 128;
+  return p.K >= 128 && p.M >= 128 && p.N >= 128 && t128 <= 1.3 * t64 ? 256 : 64;
+}
 
 // Split K when a problem has few output tiles and a long K: aim for >= 16
 // work items per problem with K chunks of at least 128.
@@ -270,11 +281,12 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
 // force_tile / force_ks (> 0) override both for every problem.
 static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build, bool allow_split = true,
                             int force_tile = 0, int op = -1, int force_ks = 0, bool dual = false) {
-  const int kinds[3] = {64, 32, 128};
-  std::vector<GemmProb> groups[3];
+  const int kinds[4] = {64, 32, 128, 256};
+  std::vector<GemmProb> groups[4];
   for (int k = 0; k < b.nm; ++k) {
     GemmProb p = build(k);
     if (p.M <= 0 || p.N <= 0) continue;
+    if (p.M >= kMaxIndex || p.N >= kMaxIndex || p.K >= kMaxIndex) throw std::string("GEMM dimension out of range");
     p.mol = k;
     int ts = 0, ks = -1;
     if (force_tile) {
@@ -287,18 +299,18 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
         ks = it->second[1];
       }
     }
-    if (dual) ts = 64;
+    if (dual) ts = force_tile ? force_tile : choose_tile_dual(p);
     else if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
     if (!allow_split) force_split(p, 1);
     else if (ks >= 1) force_split(p, ks);
     else choose_split(p, ts);
-    groups[ts == 64 ? 0 : (ts == 32 ? 1 : 2)].push_back(p);
+    groups[ts == 64 ? 0 : (ts == 32 ? 1 : (ts == 128 ? 2 : 3))].push_back(p);
   }
   // the first non-empty group is the table; the others hang off it through alt links
   GemmTable head;
   bool have = false;
   int last = -1;   // index in b.alt of the last linked part, -1: the head
-  for (int g = 0; g < 3; ++g) {
+  for (int g = 0; g < 4; ++g) {
     if (groups[g].empty()) continue;
     GemmTable t = build_group(b, groups[g], kinds[g]);
     if (!have) {
@@ -1634,6 +1646,7 @@ const char* hcc_last_error(const hcc_ctx* ctx) {
 int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C, double alpha, double beta,
                       int precision) {
   if (m < 1 || n < 1 || k < 1 || !A || !B || !C) return HCC_E_ARG;
+  if (m >= kMaxIndex || n >= kMaxIndex || k >= kMaxIndex) return HCC_E_ARG;
   if (precision != HCC_FP64) return HCC_E_ARG;
   double *dA = nullptr, *dB = nullptr, *dC = nullptr;
   GemmProb* dp = nullptr;
@@ -1673,9 +1686,11 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
 }
 
 int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, const double* A0, const float* dA,
-                                 const double* B0, const float* dB, float* Cd, double alpha, double beta, int reps,
-                                 double* ms) {
+                                 const double* B0, const float* dB, float* Cd, double alpha, double beta, int tile,
+                                 int reps, double* ms) {
   if (m < 1 || n < 1 || k < 1 || !A0 || !B0 || !Cd || (!dA && !dB) || reps < 0) return HCC_E_ARG;
+  if (tile != 64 && tile != 128) return HCC_E_ARG;
+  const int kind = tile == 128 ? 256 : 64;
   if ((long)m * k >= (1L << 27) || (long)k * n >= (1L << 27) || (long)m * n >= (1L << 27)) return HCC_E_ARG;
   double *a0 = nullptr, *b0 = nullptr;
   __bf16 *ha = nullptr, *hb = nullptr;
@@ -1713,19 +1728,19 @@ int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, 
     p.Cd = cd;
     p.hA = ha;
     p.hB = hb;
-    p.tiles_n = (n + 63) / 64;
+    p.tiles_n = (n + tile - 1) / tile;
     p.tile_begin = 0;
     p.a_kfast = a_kfast ? 1 : 0;
     p.b_nfast = b_nfast ? 1 : 0;
     p.ksplit = 1;
     p.kchunk = k;
-    const int tiles = ((m + 63) / 64) * p.tiles_n;
+    const int tiles = ((m + tile - 1) / tile) * p.tiles_n;
     const int flags = (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (dA ? 4 : 0) | (dB ? 8 : 0);
     HCHECK(hipMalloc(&dp, sizeof(GemmProb)));
     HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
     HCHECK(hipMalloc(&dtp, sizeof(int) * tiles));
     HCHECK(hipMemset(dtp, 0, sizeof(int) * tiles));
-    HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, 64));
+    HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, kind));
     HCHECK(hipDeviceSynchronize());
     HCHECK(hipMemcpy(Cd, cd, sizeof(float) * nc, hipMemcpyDeviceToHost));
     if (ms) *ms = 0.0;
@@ -1733,7 +1748,7 @@ int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, 
       HCHECK(hipEventCreate(&e0));
       HCHECK(hipEventCreate(&e1));
       HCHECK(hipEventRecord(e0, nullptr));
-      for (int r = 0; r < reps; ++r) HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, 64));
+      for (int r = 0; r < reps; ++r) HCHECK(launch_gemm_dual(dp, dtp, tiles, nullptr, nullptr, flags, 1, kind));
       HCHECK(hipEventRecord(e1, nullptr));
       HCHECK(hipEventSynchronize(e1));
       float t = 0.f;

@@ -36,11 +36,13 @@ def _bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64).reshape(a.shape)
 
 
+@pytest.mark.parametrize("tile", [64, 128])
 @pytest.mark.parametrize("a_kfast", [True, False])
 @pytest.mark.parametrize("b_nfast", [True, False])
 @pytest.mark.parametrize("m,n,k,which", [(64, 64, 32, "AB"), (70, 50, 100, "AB"), (5, 300, 7, "AB"),
-                                         (129, 65, 600, "AB"), (70, 50, 100, "A"), (70, 50, 100, "B")])
-def test_gemm_correction(m, n, k, which, a_kfast, b_nfast):
+                                         (129, 65, 600, "AB"), (70, 50, 100, "A"), (70, 50, 100, "B"),
+                                         (300, 257, 45, "AB")])
+def test_gemm_correction(m, n, k, which, a_kfast, b_nfast, tile):
     """Cd += alpha (A0 dB + dA (B0 + dB)): operands rounded to BF16 (B0 + dB
     summed in FP32 first), FP32 accumulation, against float64 products of the
     same rounded operands."""
@@ -50,7 +52,8 @@ def test_gemm_correction(m, n, k, which, a_kfast, b_nfast):
     dA = (1e-2 * rng.standard_normal((m, k))).astype(np.float32) if "A" in which else None
     dB = (1e-2 * rng.standard_normal((k, n))).astype(np.float32) if "B" in which else None
     C0 = rng.standard_normal((m, n)).astype(np.float32)
-    C, _ = capi.selftest_gemm_correction(A0, dA, B0, dB, C0, alpha=0.5, beta=-1.0, a_kfast=a_kfast, b_nfast=b_nfast)
+    C, _ = capi.selftest_gemm_correction(A0, dA, B0, dB, C0, alpha=0.5, beta=-1.0, a_kfast=a_kfast, b_nfast=b_nfast,
+                                         tile=tile)
     b0 = _bf16(B0)
     terms, mags = [], []
     if dB is not None:
