@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--no-tuning", action="store_true", help="ignore the measured GEMM tile/split table")
     ap.add_argument("--skip-reference", action="store_true",
                     help="profiler runs only: no FP64 reference solve, solves count as converged by status alone")
-    ap.add_argument("--kappa", type=float, default=0.06)
+    ap.add_argument("--kappa", type=float, default=0.1)
     ap.add_argument("--max-inner", type=int, default=40)
     ap.add_argument("--max-outer", type=int, default=20)
     args = ap.parse_args()

@@ -57,6 +57,7 @@ SIGNATURES = {
                                           _i, _P(_d)]),
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
     "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
+    "hcc_set_mixed_history": (_i, [_vp, _i]),
     "hcc_set_graphs": (_i, [_vp, _i]),
     "hcc_tune_gemm": (_i, [_vp, _i, _i, _i, _i, _P(_d)]),
     "hcc_gemm_shape": (_i, [_vp, _i, _i, _P(_i)]),
@@ -146,7 +147,7 @@ class Context:
         self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
                     "hcc_set_options")
 
-    def set_mixed(self, bf16=True, kappa=0.06, max_inner=40, max_outer=20):
+    def set_mixed(self, bf16=True, kappa=0.1, max_inner=40, max_outer=20):
         self._check(self.lib.hcc_set_mixed(self.h, int(bf16), kappa, max_inner, max_outer), "hcc_set_mixed")
 
     def batch_statistics(self):
@@ -242,6 +243,10 @@ class Context:
         pre = np.zeros(nmol)
         self._check(self.lib.hcc_gemm_flops(self.h, _dp(it), _dp(pre)), "hcc_gemm_flops")
         return it, pre
+
+    def set_mixed_history(self, keep=True):
+        """Mixed solver: keep the inner DIIS history across outer points."""
+        self._check(self.lib.hcc_set_mixed_history(self.h, int(keep)), "hcc_set_mixed_history")
 
     def set_graphs(self, enable=True):
         self._check(self.lib.hcc_set_graphs(self.h, int(enable)), "hcc_set_graphs")

@@ -40,7 +40,9 @@ void activate(State* st, int* act, int nm, hipStream_t s);
 void mixed_dress(const Mol* m0, const MolD* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,
                  hipStream_t s);
 void mixed_zero(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
-void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s);
+// t0 += dt (keep_diis: the stored inner DIIS corrections are shifted by -dt first)
+void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, Ragged rgP2, int nm, bool keep_diis,
+                  hipStream_t s);
 void mixed_update(const Mol* m0, const MolD* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                   double* part1, double* part2, hipStream_t s);
 void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, int max_outer, Ragged rgOV,

@@ -797,6 +797,32 @@ __global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restri
   else M.t1[e - n2] += dmols[mol].t1[e - n2];
 }
 
+// Before t0 += dt: the stored inner DIIS amplitude vectors are corrections
+// relative to t0, so each loses dt (their residuals stay valid).  Packed
+// layout as in k_diis_extrap.
+__global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restrict__ mols, const State* __restrict__ st,
+                                                          Ragged rg, int nm) {
+  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v * (1L + (long)mols[mol].o * mols[mol].v));
+  const State& S = st[mol];
+  if (e >= size_ || S.status != 0 || S.diis_count == 0) return;
+  const int o = M.o, v = M.v;
+  const long ne = (long)o * v;
+  long p;
+  double d;
+  if (e < ne) {
+    p = e;
+    d = M.t1[e];
+  } else {
+    const long f = e - ne;
+    const long mm = f / ne, nn = f % ne;
+    if (mm > nn) return;
+    p = ne + packed_index(mm, nn, ne);
+    const int i = mm / v, a = mm % v, j = nn / v, b = nn % v;
+    d = M.t2[(((long)i * o + j) * v + a) * v + b];
+  }
+  for (int k = 0; k < S.diis_count; ++k) M.st[(long)k * M.nd + p] -= d;
+}
+
 // mixed update, singles: Omega1 = (w1_0 incl. F~_ai) + dw1 + dF~_ai; T = t1_0 + dt1
 __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                              const State* __restrict__ st, Ragged rg, int nm, Params prm,
@@ -893,8 +919,10 @@ __global__ __launch_bounds__(kEwThreads) void k_outer_control(const Mol* __restr
   S.outer_its += 1;
   S.inner = 0;
   S.inner_its = 0;
-  S.diis_count = 0;
-  S.diis_next = 0;
+  if (!prm.keep_diis) {
+    S.diis_count = 0;
+    S.diis_next = 0;
+  }
   S.staged = 0;
   if (S.acc0 < prm.eps) { S.status = 1; act[mol] = 0; return; }
   if (!(S.acc0 < prm.divergence)) { S.status = -2; act[mol] = 0; return; }
@@ -982,7 +1010,9 @@ void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s) {
 void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s) {
   RAGGED_LAUNCH(k_zero_t1, rgOV, mols, rgOV, nm);
 }
-void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, int nm, hipStream_t s) {
+void mixed_commit(const Mol* m0, const MolD* md, const State* st, Ragged rgT2S, Ragged rgP2, int nm, bool keep_diis,
+                  hipStream_t s) {
+  if (keep_diis) RAGGED_LAUNCH(k_diis_shift, rgP2, md, st, rgP2, nm);
   RAGGED_LAUNCH(k_commit_delta, rgT2S, m0, md, st, rgT2S, nm);
 }
 void mixed_update(const Mol* m0, const MolD* md, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,

@@ -735,7 +735,8 @@ struct hcc_ctx {
   double last_iters = 0.0;
   // mixed-precision solver options
   int mixed_bf16 = 1;
-  double kappa = 0.06;
+  int keep_diis = 1;   // mixed solver: inner DIIS history kept across outer steps
+  double kappa = 0.1;
   int max_inner = 40;
   int max_outer = 20;
   // live timing of the contraction GEMM launches (HIP events on `stream`)
@@ -1039,6 +1040,12 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
 }
 
 static int need_batch(hcc_ctx* ctx);
+
+int hcc_set_mixed_history(hcc_ctx* ctx, int keep) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  ctx->keep_diis = keep ? 1 : 0;
+  return HCC_OK;
+}
 
 int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer) {
   if (ctx == nullptr) return HCC_E_ARG;
@@ -1359,6 +1366,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   Params prm = params(ctx, eps, maxiter, true);
   prm.kappa = ctx->kappa;
   prm.max_inner = ctx->max_inner;
+  prm.keep_diis = ctx->keep_diis;
   // The inner solve is a fixed-point iteration on the correction equation,
   // not the reference's iteration history: it takes the Jacobi step dt = Omega/D
   // (the '2-1' update (2 d2 + d2^T)/D is 3 Omega/D, hence 1/3 there).
@@ -1409,7 +1417,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       }
       if (alive() == 0) break;
     }
-    mixed_commit(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
+    mixed_commit(b.dm, b.ddm, b.dst, b.rgT2S, b.rgP2, b.nm, prm.keep_diis && prm.diis, s);
   }
   set_phase(b.dst, b.dact, b.nm, 0, s);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);

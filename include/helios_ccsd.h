@@ -62,8 +62,14 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
  * max|d| < kappa * max|d(t0)| or max_inner iterations; at most max_outer
  * outer points.  The inner iterations take the Jacobi step dt = Omega/D with
  * DIIS (the t1/t2 steps of hcc_set_options apply to FP64 solves only).
- * Defaults: 1, 0.06, 40, 20. */
+ * Defaults: 1, 0.1, 40, 20. */
 int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer);
+
+/* Mixed solver: keep (1, default) or restart (0) the inner DIIS history at
+ * each outer point.  Kept vectors are corrections relative to t0 and are
+ * shifted by the committed correction; their residuals stay valid to the
+ * BF16 error of the correction.  No reference counterpart. */
+int hcc_set_mixed_history(hcc_ctx* ctx, int keep);
 
 /* Per molecule of the last mixed-precision batch solve: FP64 residual
  * evaluations (outer points) and inner iterations. */

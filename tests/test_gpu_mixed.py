@@ -6,7 +6,12 @@
   corrections, so two inner solves reach the FP64 fixed point and the third
   FP64 residual evaluation confirms it -- a wrong correction step shows up as
   extra outer evaluations or a different energy;
-* BF16 mode: energies within 1e-6 eV of the FP64 energies of the same inputs.
+* BF16 mode: energies within 1e-6 eV of the FP64 energies of the same inputs,
+  with the inner DIIS history restarted or kept across outer points;
+* kept history in exact mode: the stored corrections are shifted at every
+  commit, so the kept vectors stay consistent -- the solve reaches the same
+  fixed point and needs no more inner iterations than with a restarted
+  history (a wrong shift extrapolates from inconsistent vectors).
 """
 import numpy as np
 import pytest
@@ -46,11 +51,34 @@ def test_exact_correction_reaches_fixed_point():
     assert np.abs(e[:, 0] - ref).max() < 1e-8
 
 
-def test_bf16_within_1e6_ev():
+def test_kept_history_exact_mode():
+    mols = batch([3, 7, 12, 20])
+    ref, _ = fp64_energies(mols)
+    inner_total = {}
+    for keep in (False, True):
+        ctx = capi.Context(0, capi.HCC_MIXED, 8)
+        ctx.set_mixed(bf16=False, kappa=0.1, max_inner=40, max_outer=30)
+        ctx.set_mixed_history(keep)
+        ctx.batch_load(mols)
+        ctx.batch_solve(capi.GUESS_MP2, 2000, 1e-8)
+        e, it, st = ctx.batch_results()
+        outer, inner = ctx.batch_statistics()
+        ctx.close()
+        print("keep", keep, "outer", outer, "inner", inner, "dE", e[:, 0] - ref)
+        assert (st == 1).all()
+        # residuals converged to 1e-8 leave energies a few 1e-9 from the reference
+        assert np.abs(e[:, 0] - ref).max() < 3e-8
+        inner_total[keep] = int(inner.sum())
+    assert inner_total[True] <= inner_total[False]
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_bf16_within_1e6_ev(keep):
     mols = batch([3, 4, 5, 7, 12, 20])
     ref, it64 = fp64_energies(mols)
     ctx = capi.Context(0, capi.HCC_MIXED, 8)
-    ctx.set_mixed(bf16=True, kappa=0.03, max_inner=40, max_outer=12)
+    ctx.set_mixed(bf16=True, kappa=0.1, max_inner=40, max_outer=20)
+    ctx.set_mixed_history(keep)
     ctx.batch_load(mols)
     ctx.batch_solve(capi.GUESS_MP2, 2000, 2e-7)
     e, it, st = ctx.batch_results()
