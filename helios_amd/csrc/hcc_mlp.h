@@ -16,8 +16,9 @@ constexpr int kMlpParams = 5313;
 
 // Ov[p][q] = n * sum_mu C[mu][p]^2 C[mu][q]^2 for every molecule
 void mlp_overlaps(const Mol* mols, Ragged rgNN, int nm, hipStream_t s);
-// t2 = t_mp2 + y / Dp for every amplitude (rgT2L: 1024 amplitudes per workgroup)
-void mlp_guess(const Mol* mols, Ragged rgT2L, int nm, const float* params, hipStream_t s);
+// t2 = t_mp2 + y / Dp for every amplitude, one evaluation per amplitude pair
+// (rgPairL: 1024 pairs (i a) <= (j b) per workgroup)
+void mlp_guess(const Mol* mols, Ragged rgPairL, int nm, const float* params, hipStream_t s);
 
 // device view of a loaded batch, for the trainer (hcc_solver.hip)
 struct BatchView {

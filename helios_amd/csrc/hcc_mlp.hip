@@ -201,39 +201,53 @@ __global__ __launch_bounds__(256) void k_mlp_overlap(const Mol* __restrict__ mol
 }
 
 // every amplitude of the molecule: 4 waves x 16 groups x 16 amplitudes per workgroup
+// One network evaluation per amplitude pair: work item p enumerates the pairs
+// (mm, nn) = ((i a), (j b)) with mm <= nn (p = nn (nn + 1) / 2 + mm), and the
+// value is written to t2[i][j][a][b] and its mirror t2[j][i][b][a].
 __global__ __launch_bounds__(256) void k_mlp_guess(const Mol* __restrict__ mols, Ragged rg, int nm,
                                                    const float* __restrict__ w) {
   const int mol = rg.bmol[blockIdx.x];
   const Mol& M = mols[mol];
   const int o = M.o, v = M.v;
-  const long n2 = (long)o * o * v * v;
+  const long ne = (long)o * v, npair = ne * (ne + 1) / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
   const long base = (long)(blockIdx.x - rg.prefix[mol]) * 1024 + wave * 256;
   Net net;
   net_load(net, w, lane);
   Acts unused;
   for (int grp = 0; grp < 16; ++grp) {
-    const long e = base + grp * 16 + n;
-    const bool ok = e < n2;
+    const long p = base + grp * 16 + n;
+    const bool ok = p < npair;
     float f[kMlpF];
 #pragma unroll
     for (int q = 0; q < kMlpF; ++q) f[q] = 0.f;
     double t = 0.0, Dp = 1.0;
-    if (ok && g < 2) {
-      int i, j, a, b;
-      decode(e, o, v, i, j, a, b);
-      amp_features(M, i, j, a, b, f, t, Dp);
+    long e = 0, em = 0;
+    if (ok) {
+      long nn = (long)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+      while (nn * (nn + 1) / 2 > p) --nn;
+      while ((nn + 1) * (nn + 2) / 2 <= p) ++nn;
+      const long mm = p - nn * (nn + 1) / 2;
+      const int i = (int)(mm / v), a = (int)(mm % v), j = (int)(nn / v), b = (int)(nn % v);
+      e = (((long)i * o + j) * v + a) * v + b;
+      em = (((long)j * o + i) * v + b) * v + a;
+      if (g < 2) amp_features(M, i, j, a, b, f, t, Dp);
     }
     const float y = net_forward<false>(net, pack_x(f, g), unused);
-    if (ok && g == 0) M.t2[e] = t + (double)y / Dp;
+    if (ok && g == 0) {
+      const double val = t + (double)y / Dp;
+      M.t2[e] = val;
+      M.t2[em] = val;
+    }
   }
 }
 
 void mlp_overlaps(const Mol* mols, Ragged rgNN, int nm, hipStream_t s) {
   if (rgNN.nblocks > 0) hipLaunchKernelGGL(k_mlp_overlap, dim3(rgNN.nblocks), dim3(256), 0, s, mols, rgNN, nm);
 }
-void mlp_guess(const Mol* mols, Ragged rgT2L, int nm, const float* params, hipStream_t s) {
-  if (rgT2L.nblocks > 0) hipLaunchKernelGGL(k_mlp_guess, dim3(rgT2L.nblocks), dim3(256), 0, s, mols, rgT2L, nm, params);
+void mlp_guess(const Mol* mols, Ragged rgPairL, int nm, const float* params, hipStream_t s) {
+  if (rgPairL.nblocks > 0)
+    hipLaunchKernelGGL(k_mlp_guess, dim3(rgPairL.nblocks), dim3(256), 0, s, mols, rgPairL, nm, params);
 }
 
 // ---------------------------------------------------------------------------

@@ -115,7 +115,7 @@ struct Batch {
   State* dst = nullptr;
   int* dact = nullptr;
   std::vector<void*> allocs;
-  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgT2L;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL;
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   std::vector<GemmTable> alt;   // second (other tile edge) parts of split contraction tables
@@ -501,7 +501,8 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgNN = make_ragged(b, sz_nn);
   b.rgN3 = make_ragged(b, sz_n3);
   b.rgT2 = make_ragged(b, sz_t2);
-  b.rgT2L = make_ragged(b, sz_t2, 4);   // learned guess: 1024 amplitudes per workgroup
+  // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
+  b.rgPairL = make_ragged(b, [](int, int o, int v) { const long ne = (long)o * v; return ne * (ne + 1) / 2; }, 4);
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
   b.rgOV = make_ragged(b, sz_ov);
   b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
@@ -912,7 +913,7 @@ static void apply_guess(hcc_ctx* c, int kind) {
   guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind == HCC_GUESS_LEARNED ? HCC_GUESS_MP2 : kind, c->stream);
   if (kind == HCC_GUESS_LEARNED) {
     mlp_overlaps(b.dm, b.rgNN, b.nm, c->stream);
-    mlp_guess(b.dm, b.rgT2L, b.nm, c->guess_net, c->stream);
+    mlp_guess(b.dm, b.rgPairL, b.nm, c->guess_net, c->stream);
     HCHECK(hipGetLastError());
   }
 }
