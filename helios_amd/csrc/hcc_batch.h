@@ -8,7 +8,9 @@ namespace hcc {
 // Device pointers of one molecule's blocks in the per-buffer arenas.  V is
 // the element type of the buffers that change with the amplitudes: double for
 // the solver state (Mol), float for the corrections of the mixed-precision
-// solver (MolD), whose inputs, DIIS storage and constants stay FP64.
+// solver (MolD), whose inputs and constants stay FP64.  The DIIS vectors
+// follow V (MolD's alias the FP64 solver's storage, unused by the mixed
+// solver); the B matrix stays FP64.
 template <class V>
 struct MolT {
   int n, o, v, m;          // sites, occupied, virtual, DIIS depth
@@ -26,7 +28,8 @@ struct MolT {
   double *Gklcd, *Lldkc, *Lt;
   V *gvvov, *gooov, *Xvv, *Xoo;
   // DIIS history: m slots of nd (amplitudes st, errors sd); B matrix m x m
-  double *st, *sd, *Bm;
+  V *st, *sd;
+  double *Bm;
   // guess features / MP2 amplitudes (o^2 v^2)
   double *t2mp2;
   // dressed-density diagonal and its Coulomb field (2n)

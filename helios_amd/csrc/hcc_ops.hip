@@ -362,13 +362,13 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const MT* __restrict__ 
     return;
   }
   const long base = (long)lb * kEwThreads * kDotItems;
-  const double* snew = M.sd + (long)S.diis_next * M.nd;
+  const auto* snew = M.sd + (long)S.diis_next * M.nd;
   for (int k = 0; k < S.diis_count; ++k) {
-    const double* sk = M.sd + (long)k * M.nd;
+    const auto* sk = M.sd + (long)k * M.nd;
     double acc = 0.0;
     for (int it = 0; it < kDotItems; ++it) {
       const long e = base + (long)it * kEwThreads + threadIdx.x;
-      if (e < M.nd) acc += sk[e] * snew[e];
+      if (e < M.nd) acc += (double)sk[e] * (double)snew[e];
     }
     const double r = block_sum(acc, sh);
     if (threadIdx.x == 0) part[(long)blockIdx.x * m + k] = r;
@@ -843,8 +843,8 @@ __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restr
     amax = fabs(d1);
     if (prm.diis) {
       const State& S = st[mol];
-      M.sd[(long)S.diis_next * M.nd + e] = d1;
-      M.st[(long)S.diis_next * M.nd + e] = dt;
+      D.sd[(long)S.diis_next * M.nd + e] = (float)d1;
+      D.st[(long)S.diis_next * M.nd + e] = dt;
     }
   }
   const double r = block_max(amax, sh);
@@ -887,8 +887,8 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
       if (prm.diis) {
         const State& S = st[mol];
         const long p = ne + packed_index(mm, nn, ne);
-        M.sd[(long)S.diis_next * M.nd + p] = num;
-        M.st[(long)S.diis_next * M.nd + p] = dt;
+        Dm.sd[(long)S.diis_next * M.nd + p] = (float)num;
+        Dm.st[(long)S.diis_next * M.nd + p] = dt;
       }
     }
   }

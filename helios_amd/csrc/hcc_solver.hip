@@ -469,7 +469,10 @@ static void allocate(Batch& b, bool keep_mp2) {
       D.n = H.n; D.o = H.o; D.v = H.v; D.m = H.m; D.nd = H.nd; D.eref = H.eref;
       D.C = H.C; D.G = H.G; D.F = H.F; D.Fao = H.Fao;
       D.Gklcd = H.Gklcd; D.Lldkc = H.Lldkc; D.Lt = H.Lt;
-      D.st = H.st; D.sd = H.sd; D.Bm = H.Bm; D.t2mp2 = H.t2mp2; D.Ov = H.Ov;
+      // FP32 DIIS vectors in the (otherwise unused) FP64 ring of the molecule
+      D.st = reinterpret_cast<float*>(H.st);
+      D.sd = reinterpret_cast<float*>(H.sd);
+      D.Bm = H.Bm; D.t2mp2 = H.t2mp2; D.Ov = H.Ov;
     }
     for (const auto& f : variable_fields()) {
       long total = 0;
