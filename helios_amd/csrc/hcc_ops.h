@@ -16,6 +16,12 @@ void ew_u(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s)
 void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
 void ew_init_xx(const MolD* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
 void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
+// x += t2 Xvv^T - Xoo^T t2 (the E2 terms, both index placements) for molecules
+// with o, v <= kE2Max (the others take the GEMM path); rgOcc: one block per
+// occupied orbital
+constexpr int kE2Max = 32;
+void ew_e2(const Mol* mols, const State* st, Ragged rgOcc, int nm, hipStream_t s);
+void mixed_e2(const Mol* m0, const MolD* md, const State* st, Ragged rgOcc, int nm, hipStream_t s);
 void ew_assemble(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
 void ew_update(const Mol* mols, const State* st, Ragged rgOV, Ragged rgT2, int nm, const Params& prm,
                double* part1, double* part2, hipStream_t s);

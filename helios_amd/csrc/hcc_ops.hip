@@ -823,6 +823,87 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restric
   for (int k = 0; k < S.diis_count; ++k) M.st[(long)k * M.nd + p] -= d;
 }
 
+// ---------------------------------------------------------------------------
+// The two thin contractions of the doubles residual,
+//   x[i][j][a][b] += sum_c t2[i][j][a][c] Xvv[b][c] - sum_k t2[i][k][a][b] Xoo[k][j],
+// (K = v and K = o, outputs v and o wide) as one elementwise pass for
+// molecules with o, v <= kE2Max.  One workgroup per (molecule, i): Xvv and Xoo
+// stay in LDS, and for each a the slab S[j][c] = t2[i][j][a][c] feeds both
+// sums (the second reads it as S[k][b]).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_e2(const Mol* __restrict__ mols, const State* __restrict__ st,
+                                                  Ragged rg, int nm) {
+  __shared__ double sX[kE2Max * (kE2Max + 1)], sO[kE2Max * (kE2Max + 1)], sS[kE2Max * kE2Max];
+  const int mol = rg.bmol[blockIdx.x];
+  const int i = blockIdx.x - rg.prefix[mol];
+  const Mol& M = mols[mol];
+  const int o = M.o, v = M.v;
+  if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
+  for (int q = threadIdx.x; q < v * v; q += kEwThreads) sX[(q / v) * (v + 1) + q % v] = M.Xvv[q];
+  for (int q = threadIdx.x; q < o * o; q += kEwThreads) sO[(q / o) * (o + 1) + q % o] = M.Xoo[q];
+  for (int a = 0; a < v; ++a) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < o * v; q += kEwThreads)
+      sS[q] = M.t2[(((long)i * o + q / v) * v + a) * v + q % v];
+    __syncthreads();
+    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
+      const int j = q / v, b = q % v;
+      double s = 0.0;
+      for (int c = 0; c < v; ++c) s += sS[j * v + c] * sX[b * (v + 1) + c];
+      for (int k = 0; k < o; ++k) s -= sS[k * v + b] * sO[k * (o + 1) + j];
+      M.x[(((long)i * o + j) * v + a) * v + b] += s;
+    }
+  }
+}
+
+// Correction form of k_e2 (FP32): with X1 = X0 + dX,
+//   dx += sum_c (t2_0 dXvv + dt2 Xvv1) - sum_k (t2_0 dXoo + dt2 Xoo1)
+// -- the exact change of the FP64 terms up to FP32 rounding.
+__global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
+                                                    const State* __restrict__ st, Ragged rg, int nm) {
+  constexpr int kP = kE2Max * (kE2Max + 1);
+  __shared__ float sXd[kP], sX1[kP], sOd[kP], sO1[kP], sS0[kE2Max * kE2Max], sSd[kE2Max * kE2Max];
+  const int mol = rg.bmol[blockIdx.x];
+  const int i = blockIdx.x - rg.prefix[mol];
+  const Mol& M = mols[mol];
+  const MolD& D = dmols[mol];
+  const int o = M.o, v = M.v;
+  if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
+  for (int q = threadIdx.x; q < v * v; q += kEwThreads) {
+    const int r = (q / v) * (v + 1) + q % v;
+    sXd[r] = D.Xvv[q];
+    sX1[r] = (float)(M.Xvv[q] + (double)D.Xvv[q]);
+  }
+  for (int q = threadIdx.x; q < o * o; q += kEwThreads) {
+    const int r = (q / o) * (o + 1) + q % o;
+    sOd[r] = D.Xoo[q];
+    sO1[r] = (float)(M.Xoo[q] + (double)D.Xoo[q]);
+  }
+  for (int a = 0; a < v; ++a) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
+      const long e = (((long)i * o + q / v) * v + a) * v + q % v;
+      sS0[q] = (float)M.t2[e];
+      sSd[q] = D.t2[e];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
+      const int j = q / v, b = q % v;
+      float s = 0.f;
+      for (int c = 0; c < v; ++c) s += sS0[j * v + c] * sXd[b * (v + 1) + c] + sSd[j * v + c] * sX1[b * (v + 1) + c];
+      for (int k = 0; k < o; ++k) s -= sS0[k * v + b] * sOd[k * (o + 1) + j] + sSd[k * v + b] * sO1[k * (o + 1) + j];
+      D.x[(((long)i * o + j) * v + a) * v + b] += s;
+    }
+  }
+}
+
+void ew_e2(const Mol* mols, const State* st, Ragged rgOcc, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_e2, rgOcc, mols, st, rgOcc, nm);
+}
+void mixed_e2(const Mol* m0, const MolD* md, const State* st, Ragged rgOcc, int nm, hipStream_t s) {
+  RAGGED_LAUNCH(k_e2_d, rgOcc, m0, md, st, rgOcc, nm);
+}
+
 // mixed update, singles: Omega1 = (w1_0 incl. F~_ai) + dw1 + dF~_ai; T = t1_0 + dt1
 __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                              const State* __restrict__ st, Ragged rg, int nm, Params prm,

@@ -115,7 +115,7 @@ struct Batch {
   State* dst = nullptr;
   int* dact = nullptr;
   std::vector<void*> allocs;
-  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL, rgOcc;
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   std::vector<GemmTable> alt;   // second (other tile edge) parts of split contraction tables
@@ -504,6 +504,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgNN = make_ragged(b, sz_nn);
   b.rgN3 = make_ragged(b, sz_n3);
   b.rgT2 = make_ragged(b, sz_t2);
+  b.rgOcc = make_ragged(b, [](int, int o, int) { return (long)o * kEwThreads; });   // one block per i
   // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
   b.rgPairL = make_ragged(b, [](int, int o, int v) { const long ne = (long)o * v; return ne * (ne + 1) / 2; }, 4);
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
@@ -683,14 +684,16 @@ static void build_tables(Batch& b) {
     return prob(cmp(M.Gklcd, 1, v * v, 0, v * v, ov * v, 1), cmp(M.u, v * v, ov * v, 1, 1, v * v, 0),
                 rm(M.Xoo, o), o, o, ov * v, 1.0, 1.0);
   });
-  // g23: x += t2 Xvv^T
+  // g23: x += t2 Xvv^T, and g24: x -= t2[i][k][ab] Xoo[k][j]; molecules with
+  // o, v <= kE2Max take both in the elementwise pass k_e2 instead (empty problems here)
   add([](const Mol& M) {
     const int o = M.o, v = M.v;
+    if (o <= kE2Max && v <= kE2Max) return GemmProb{};
     return prob(rm(M.t2, v), trn(M.Xvv, v), rm(M.x, v), o * o * v, v, v, 1.0, 1.0);
   });
-  // g24: x -= t2[i][k][ab] Xoo[k][j]
   add([](const Mol& M) {
     const int o = M.o, v = M.v, ov = o * v;
+    if (o <= kE2Max && v <= kE2Max) return GemmProb{};
     return prob(cmp(M.t2, v * v, ov * v, 1, 1, v * v, 0), rm(M.Xoo, o), cmp(M.x, v * v, ov * v, 1, 1, v * v, 0),
                 ov * v, o, o, -1.0, 1.0);
   });
@@ -702,6 +705,7 @@ static void build_delta_tables(Batch& b) {
   for (const auto& f : b.ops) {
     b.dit.push_back(make_table(b, [&](int k) {
       GemmProb p = f(b.hm[k]);
+      if (p.M <= 0 || p.N <= 0) return p;   // handled outside the GEMM tables
       p.dA = delta_of(b, k, p.A.p);
       p.dB = delta_of(b, k, p.B.p);
       p.Cd = delta_of(b, k, p.C.p);
@@ -837,6 +841,7 @@ static void residual(hcc_ctx* c) {
   for (int g = G3; g <= G20; ++g) gemm(c, b.it[g], g);
   ew_init_xx(b.dm, b.dst, b.rgXX, b.nm, s);
   for (int g = G21; g <= G24; ++g) gemm(c, b.it[g], g);
+  ew_e2(b.dm, b.dst, b.rgOcc, b.nm, s);
   ew_assemble(b.dm, b.dst, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
 }
@@ -889,6 +894,7 @@ static void residual_delta(hcc_ctx* c) {
   for (int g = G3; g <= G20; ++g) dgemm(c, b.dit[g], 32 + g);
   ew_init_xx(b.ddm, b.dst, b.rgXX, b.nm, s);
   for (int g = G21; g <= G24; ++g) dgemm(c, b.dit[g], 32 + g);
+  mixed_e2(b.dm, b.ddm, b.dst, b.rgOcc, b.nm, s);
   ew_assemble(b.ddm, b.dst, b.rgT2, b.nm, s);
   HCHECK(hipGetLastError());
 }
