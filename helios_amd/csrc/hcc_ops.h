@@ -17,9 +17,10 @@ void ew_init_xx(const Mol* mols, const State* st, Ragged rgXX, int nm, hipStream
 void ew_init_xx(const MolD* mols, const State* st, Ragged rgXX, int nm, hipStream_t s);
 void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);
 // x += t2 Xvv^T - Xoo^T t2 (the E2 terms, both index placements) for molecules
-// with o, v <= kE2Max (the others take the GEMM path); rgOcc: one block per
-// occupied orbital
+// with o, v <= kE2Max (the others take the GEMM path); rgOcc: kE2Split blocks
+// per occupied orbital
 constexpr int kE2Max = 32;
+constexpr int kE2Split = 4;
 void ew_e2(const Mol* mols, const State* st, Ragged rgOcc, int nm, hipStream_t s);
 void mixed_e2(const Mol* m0, const MolD* md, const State* st, Ragged rgOcc, int nm, hipStream_t s);
 void ew_assemble(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);

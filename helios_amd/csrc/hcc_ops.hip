@@ -827,21 +827,22 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restric
 // The two thin contractions of the doubles residual,
 //   x[i][j][a][b] += sum_c t2[i][j][a][c] Xvv[b][c] - sum_k t2[i][k][a][b] Xoo[k][j],
 // (K = v and K = o, outputs v and o wide) as one elementwise pass for
-// molecules with o, v <= kE2Max.  One workgroup per (molecule, i): Xvv and Xoo
-// stay in LDS, and for each a the slab S[j][c] = t2[i][j][a][c] feeds both
-// sums (the second reads it as S[k][b]).
+// molecules with o, v <= kE2Max.  kE2Split workgroups per (molecule, i), each
+// over a range of a: Xvv and Xoo stay in LDS, and for each a the slab
+// S[j][c] = t2[i][j][a][c] feeds both sums (the second reads it as S[k][b]).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kEwThreads) void k_e2(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                   Ragged rg, int nm) {
   __shared__ double sX[kE2Max * (kE2Max + 1)], sO[kE2Max * (kE2Max + 1)], sS[kE2Max * kE2Max];
   const int mol = rg.bmol[blockIdx.x];
-  const int i = blockIdx.x - rg.prefix[mol];
+  const int lb = blockIdx.x - rg.prefix[mol], i = lb / kE2Split, part = lb % kE2Split;
   const Mol& M = mols[mol];
   const int o = M.o, v = M.v;
   if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
+  const int na = (v + kE2Split - 1) / kE2Split, a0 = part * na, a1 = min(v, a0 + na);
   for (int q = threadIdx.x; q < v * v; q += kEwThreads) sX[(q / v) * (v + 1) + q % v] = M.Xvv[q];
   for (int q = threadIdx.x; q < o * o; q += kEwThreads) sO[(q / o) * (o + 1) + q % o] = M.Xoo[q];
-  for (int a = 0; a < v; ++a) {
+  for (int a = a0; a < a1; ++a) {
     __syncthreads();
     for (int q = threadIdx.x; q < o * v; q += kEwThreads)
       sS[q] = M.t2[(((long)i * o + q / v) * v + a) * v + q % v];
@@ -864,11 +865,12 @@ __global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mol
   constexpr int kP = kE2Max * (kE2Max + 1);
   __shared__ float sXd[kP], sX1[kP], sOd[kP], sO1[kP], sS0[kE2Max * kE2Max], sSd[kE2Max * kE2Max];
   const int mol = rg.bmol[blockIdx.x];
-  const int i = blockIdx.x - rg.prefix[mol];
+  const int lb = blockIdx.x - rg.prefix[mol], i = lb / kE2Split, part = lb % kE2Split;
   const Mol& M = mols[mol];
   const MolD& D = dmols[mol];
   const int o = M.o, v = M.v;
   if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
+  const int na = (v + kE2Split - 1) / kE2Split, a0 = part * na, a1 = min(v, a0 + na);
   for (int q = threadIdx.x; q < v * v; q += kEwThreads) {
     const int r = (q / v) * (v + 1) + q % v;
     sXd[r] = D.Xvv[q];
@@ -879,7 +881,7 @@ __global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mol
     sOd[r] = D.Xoo[q];
     sO1[r] = (float)(M.Xoo[q] + (double)D.Xoo[q]);
   }
-  for (int a = 0; a < v; ++a) {
+  for (int a = a0; a < a1; ++a) {
     __syncthreads();
     for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
       const long e = (((long)i * o + q / v) * v + a) * v + q % v;

@@ -504,7 +504,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgNN = make_ragged(b, sz_nn);
   b.rgN3 = make_ragged(b, sz_n3);
   b.rgT2 = make_ragged(b, sz_t2);
-  b.rgOcc = make_ragged(b, [](int, int o, int) { return (long)o * kEwThreads; });   // one block per i
+  b.rgOcc = make_ragged(b, [](int, int o, int) { return (long)o * kE2Split * kEwThreads; });   // kE2Split blocks per i
   // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
   b.rgPairL = make_ragged(b, [](int, int o, int v) { const long ne = (long)o * v; return ne * (ne + 1) / 2; }, 4);
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
