@@ -125,6 +125,7 @@ struct Batch {
     long count;        // elements spanned by the arena
     __bf16* copy;      // BF16 copy, same element offsets
     bool variable;     // changes at every outer point
+    bool used;         // an operand of some correction GEMM (refreshed only then)
   };
   std::vector<Shadow> shadows;
   // measured tile/split choices per (contraction, n_occ, n_virt); null: cost heuristic
@@ -379,9 +380,12 @@ static const std::vector<FieldDef>& constant_fields() {
 }
 
 // BF16 copy of an outer-point operand element, or null when p lies in no arena
-static const __bf16* shadow_of(const Batch& b, const double* p) {
-  for (const auto& sh : b.shadows)
-    if (p >= sh.base && p < sh.base + sh.count) return sh.copy + (p - sh.base);
+static const __bf16* shadow_of(Batch& b, const double* p) {
+  for (auto& sh : b.shadows)
+    if (p >= sh.base && p < sh.base + sh.count) {
+      sh.used = true;
+      return sh.copy + (p - sh.base);
+    }
   return nullptr;
 }
 
@@ -491,7 +495,7 @@ static void allocate(Batch& b, bool keep_mp2) {
       const Mol& last = b.hm[b.nm - 1];
       double* base = first.*(f.member);
       const long count = (last.*(f.member) - base) + f.size(last.n, last.o, last.v);
-      b.shadows.push_back({base, count, b.dalloc<__bf16>(count), variable});
+      b.shadows.push_back({base, count, b.dalloc<__bf16>(count), variable, false});
     };
     for (const auto& f : variable_fields()) add_shadow(f, true);
     for (const auto& f : constant_fields()) add_shadow(f, false);
@@ -1355,7 +1359,7 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
 // BF16 copies of the outer-point operands (variable arenas; constants too when asked)
 static void refresh_shadows(hcc_ctx* c, bool constants) {
   for (const auto& sh : c->batch->shadows)
-    if (sh.variable || constants) to_bf16(sh.base, sh.copy, sh.count, c->stream);
+    if (sh.used && (sh.variable || constants)) to_bf16(sh.base, sh.copy, sh.count, c->stream);
   HCHECK(hipGetLastError());
 }
 
