@@ -29,11 +29,13 @@ from helios_amd import autotune, capi, learned_guess, molecules, ppp  # noqa: E4
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
-# HBM bytes per gemm_f64_kernel dispatch in the FP64 solve of the default 4096-molecule
-# batch: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy), two rocprofv3
-# --pmc passes summarised in profiles/hbm_traffic_fp64_batch4096.md.  A counter pass
+# HBM bytes per dispatch of the dominant kernels on the default 4096-molecule batch
+# with the default solver settings: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
+# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --steps 1 --warmup 0 --no-cpu
+# --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  A counter pass
 # cannot run inside this process, so other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 2.879e9
+GEMM_TRAFFIC_BATCH4096 = 2.843e9        # gemm_f64_kernel, FP64 solve
+DUAL_GEMM_TRAFFIC_BATCH4096 = 1.591e9   # gemm_dual_bf16, mixed solve
 
 
 def _prep(k):
@@ -179,6 +181,8 @@ def main():
                   file=sys.stderr, flush=True)
     head = rmx if rmx is not None else r64
 
+    default_workload = (args.nmol == 4096 and args.diis == 8 and args.eps == 2e-7 and args.kappa == 0.1
+                        and args.max_inner == 40 and args.max_outer == 20 and not args.no_tuning)
     # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
     gemm_flops = args.steps * float((r64["fl_it"] * r64["iters"]).sum() + r64["fl_pre"].sum())
     achieved = gemm_flops / (r64["gemm_ms"] * 1e-3)
@@ -186,7 +190,7 @@ def main():
 
     rfp64 = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
              "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-             "traffic": GEMM_TRAFFIC_BATCH4096 if (args.nmol == 4096 and args.diis == 8) else None,
+             "traffic": GEMM_TRAFFIC_BATCH4096 if default_workload else None,
              "kernel": "gemm_f64_kernel (FP64 solver)", "launches": int(r64["launches"]),
              "kernel_ms": r64["gemm_ms"], "gemm_flops": gemm_flops}
     roofline = rfp64
@@ -196,7 +200,7 @@ def main():
         corr_flops = args.steps * float((rmx["fl_corr"] * rmx["inner"]).sum())
         ach = corr_flops / (corr_ms * 1e-3)
         roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_BF16, "traffic": None,
+                    "frac": ach / PEAK_BF16, "traffic": DUAL_GEMM_TRAFFIC_BATCH4096 if default_workload else None,
                     "kernel": "gemm_dual_bf16 (mixed solver inner iterations; bound in practice by operand "
                               "delivery, see roofline_fp64_solver for the FP64 GEMM)",
                     "kernel_ms": corr_ms, "gemm_flops": corr_flops}
