@@ -65,6 +65,7 @@ struct Params {
   double kappa;          // mixed solver: inner solve stops at acc < kappa * acc0
   int max_inner;         // mixed solver: inner iterations per outer step
   int keep_diis;         // mixed solver: the inner DIIS history survives outer steps
+  int outer_only;        // mixed solver: the update reads the outer residual only (dt = 0 step)
   double divergence;     // divergence threshold
   double s1, s2;         // t1/t2 steps
   int projtype;          // 1 or 2 (the reference's '2-1')

@@ -918,7 +918,9 @@ __global__ __launch_bounds__(kEwThreads) void k_update1_mixed(const Mol* __restr
     const MolD& D = dmols[mol];
     const int N = M.n, o = M.o, v = M.v;
     const int i = e / v, a = e % v;
-    const double d1 = M.w1[e] + D.w1[e] + D.Ft[(long)(o + a) * N + i];
+    // at the first step of an inner solve (dt = 0) the correction residual
+    // arrays hold the previous solve's values and are not read
+    const double d1 = M.w1[e] + (prm.outer_only ? 0.0 : (double)D.w1[e] + (double)D.Ft[(long)(o + a) * N + i]);
     const double Dn = M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a];
     const double tn = M.t1[e] + D.t1[e] + prm.s1 * d1 / Dn;
     const float dt = (float)(tn - M.t1[e]);
@@ -956,8 +958,8 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
     const long r = (((long)I * o + J) * v + A) * v + B;
     const long mb = (long)I * v + B, na = (long)J * v + A;
     const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
-    const double om = M.w2[r] + (double)Dm.w2[r];
-    const double d2 = 2.0 * om - (M.w2[rx] + (double)Dm.w2[rx]);
+    const double om = M.w2[r] + (prm.outer_only ? 0.0 : (double)Dm.w2[r]);
+    const double d2 = 2.0 * om - (M.w2[rx] + (prm.outer_only ? 0.0 : (double)Dm.w2[rx]));
     amax = fabs(d2);
     const double Dn = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
                       M.F[(long)(o + B) * N + o + B];

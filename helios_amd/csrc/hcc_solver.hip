@@ -1407,6 +1407,19 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
     }
   };
+  // The first step of every inner solve starts from dt = 0, whose correction
+  // residual is zero: it takes the update from the outer FP64 residual
+  // without evaluating the correction contractions.
+  Params prm1 = prm;
+  prm1.outer_only = 1;
+  auto first_step = [&]() {
+    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm1, b.part1, b.part2, s);
+    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    if (prm1.diis) {
+      diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
+      mixed_inner_advance(b.dm, b.dst, b.nm, s);
+    }
+  };
   // blocks of 4 inner iterations replay one captured graph (see run_solve)
   GraphExec inner_block;
   const bool graphs = ctx->graphs && !ctx->prof;
@@ -1419,7 +1432,8 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     set_phase(b.dst, b.dact, b.nm, 1, s);
     if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
-    for (int inner = 0; inner < ctx->max_inner;) {
+    first_step();
+    for (int inner = 1; inner < ctx->max_inner;) {
       if (graphs && inner + 4 <= ctx->max_inner) {
         if (!inner_block.e) capture(inner_block, s, [&]() {
             for (int q = 0; q < 4; ++q) inner_iteration();
