@@ -72,7 +72,10 @@ int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_o
 int hcc_set_mixed_history(hcc_ctx* ctx, int keep);
 
 /* Per molecule of the last mixed-precision batch solve: FP64 residual
- * evaluations (outer points) and inner iterations. */
+ * evaluations (outer points) and inner iterations.  The first inner
+ * iteration after each outer point takes its step from the FP64 residual
+ * and evaluates no correction contractions, so a molecule evaluates them
+ * inner - (outer - 1) times. */
 int hcc_batch_statistics(hcc_ctx* ctx, int* outer_evaluations, int* inner_iterations);
 
 /* Upload one molecule and build its integrals: n orbitals (= sites), n_occ
