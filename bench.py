@@ -98,9 +98,9 @@ def main():
     ap.add_argument("--no-tuning", action="store_true", help="ignore the measured GEMM tile/split table")
     ap.add_argument("--skip-reference", action="store_true",
                     help="profiler runs only: no FP64 reference solve, solves count as converged by status alone")
-    ap.add_argument("--kappa", type=float, default=0.1)
-    ap.add_argument("--max-inner", type=int, default=40)
-    ap.add_argument("--max-outer", type=int, default=20)
+    ap.add_argument("--kappa", type=float, default=0.05)
+    ap.add_argument("--max-inner", type=int, default=5)
+    ap.add_argument("--max-outer", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark runs on one MI355X; "
@@ -181,8 +181,8 @@ def main():
                   file=sys.stderr, flush=True)
     head = rmx if rmx is not None else r64
 
-    default_workload = (args.nmol == 4096 and args.diis == 8 and args.eps == 2e-7 and args.kappa == 0.1
-                        and args.max_inner == 40 and args.max_outer == 20 and not args.no_tuning)
+    default_workload = (args.nmol == 4096 and args.diis == 8 and args.eps == 2e-7 and args.kappa == 0.05
+                        and args.max_inner == 5 and args.max_outer == 30 and not args.no_tuning)
     # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
     gemm_flops = args.steps * float((r64["fl_it"] * r64["iters"]).sum() + r64["fl_pre"].sum())
     achieved = gemm_flops / (r64["gemm_ms"] * 1e-3)

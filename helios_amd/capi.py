@@ -147,7 +147,7 @@ class Context:
         self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
                     "hcc_set_options")
 
-    def set_mixed(self, bf16=True, kappa=0.1, max_inner=40, max_outer=20):
+    def set_mixed(self, bf16=True, kappa=0.05, max_inner=5, max_outer=30):
         self._check(self.lib.hcc_set_mixed(self.h, int(bf16), kappa, max_inner, max_outer), "hcc_set_mixed")
 
     def batch_statistics(self):

@@ -748,9 +748,9 @@ struct hcc_ctx {
   // mixed-precision solver options
   int mixed_bf16 = 1;
   int keep_diis = 1;   // mixed solver: inner DIIS history kept across outer steps
-  double kappa = 0.1;
-  int max_inner = 40;
-  int max_outer = 20;
+  double kappa = 0.05;
+  int max_inner = 5;
+  int max_outer = 30;
   // live timing of the contraction GEMM launches (HIP events on `stream`)
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -1368,6 +1368,8 @@ static void refresh_shadows(hcc_ctx* c, bool constants) {
 // dOmega from correction-form GEMMs (BF16 operands, FP32 accumulation), until
 // max|d| < kappa * max|d(t0)|; then t0 += dt.  The FP64 residual at each outer
 // point decides convergence, so the answer is the FP64 fixed point.
+constexpr int kInnerBlock = 2;
+
 static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool build_integrals) {
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
@@ -1420,7 +1422,10 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
     }
   };
-  // blocks of 4 inner iterations replay one captured graph (see run_solve)
+  // blocks of kInnerBlock inner iterations replay one captured graph (see
+  // run_solve); the host checks between blocks whether any molecule is still
+  // in its inner solve, so short blocks cut the iterations that run after the
+  // last molecule has finished
   GraphExec inner_block;
   const bool graphs = ctx->graphs && !ctx->prof;
   for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
@@ -1434,14 +1439,14 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
     first_step();
     for (int inner = 1; inner < ctx->max_inner;) {
-      if (graphs && inner + 4 <= ctx->max_inner) {
+      if (graphs && inner + kInnerBlock <= ctx->max_inner) {
         if (!inner_block.e) capture(inner_block, s, [&]() {
-            for (int q = 0; q < 4; ++q) inner_iteration();
+            for (int q = 0; q < kInnerBlock; ++q) inner_iteration();
           });
         inner_block.launch(s);
-        inner += 4;
+        inner += kInnerBlock;
       } else {
-        for (int q = 0; q < 4 && inner < ctx->max_inner; ++q, ++inner) inner_iteration();
+        for (int q = 0; q < kInnerBlock && inner < ctx->max_inner; ++q, ++inner) inner_iteration();
       }
       if (alive() == 0) break;
     }

@@ -62,7 +62,7 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
  * max|d| < kappa * max|d(t0)| or max_inner iterations; at most max_outer
  * outer points.  The inner iterations take the Jacobi step dt = Omega/D with
  * DIIS (the t1/t2 steps of hcc_set_options apply to FP64 solves only).
- * Defaults: 1, 0.1, 40, 20. */
+ * Defaults: 1, 0.05, 5, 30. */
 int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer);
 
 /* Mixed solver: keep (1, default) or restart (0) the inner DIIS history at
