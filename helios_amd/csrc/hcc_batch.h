@@ -19,8 +19,8 @@ struct MolT {
   double eref;
   // inputs (row-major)
   double *C, *G, *F, *Fao;
-  // dressing
-  V *Xf, *Yf, *Mf, *P, *Q, *Ft, *tmp;
+  // dressing (PT[mu][p][q] = P[mu][q][p])
+  V *Xf, *Yf, *Mf, *P, *PT, *Q, *Ft, *tmp;
   // amplitudes and residuals
   V *t1, *t2, *u, *w1, *w2, *x, *Zc, *WC, *WD;
   // integrals and intermediates

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m(const Mol* __restrict__ m
   M.Mf[e] = val;
 }
 
-// P[mu][p*N+q] = Xf[mu][p] * Yf[mu][q]
+// P[mu][p*N+q] = Xf[mu][p] * Yf[mu][q] and its transpose PT[mu][p*N+q] = P[mu][q*N+p]
 __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                       Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ m
   const long mu = e / N2, pq = e % N2;
   const long p = pq / N, q = pq % N;
   M.P[e] = M.Xf[mu * N + p] * M.Yf[mu * N + q];
+  M.PT[e] = M.Xf[mu * N + q] * M.Yf[mu * N + p];
 }
 
 // u = 2 t2 - t2(a<->b)
@@ -758,7 +759,7 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__
   D.Mf[e] = val;
 }
 
-// dP = X0 (x) dY + dX (x) (Y0 + dY)
+// dP = X0 (x) dY + dX (x) (Y0 + dY), and dPT likewise
 __global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                         const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
@@ -769,6 +770,7 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__
   const long p = pq / N, q = pq % N;
   const long ip = mu * N + p, iq = mu * N + q;
   D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + (double)D.Yf[iq]);
+  D.PT[e] = M.Xf[iq] * D.Yf[ip] + D.Xf[iq] * (M.Yf[ip] + (double)D.Yf[ip]);
 }
 
 // t (x) 0 for the corrections of active molecules

@@ -7,9 +7,12 @@
 // GEMM over sites,
 //   g~(pq|rs) = sum_mu P[mu][pq] Q[mu][rs],   P[mu][p*N+q] = X[mu][p] Y[mu][q],
 //   Q = gamma P,   X = C(1 - t1^T) (bra), Y = C(1 + t1) (ket),
-// and the CCSD contractions (particle-particle ladder, hole-hole ladder, the
-// two ring intermediates and their contractions, the Fock-like intermediates)
-// are GEMMs over composite indices.  The reference's contravariant residuals
+// and, gamma being symmetric, also sum_mu Q[mu][pq] P[mu][rs]; the transpose
+// PT[mu][q*N+p] = P[mu][p*N+q] lets a block's GEMM take its output's unit
+// stride along the column index, where the stores coalesce.  The CCSD
+// contractions (particle-particle ladder, hole-hole ladder, the two ring
+// intermediates and their contractions, the Fock-like intermediates) are
+// GEMMs over composite indices.  The reference's contravariant residuals
 // (proj-ccsd-spat-hf-1/2.f90) are d1 = Omega1 and d2 = 2 Omega2 - Omega2(a<->b);
 // the update follows stepCC (mod_cc.f90:987-1017) and the DIIS follows
 // pushCCVectors/newCCVectors (mod_cc.f90:1390-1423, 1514-1555) with a ring
@@ -354,7 +357,7 @@ struct FieldDef {
 static const std::vector<FieldDef>& variable_fields() {
   static const std::vector<FieldDef> f = {
       HCC_VF(Xf, sz_nn), HCC_VF(Yf, sz_nn), HCC_VF(Mf, sz_nn), HCC_VF(Ft, sz_nn), HCC_VF(tmp, sz_nn),
-      HCC_VF(P, sz_n3), HCC_VF(Q, sz_n3), HCC_VF(t1, sz_ov), HCC_VF(w1, sz_ov), HCC_VF(t2, sz_t2),
+      HCC_VF(P, sz_n3), HCC_VF(PT, sz_n3), HCC_VF(Q, sz_n3), HCC_VF(t1, sz_ov), HCC_VF(w1, sz_ov), HCC_VF(t2, sz_t2),
       HCC_VF(u, sz_t2), HCC_VF(w2, sz_t2), HCC_VF(x, sz_t2), HCC_VF(Zc, sz_t2), HCC_VF(WC, sz_t2),
       HCC_VF(WD, sz_t2),
       HCC_VF(G4, [](int, int, int v) { return (long)v * v * v * v; }),
@@ -419,7 +422,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   const int m = b.m;
   auto C = arena(sz_nn), G = arena(sz_nn), F = arena(sz_nn), Fao = arena(sz_nn);
   auto Xf = arena(sz_nn), Yf = arena(sz_nn), Mf = arena(sz_nn), Ft = arena(sz_nn), tmp = arena(sz_nn);
-  auto P = arena(sz_n3), Q = arena(sz_n3);
+  auto P = arena(sz_n3), PT = arena(sz_n3), Q = arena(sz_n3);
   auto t1 = arena(sz_ov), w1 = arena(sz_ov);
   auto t2 = arena(sz_t2), u = arena(sz_t2), w2 = arena(sz_t2), x = arena(sz_t2), Zc = arena(sz_t2);
   auto WC = arena(sz_t2), WD = arena(sz_t2), Gklcd = arena(sz_t2), Lldkc = arena(sz_t2), Lt = arena(sz_t2);
@@ -451,7 +454,7 @@ static void allocate(Batch& b, bool keep_mp2) {
     M.nd = (int)(ov + ov * (ov + 1) / 2);
     M.eref = b.eref[k];
     M.C = C[k]; M.G = G[k]; M.F = F[k]; M.Fao = Fao[k];
-    M.Xf = Xf[k]; M.Yf = Yf[k]; M.Mf = Mf[k]; M.P = P[k]; M.Q = Q[k]; M.Ft = Ft[k]; M.tmp = tmp[k];
+    M.Xf = Xf[k]; M.Yf = Yf[k]; M.Mf = Mf[k]; M.P = P[k]; M.PT = PT[k]; M.Q = Q[k]; M.Ft = Ft[k]; M.tmp = tmp[k];
     M.t1 = t1[k]; M.t2 = t2[k]; M.u = u[k]; M.w1 = w1[k]; M.w2 = w2[k]; M.x = x[k]; M.Zc = Zc[k];
     M.WC = WC[k]; M.WD = WD[k];
     M.G4 = G4[k]; M.Wkl = Wkl[k]; M.Gklcd = Gklcd[k]; M.Lldkc = Lldkc[k]; M.Lt = Lt[k];
@@ -572,11 +575,12 @@ static void build_tables(Batch& b) {
     const int N = M.n;
     return prob(trn(M.Xf, N), rm(M.tmp, N), rm(M.Ft, N), N, N, N, 1.0, 0.0);
   });
-  // g3: gvvov[k][c][d][a] = g~(ad|kc)
+  // g3: gvvov[k][c][d][a] = g~(kc|ad) = sum_mu Q[mu][k][c] PT[mu][d][a], a row-major
+  // (k,c) x (d,a) product
   add([](const Mol& M) {
     const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
-    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1),
-                cmp(M.gvvov, v, 1, v, v, v * v * v, v * v), v * v, ov, N, 1.0, 0.0);
+    return prob(cmp(M.Q + o, v, N, 1, 1, N2, 0), cmp(M.PT + o * N + o, 1, N2, 0, v, N, 1), rm(M.gvvov, v * v),
+                ov, v * v, N, 1.0, 0.0);
   });
   // g4: w1[i][a] = sum_{k,cd} u[k][i][cd] gvvov[(k,c,d)][a]
   add([](const Mol& M) {
@@ -602,17 +606,19 @@ static void build_tables(Batch& b) {
     return prob(cmp(M.u, v, ov * v, v, v, v * v, 1), cmp(M.Ft + o, v, N, 1, 1, 0, 0),
                 cmp(M.w1, 1, 1, 0, 1, 0, 0), ov, 1, ov, 1.0, 1.0);
   });
-  // g8: w2[i][j][a][b] = g~(ai|bj)
+  // g8: w2[i][j][a][b] = g~(ai|bj) = sum_mu Q[mu][a][i] PT[mu][j][b]: rows (a,i),
+  // columns (j,b) so that the output's unit stride (b) runs along consecutive columns
   add([](const Mol& M) {
     const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
-    return prob(cmp(M.P + o * N, o, N, 1, 1, N2, 0), cmp(M.Q + o * N, 1, N2, 0, o, N, 1),
-                cmp(M.w2, o, v, ov * v, o, 1, v * v), ov, ov, N, 1.0, 0.0);
+    return prob(cmp(M.Q + o * N, o, N, 1, 1, N2, 0), cmp(M.PT + o, 1, N2, 0, v, N, 1),
+                cmp(M.w2, o, v, ov * v, v, v * v, 1), ov, ov, N, 1.0, 0.0);
   });
-  // g9: G4[c][d][a][b] = g~(ac|bd)
+  // g9: G4[c][d][a][b] = g~(ac|bd) = sum_mu Q[mu][a][c] PT[mu][d][b]: rows (a,c),
+  // columns (d,b)
   add([](const Mol& M) {
     const int N = M.n, o = M.o, v = M.v, N2 = N * N;
-    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q + o * N + o, 1, N2, 0, v, N, 1),
-                cmp(M.G4, v, v, v * v * v, v, 1, v * v), v * v, v * v, N, 1.0, 0.0);
+    return prob(cmp(M.Q + o * N + o, v, N, 1, 1, N2, 0), cmp(M.PT + o * N + o, 1, N2, 0, v, N, 1),
+                cmp(M.G4, v, v, v * v * v, v, v * v, 1), v * v, v * v, N, 1.0, 0.0);
   });
   // g10: particle-particle ladder w2 += t2 G4
   add([](const Mol& M) {
@@ -647,10 +653,11 @@ static void build_tables(Batch& b) {
     return prob(cmp(M.t2, o, v, v * v, v, ov * v, 1), cmp(M.Gklcd, v, ov * v, 1, v, v * v, v), rm(M.WC, ov),
                 ov, ov, ov, -0.5, 1.0);
   });
-  // g16: Zc[(a,i),(b,j)] = WC t2[k][j][b][c], stored at [i][j][a][b]
+  // g16: Zc[(a,i),(j,b)] = WC t2[k][j][b][c], stored at [i][j][a][b] (columns ordered
+  // (j, b) so that the output's unit stride runs along consecutive columns)
   add([](const Mol& M) {
     const int o = M.o, v = M.v, ov = o * v;
-    return prob(rm(M.WC, ov), cmp(M.t2, v, ov * v, 1, o, v, v * v), cmp(M.Zc, o, v, ov * v, o, 1, v * v), ov,
+    return prob(rm(M.WC, ov), cmp(M.t2, v, ov * v, 1, v, v * v, v), cmp(M.Zc, o, v, ov * v, v, v * v, 1), ov,
                 ov, ov, 1.0, 0.0);
   });
   // g17: WD = 2 g~(ai|kc)
@@ -659,21 +666,22 @@ static void build_tables(Batch& b) {
     return prob(cmp(M.P + o * N, o, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1), rm(M.WD, ov), ov, ov,
                 N, 2.0, 0.0);
   });
-  // g18: WD -= g~(ac|ki)
+  // g18: WD -= g~(ac|ki), as the (k,i) x (a,c) problem so that the output's
+  // unit stride (c) runs along consecutive columns
   add([](const Mol& M) {
     const int N = M.n, o = M.o, v = M.v, N2 = N * N, ov = o * v;
-    return prob(cmp(M.P + o * N + o, v, N, 1, 1, N2, 0), cmp(M.Q, 1, N2, 0, o, N, 1),
-                cmp(M.WD, v, o * ov, 1, o, v, ov), v * v, o * o, N, -1.0, 1.0);
+    return prob(cmp(M.Q, o, N, 1, 1, N2, 0), cmp(M.P + o * N + o, 1, N2, 0, v, N, 1),
+                cmp(M.WD, o, v, ov, v, o * ov, 1), o * o, v * v, N, -1.0, 1.0);
   });
   // g19: WD += 1/2 u[i][l][a][d] L[(l,d),(k,c)]
   add([](const Mol& M) {
     const int o = M.o, v = M.v, ov = o * v;
     return prob(cmp(M.u, o, v, ov * v, v, v * v, 1), rm(M.Lldkc, ov), rm(M.WD, ov), ov, ov, ov, 0.5, 1.0);
   });
-  // g20: x[i][j][a][b] = 1/2 WD u[j][k][b][c]
+  // g20: x[i][j][a][b] = 1/2 WD u[j][k][b][c] (columns (j, b), as in g16)
   add([](const Mol& M) {
     const int o = M.o, v = M.v, ov = o * v;
-    return prob(rm(M.WD, ov), cmp(M.u, v, v * v, 1, o, v, ov * v), cmp(M.x, o, v, ov * v, o, 1, v * v), ov,
+    return prob(rm(M.WD, ov), cmp(M.u, v, v * v, 1, v, ov * v, v), cmp(M.x, o, v, ov * v, v, v * v, 1), ov,
                 ov, ov, 0.5, 0.0);
   });
   // g21: Xvv -= u[(kl)][b][d] Gklcd[(kl)][c][d]
