@@ -435,7 +435,8 @@ struct Stage {
   }
 };
 constexpr int kDualBK = 32;   // 64-deep slabs measured slower (K padding of the short contractions, fewer waves)
-// TS: output tile edge, 64 (4 waves of 32x32) or 128 (8 waves of 32x64)
+// TS: output tile edge, 32 (2 waves of 32x16), 64 (4 waves of 32x32) or 128
+// (8 waves of 32x64)
 template <bool AKF, bool BNF, bool HA, bool HB, int TS>
 __global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(4, 8))) void gemm_dual_bf16(
     const GemmProb* __restrict__ probs, const int* __restrict__ tile_prob, int total_tiles,
@@ -623,6 +624,7 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tile
       default: HCC_DUAL16(true, true, TS) break;   \
     }
     if (tile == 64) { HCC_DUAL16_T(64) }
+    else if (tile == 32) { HCC_DUAL16_T(32) }
     else if (tile == 256) { HCC_DUAL16_T(128) }
     else return hipErrorInvalidValue;
 #undef HCC_DUAL16_T

@@ -82,8 +82,8 @@ constexpr int kBM = 64, kBN = 64, kBK = 16, kThreads = 256;
 hipError_t launch_gemm_f64(const GemmProb* probs, const int* tile_prob, int total_tiles,
                            const int* active, hipStream_t s, int tile = 64, int flags = 3);
 // correction-form GEMM (see GemmProb::dA/dB); bf16 = 1 rounds the operands to
-// BF16 for V_MFMA_F32_16X16X32_BF16 (FP32 accumulation), 0 keeps FP64;
-// tile kind 64 (64x64 output tiles) only
+// BF16 for V_MFMA_F32_16X16X32_BF16 (FP32 accumulation; tile kinds 32, 64 and
+// 256 = 128x128 output tiles), 0 keeps FP64 (tile kind 64 only)
 hipError_t launch_gemm_dual(const GemmProb* probs, const int* tile_prob, int total_tiles, const int* active,
                             hipStream_t s, int flags, int bf16, int tile);
 // C = alpha * sum_k ws_k + beta * C for the split problems of a table;

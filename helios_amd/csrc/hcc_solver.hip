@@ -63,18 +63,24 @@ struct GemmTable {
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
 // 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide); of the
-// BF16 correction GEMM: 64 (64x64, 4 waves) and 256 (128x128, 8 waves).
+// BF16 correction GEMM: 64 (64x64, 4 waves), 32 (32x32, 2 waves) and 256
+// (128x128, 8 waves).
 static int tile_m(int kind) { return kind == 128 || kind == 256 ? 128 : kind; }
 static int tile_n(int kind) { return kind == 128 ? 32 : (kind == 256 ? 128 : kind); }
 
 // Tile kind of a correction-form problem.  128x128 tiles halve the operand
 // staging per product and win on long contractions wherever their padding
 // stays close to that of 64x64 tiles; short contractions (K < 128) are bound
-// by their output stores and stay on 64x64 tiles, in one launch.
+// by their output stores and stay on 64x64 tiles, in one launch.  Every lane
+// stages the same number of elements whatever the tile, so 32x32 tiles (half
+// the staging per product of 64x64 ones) take the narrow outputs whose
+// 64x64 tiles would be mostly padding.
 static int choose_tile_dual(const GemmProb& p) {
   const double t64 = (double)((p.M + 63) / 64) * ((p.N + 63) / 64) * 64 * 64;
+  const double t32 = (double)((p.M + 31) / 32) * ((p.N + 31) / 32) * 32 * 32;
   const double t128 = (double)((p.M + 127) / 128) * ((p.N + 127) / 128) * 128 * 128;
-  return p.K >= 128 && p.M >= 128 && p.N >= 128 && t128 <= 1.3 * t64 ? 256 : 64;
+  if (p.K >= 128 && p.M >= 128 && p.N >= 128 && t128 <= 1.3 * t64) return 256;
+  return 2.0 * t32 < t64 ? 32 : 64;
 }
 
 // Split K when a problem has few output tiles and a long K: aim for >= 16
@@ -1738,8 +1744,8 @@ int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, 
                                  const double* B0, const float* dB, float* Cd, double alpha, double beta, int tile,
                                  int reps, double* ms) {
   if (m < 1 || n < 1 || k < 1 || !A0 || !B0 || !Cd || (!dA && !dB) || reps < 0) return HCC_E_ARG;
-  if (tile != 64 && tile != 128) return HCC_E_ARG;
-  const int kind = tile == 128 ? 256 : 64;
+  if (tile != 32 && tile != 64 && tile != 128) return HCC_E_ARG;
+  const int kind = tile == 128 ? 256 : tile;
   if ((long)m * k >= (1L << 27) || (long)k * n >= (1L << 27) || (long)m * n >= (1L << 27)) return HCC_E_ARG;
   double *a0 = nullptr, *b0 = nullptr;
   __bf16 *ha = nullptr, *hb = nullptr;

@@ -242,7 +242,7 @@ int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, dou
  * when a_kfast, else its transpose stored k x m; B0 is k x n row-major when
  * b_nfast, else its transpose stored n x k; dA / dB (float) share those
  * layouts and one of them may be null (constant operand).  Cd is m x n
- * row-major.  tile: output tile edge, 64 or 128.  With reps > 0 the launch
+ * row-major.  tile: output tile edge, 32, 64 or 128.  With reps > 0 the launch
  * is repeated reps more times and *ms receives the mean kernel time.  Every
  * matrix must stay below 2^27 elements.  No reference counterpart; used by
  * the tests. */

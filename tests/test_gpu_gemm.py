@@ -36,7 +36,7 @@ def _bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64).reshape(a.shape)
 
 
-@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("tile", [32, 64, 128])
 @pytest.mark.parametrize("a_kfast", [True, False])
 @pytest.mark.parametrize("b_nfast", [True, False])
 @pytest.mark.parametrize("m,n,k,which", [(64, 64, 32, "AB"), (70, 50, 100, "AB"), (5, 300, 7, "AB"),
