@@ -33,6 +33,24 @@ __device__ __forceinline__ int block_mol(const int* __restrict__ prefix, int nm,
   const long size_ = (SIZE);                                         \
   (void)size_;
 
+// Index decompositions in 32-bit unsigned arithmetic (64-bit division is a
+// long software sequence on the GPU): every per-molecule array stays below
+// 2^28 elements, so element indices within a molecule fit.
+// (i, j, a, b) of element e of an o x o x v x v array (row-major)
+__device__ __forceinline__ void ijab_of(long e, int o, int v, int& i, int& j, int& a, int& b) {
+  const unsigned q1 = (unsigned)e / (unsigned)v, q2 = q1 / (unsigned)v, q3 = q2 / (unsigned)o;
+  b = (int)((unsigned)e - q1 * (unsigned)v);
+  a = (int)(q1 - q2 * (unsigned)v);
+  j = (int)(q2 - q3 * (unsigned)o);
+  i = (int)q3;
+}
+// quotient and remainder of 0 <= x < 2^31 by d >= 1
+__device__ __forceinline__ void divmod(long x, int d, int& q, int& r) {
+  const unsigned u = (unsigned)x / (unsigned)d;
+  q = (int)u;
+  r = (int)((unsigned)x - u * (unsigned)d);
+}
+
 // ---------------------------------------------------------------------------
 // T1 dressing
 // ---------------------------------------------------------------------------
@@ -99,8 +117,9 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ m
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
   if (e >= size_ || !st[mol].active) return;
   const long N = M.n, N2 = N * N;
-  const long mu = e / N2, pq = e % N2;
-  const long p = pq / N, q = pq % N;
+  int mu, pq, p, q;
+  divmod(e, (int)N2, mu, pq);
+  divmod(pq, (int)N, p, q);
   M.P[e] = M.Xf[mu * N + p] * M.Yf[mu * N + q];
   M.PT[e] = M.Xf[mu * N + q] * M.Yf[mu * N + p];
 }
@@ -112,9 +131,10 @@ __global__ __launch_bounds__(kEwThreads) void k_u(const MT* __restrict__ mols, c
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
   const int v = M.v;
-  const long ab = e % ((long)v * v);
+  int q, ab, a, b;
+  divmod(e, v * v, q, ab);
+  divmod(ab, v, a, b);
   const long base = e - ab;
-  const int a = ab / v, b = ab % v;
   M.u[e] = 2.0 * M.t2[e] - (double)M.t2[base + (long)b * v + a];
 }
 
@@ -143,10 +163,8 @@ __global__ __launch_bounds__(kEwThreads) void k_assemble(const MT* __restrict__ 
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_ || !st[mol].active) return;
   const int o = M.o, v = M.v;
-  const int b = e % v;
-  const int a = (e / v) % v;
-  const int j = (e / ((long)v * v)) % o;
-  const int i = e / ((long)o * v * v);
+  int i, j, a, b;
+  ijab_of(e, o, v, i, j, a, b);
   const long ij = ((long)i * o + j) * v * v, ji = ((long)j * o + i) * v * v;
   const long ab = (long)a * v + b, ba = (long)b * v + a;
   const auto* Z = M.Zc;
@@ -163,6 +181,7 @@ __device__ __forceinline__ long packed_index(long mm, long nn, long ne) {
   // upper triangle, row-major: pairs (mm, nn) with mm <= nn
   return mm * ne - mm * (mm - 1) / 2 + (nn - mm);
 }
+
 
 // maximum that propagates NaN (fmax drops it), so that a NaN residual can
 // never read as a converged one
@@ -228,10 +247,8 @@ __global__ __launch_bounds__(kEwThreads) void k_update2(const Mol* __restrict__ 
   if (e < size_) {
     const int N = M.n, o = M.o, v = M.v;
     const long ne = (long)o * v;
-    const int b = e % v;
-    const int a = (e / v) % v;
-    const int j = (e / ((long)v * v)) % o;
-    const int i = e / ((long)o * v * v);
+    int i, j, a, b;
+    ijab_of(e, o, v, i, j, a, b);
     const long mm = (long)i * v + a, nn = (long)j * v + b;
     int I = i, J = j, A = a, B = b;
     if (mm > nn) { I = j; J = i; A = b; B = a; }
@@ -278,10 +295,8 @@ __global__ __launch_bounds__(kEwThreads) void k_contravariant(const Mol* __restr
     d1[d1_off + f] = M.w1[f] + M.Ft[(long)(o + a) * N + i];
     return;
   }
-  const int b = e % v;
-  const int a = (e / v) % v;
-  const int j = (e / ((long)v * v)) % o;
-  const int i = e / ((long)o * v * v);
+  int i, j, a, b;
+  ijab_of(e, o, v, i, j, a, b);
   const long mm = (long)i * v + a, nn = (long)j * v + b;
   const long r = mm <= nn ? e : (((long)j * o + i) * v + b) * v + a;
   const long mb = (long)i * v + b, na = (long)j * v + a;
@@ -461,12 +476,14 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict
     return;
   }
   const long f = e - ne;
-  const long mm = f / ne, nn = f % ne;
+  int mm, nn, i, a, j, b;
+  divmod(f, (int)ne, mm, nn);
   if (mm > nn) return;
   const long p = ne + packed_index(mm, nn, ne);
   double s = 0.0;
   for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + p];
-  const int i = mm / v, a = mm % v, j = nn / v, b = nn % v;
+  divmod(mm, v, i, a);
+  divmod(nn, v, j, b);
   M.t2[(((long)i * o + j) * v + a) * v + b] = s;
   M.t2[(((long)j * o + i) * v + b) * v + a] = s;
 }
@@ -487,10 +504,8 @@ __global__ __launch_bounds__(kEwThreads) void k_energy_part(const Mol* __restric
       const int i = f / v, a = f % v;
       s1 = M.F[(long)i * N + o + a] * M.t1[f];
     } else {
-      const int b = e % v;
-      const int a = (e / v) % v;
-      const int j = (e / ((long)v * v)) % o;
-      const int i = e / ((long)o * v * v);
+      int i, j, a, b;
+      ijab_of(e, o, v, i, j, a, b);
       s2 = M.Lt[e] * (M.t2[e] + M.t1[i * v + a] * M.t1[j * v + b]);
     }
   }
@@ -532,10 +547,8 @@ __global__ __launch_bounds__(kEwThreads) void k_make_L(const Mol* __restrict__ m
   const long ov = (long)o * v;
   const double* G = M.Gklcd;
   {
-    const int b = e % v;
-    const int a = (e / v) % v;
-    const int j = (e / ((long)v * v)) % o;
-    const int i = e / ((long)o * v * v);
+    int i, j, a, b;
+    ijab_of(e, o, v, i, j, a, b);
     const long base = ((long)i * o + j) * v * v;
     M.Lt[e] = 2.0 * G[base + (long)a * v + b] - G[base + (long)b * v + a];
   }
@@ -562,10 +575,8 @@ __global__ __launch_bounds__(kEwThreads) void k_guess(const Mol* __restrict__ mo
     M.t1[f] = M.F[(long)i * N + o + a] / (M.F[(long)i * N + i] - M.F[(long)(o + a) * N + o + a]);
     return;
   }
-  const int b = e % v;
-  const int a = (e / v) % v;
-  const int j = (e / ((long)v * v)) % o;
-  const int i = e / ((long)o * v * v);
+  int i, j, a, b;
+  ijab_of(e, o, v, i, j, a, b);
   const long mm = (long)i * v + a, nn = (long)j * v + b;
   if (mm > nn) return;   // the representative of the pair writes both elements
   const double D = M.F[(long)i * N + i] + M.F[(long)j * N + j] - M.F[(long)(o + a) * N + o + a] -
@@ -766,8 +777,9 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__
   if (e >= size_ || !st[mol].active) return;
   const MolD& D = dmols[mol];
   const long N = M.n, N2 = N * N;
-  const long mu = e / N2, pq = e % N2;
-  const long p = pq / N, q = pq % N;
+  int mu, pq, p, q;
+  divmod(e, (int)N2, mu, pq);
+  divmod(pq, (int)N, p, q);
   const long ip = mu * N + p, iq = mu * N + q;
   D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + (double)D.Yf[iq]);
   D.PT[e] = M.Xf[iq] * D.Yf[ip] + D.Xf[iq] * (M.Yf[ip] + (double)D.Yf[ip]);
@@ -816,10 +828,12 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restric
     d = M.t1[e];
   } else {
     const long f = e - ne;
-    const long mm = f / ne, nn = f % ne;
+    int mm, nn, i, a, j, b;
+    divmod(f, (int)ne, mm, nn);
     if (mm > nn) return;
     p = ne + packed_index(mm, nn, ne);
-    const int i = mm / v, a = mm % v, j = nn / v, b = nn % v;
+    divmod(mm, v, i, a);
+    divmod(nn, v, j, b);
     d = M.t2[(((long)i * o + j) * v + a) * v + b];
   }
   for (int k = 0; k < S.diis_count; ++k) M.st[(long)k * M.nd + p] -= d;
@@ -950,10 +964,8 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
     const MolD& Dm = dmols[mol];
     const int N = M.n, o = M.o, v = M.v;
     const long ne = (long)o * v;
-    const int b = e % v;
-    const int a = (e / v) % v;
-    const int j = (e / ((long)v * v)) % o;
-    const int i = e / ((long)o * v * v);
+    int i, j, a, b;
+    ijab_of(e, o, v, i, j, a, b);
     const long mm = (long)i * v + a, nn = (long)j * v + b;
     int I = i, J = j, A = a, B = b;
     if (mm > nn) { I = j; J = i; A = b; B = a; }
