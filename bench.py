@@ -34,8 +34,8 @@ PEAK_BF16 = 2.5e15
 # 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --steps 1 --warmup 0 --no-cpu
 # --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  A counter pass
 # cannot run inside this process, so other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 2.848e9        # gemm_f64_kernel, FP64 solve
-DUAL_GEMM_TRAFFIC_BATCH4096 = 2.173e9   # gemm_dual_bf16, mixed solve
+GEMM_TRAFFIC_BATCH4096 = 2.574e9        # gemm_f64_kernel, FP64 solve
+DUAL_GEMM_TRAFFIC_BATCH4096 = 1.282e9   # gemm_dual_bf16, mixed solve
 
 
 def _prep(k):
