@@ -31,9 +31,10 @@ PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
 # HBM bytes per dispatch of the dominant kernels on the default 4096-molecule batch
 # with the default solver settings: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
-# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --steps 1 --warmup 0 --no-cpu
-# --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  A counter pass
-# cannot run inside this process, so other workloads report null.
+# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --diis 8 --steps 1 --warmup 0
+# --no-cpu --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  The GEMM
+# kernels never touch the DIIS arrays, so the figures hold at any DIIS depth.  A counter
+# pass cannot run inside this process, so other workloads report null.
 GEMM_TRAFFIC_BATCH4096 = 2.574e9        # gemm_f64_kernel, FP64 solve
 DUAL_GEMM_TRAFFIC_BATCH4096 = 1.282e9   # gemm_dual_bf16, mixed solve
 
@@ -89,7 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--eps", type=float, default=2e-7, help="residual threshold of the timed solves")
-    ap.add_argument("--diis", type=int, default=8)
+    ap.add_argument("--diis", type=int, default=20,
+                    help="DIIS vectors of both solvers (the reference's default diis-steps)")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -181,7 +183,7 @@ def main():
                   file=sys.stderr, flush=True)
     head = rmx if rmx is not None else r64
 
-    default_workload = (args.nmol == 4096 and args.diis == 8 and args.eps == 2e-7 and args.kappa == 0.05
+    default_workload = (args.nmol == 4096 and args.eps == 2e-7 and args.kappa == 0.05
                         and args.max_inner == 5 and args.max_outer == 30 and not args.no_tuning)
     # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
     gemm_flops = args.steps * float((r64["fl_it"] * r64["iters"]).sum() + r64["fl_pre"].sum())
