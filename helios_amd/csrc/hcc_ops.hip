@@ -209,6 +209,56 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// block_sum of 4 values at once (same order per value; sh holds 4 x waves)
+__device__ __forceinline__ void block_sum4(double (&v)[4], double* sh) {
+  constexpr int W = kEwThreads / 64;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sh[q * W + w] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double r = sh[q * W];
+    for (int k = 1; k < W; ++k) r += sh[q * W + k];
+    v[q] = r;
+  }
+  __syncthreads();
+}
+
+// DIIS history loops issue the loads of kDiisBatch slots before using any of
+// them (a loop of load-then-accumulate waits for every load in turn), through
+// global-address-space pointers: generic (flat) loads also count against the
+// scalar-load counter, so each wait for a scalar coefficient would drain them.
+constexpr int kDiisBatch = 8;
+template <class T>
+using gptr = const T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ gptr<T> global_view(const T* p) {
+  return (gptr<T>)p;
+}
+// sum_k coef[k] x[k nd + p] over the slots in slot order; the loads of a
+// partial batch repeat the last slot (in bounds, unused)
+template <class T>
+__device__ __forceinline__ double diis_combine(const State& S, const T* x, long nd, long p) {
+  const gptr<T> gx = global_view(x);
+  const int cnt = S.diis_count;
+  double s = 0.0;
+  for (int k0 = 0; k0 < cnt; k0 += kDiisBatch) {
+    T a[kDiisBatch];
+#pragma unroll
+    for (int q = 0; q < kDiisBatch; ++q) a[q] = gx[(long)min(k0 + q, cnt - 1) * nd + p];
+#pragma unroll
+    for (int q = 0; q < kDiisBatch; ++q)
+      if (k0 + q < cnt) s += S.coef[k0 + q] * a[q];
+  }
+  return s;
+}
+
 // Singles: d1 = w1 + Ft_ai; t1 += s1 d1 / (F_ii - F_aa).  size o*v
 __global__ __launch_bounds__(kEwThreads) void k_update1(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                        Ragged rg, int nm, Params prm, double* __restrict__ part) {
@@ -367,7 +417,7 @@ __global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st,
 template <class MT>
 __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const MT* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm, double* __restrict__ part) {
-  __shared__ double sh[kEwThreads / 64];
+  __shared__ double sh[4 * (kEwThreads / 64)];
   const int mol = rg.bmol[blockIdx.x];
   const int lb = blockIdx.x - rg.prefix[mol];
   const auto& M = mols[mol];
@@ -378,16 +428,44 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_dot(const MT* __restrict__ 
     return;
   }
   const long base = (long)lb * kEwThreads * kDotItems;
-  const auto* snew = M.sd + (long)S.diis_next * M.nd;
-  for (int k = 0; k < S.diis_count; ++k) {
-    const auto* sk = M.sd + (long)k * M.nd;
-    double acc = 0.0;
-    for (int it = 0; it < kDotItems; ++it) {
-      const long e = base + (long)it * kEwThreads + threadIdx.x;
-      if (e < M.nd) acc += (double)sk[e] * (double)snew[e];
+  const auto gsd = global_view(M.sd);
+  const auto snew = gsd + (long)S.diis_next * M.nd;
+  double xn[kDotItems];
+#pragma unroll
+  for (int it = 0; it < kDotItems; ++it) {
+    const long e = base + (long)it * kEwThreads + threadIdx.x;
+    xn[it] = e < M.nd ? (double)snew[e] : 0.0;
+  }
+  // four slots per pass: their loads are in flight together, one reduction;
+  // a partial pass loads the last slot again (in bounds, unused)
+  const int cnt = S.diis_count;
+  const bool full = base + (long)kEwThreads * kDotItems <= M.nd;   // no element past the vector
+  for (int k0 = 0; k0 < cnt; k0 += 4) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (full) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const auto sk = gsd + (long)min(k0 + q, cnt - 1) * M.nd + base + threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < kDotItems; ++it) acc[q] += (double)sk[it * kEwThreads] * xn[it];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const auto sk = gsd + (long)min(k0 + q, cnt - 1) * M.nd;
+#pragma unroll
+        for (int it = 0; it < kDotItems; ++it) {
+          const long e = base + (long)it * kEwThreads + threadIdx.x;
+          if (e < M.nd) acc[q] += (double)sk[e] * xn[it];
+        }
+      }
     }
-    const double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[(long)blockIdx.x * m + k] = r;
+    block_sum4(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k0 + q < cnt) part[(long)blockIdx.x * m + k0 + q] = acc[q];
+    }
   }
 }
 
@@ -470,9 +548,7 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict
   const int o = M.o, v = M.v;
   const long ne = (long)o * v;
   if (e < ne) {
-    double s = 0.0;
-    for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + e];
-    M.t1[e] = s;
+    M.t1[e] = diis_combine(S, M.st, M.nd, e);
     return;
   }
   const long f = e - ne;
@@ -480,8 +556,7 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict
   divmod(f, (int)ne, mm, nn);
   if (mm > nn) return;
   const long p = ne + packed_index(mm, nn, ne);
-  double s = 0.0;
-  for (int k = 0; k < S.diis_count; ++k) s += S.coef[k] * M.st[(long)k * M.nd + p];
+  const double s = diis_combine(S, M.st, M.nd, p);
   divmod(mm, v, i, a);
   divmod(nn, v, j, b);
   M.t2[(((long)i * o + j) * v + a) * v + b] = s;
@@ -836,7 +911,16 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restric
     divmod(nn, v, j, b);
     d = M.t2[(((long)i * o + j) * v + a) * v + b];
   }
-  for (int k = 0; k < S.diis_count; ++k) M.st[(long)k * M.nd + p] -= d;
+  const int cnt = S.diis_count;
+  const gptr<float> gst = global_view(M.st);
+  for (int k0 = 0; k0 < cnt; k0 += kDiisBatch) {
+    float x[kDiisBatch];
+#pragma unroll
+    for (int q = 0; q < kDiisBatch; ++q) x[q] = gst[(long)min(k0 + q, cnt - 1) * M.nd + p];
+#pragma unroll
+    for (int q = 0; q < kDiisBatch; ++q)
+      if (k0 + q < cnt) M.st[(long)(k0 + q) * M.nd + p] = x[q] - d;
+  }
 }
 
 // ---------------------------------------------------------------------------
