@@ -44,6 +44,21 @@ __device__ __forceinline__ void ijab_of(long e, int o, int v, int& i, int& j, in
   j = (int)(q2 - q3 * (unsigned)o);
   i = (int)q3;
 }
+// Loads through global-address-space pointers: pointers read from a molecule
+// descriptor are generic, and generic (flat) loads also count against the
+// scalar-load counter, so each wait for a scalar load would drain them.
+template <class T>
+using gptr = const T __attribute__((address_space(1)))*;
+template <class T>
+using gwptr = T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ gptr<T> global_view(const T* p) {
+  return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ gwptr<T> global_out(T* p) {
+  return (gwptr<T>)p;
+}
 // quotient and remainder of 0 <= x < 2^31 by d >= 1
 __device__ __forceinline__ void divmod(long x, int d, int& q, int& r) {
   const unsigned u = (unsigned)x / (unsigned)d;
@@ -111,17 +126,39 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m(const Mol* __restrict__ m
   M.Mf[e] = val;
 }
 
-// P[mu][p*N+q] = Xf[mu][p] * Yf[mu][q] and its transpose PT[mu][p*N+q] = P[mu][q*N+p]
+// P[mu][p*N+q] = Xf[mu][p] * Yf[mu][q] and its transpose PT[mu][p*N+q] = P[mu][q*N+p];
+// kPItems elements per thread (the ragged table of these kernels counts
+// kEwThreads * kPItems elements per workgroup)
 __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                       Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
-  if (e >= size_ || !st[mol].active) return;
-  const long N = M.n, N2 = N * N;
-  int mu, pq, p, q;
-  divmod(e, (int)N2, mu, pq);
-  divmod(pq, (int)N, p, q);
-  M.P[e] = M.Xf[mu * N + p] * M.Yf[mu * N + q];
-  M.PT[e] = M.Xf[mu * N + q] * M.Yf[mu * N + p];
+  if (!st[mol].active) return;
+  const int N = M.n, N2 = N * N;
+  const gptr<double> X = global_view(M.Xf), Y = global_view(M.Yf);
+  const gwptr<double> P = global_out(M.P), PT = global_out(M.PT);
+  const long f0 = (long)lb * kPItems * kEwThreads + threadIdx.x;
+  // all loads of the thread's elements first, then the stores (elements past
+  // the molecule's block read element 0 and store nothing)
+  double xp[kPItems], yq[kPItems], xq[kPItems], yp[kPItems];
+#pragma unroll
+  for (int it = 0; it < kPItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    int mu, pq, p, q;
+    divmod(f < size_ ? f : 0, N2, mu, pq);
+    divmod(pq, N, p, q);
+    xp[it] = X[mu * N + p];
+    yq[it] = Y[mu * N + q];
+    xq[it] = X[mu * N + q];
+    yp[it] = Y[mu * N + p];
+  }
+#pragma unroll
+  for (int it = 0; it < kPItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < size_) {
+      P[f] = xp[it] * yq[it];
+      PT[f] = xq[it] * yp[it];
+    }
+  }
 }
 
 // u = 2 t2 - t2(a<->b)
@@ -231,16 +268,8 @@ __device__ __forceinline__ void block_sum4(double (&v)[4], double* sh) {
 }
 
 // DIIS history loops issue the loads of kDiisBatch slots before using any of
-// them (a loop of load-then-accumulate waits for every load in turn), through
-// global-address-space pointers: generic (flat) loads also count against the
-// scalar-load counter, so each wait for a scalar coefficient would drain them.
+// them (a loop of load-then-accumulate waits for every load in turn).
 constexpr int kDiisBatch = 8;
-template <class T>
-using gptr = const T __attribute__((address_space(1)))*;
-template <class T>
-__device__ __forceinline__ gptr<T> global_view(const T* p) {
-  return (gptr<T>)p;
-}
 // sum_k coef[k] x[k nd + p] over the slots in slot order; the loads of a
 // partial batch repeat the last slot (in bounds, unused)
 template <class T>
@@ -849,15 +878,40 @@ __global__ __launch_bounds__(kEwThreads) void k_fock_m_d(const Mol* __restrict__
 __global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                         const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].n * mols[mol].n * mols[mol].n);
-  if (e >= size_ || !st[mol].active) return;
+  if (!st[mol].active) return;
   const MolD& D = dmols[mol];
-  const long N = M.n, N2 = N * N;
-  int mu, pq, p, q;
-  divmod(e, (int)N2, mu, pq);
-  divmod(pq, (int)N, p, q);
-  const long ip = mu * N + p, iq = mu * N + q;
-  D.P[e] = M.Xf[ip] * D.Yf[iq] + D.Xf[ip] * (M.Yf[iq] + (double)D.Yf[iq]);
-  D.PT[e] = M.Xf[iq] * D.Yf[ip] + D.Xf[iq] * (M.Yf[ip] + (double)D.Yf[ip]);
+  const int N = M.n, N2 = N * N;
+  const gptr<double> X0 = global_view(M.Xf), Y0 = global_view(M.Yf);
+  const gptr<float> dX = global_view(D.Xf), dY = global_view(D.Yf);
+  const gwptr<float> P = global_out(D.P), PT = global_out(D.PT);
+  const long f0 = (long)lb * kPItems * kEwThreads + threadIdx.x;
+  // all loads first, then the stores (as in k_pouter)
+  double x0p[kPItems], y0q[kPItems], x0q[kPItems], y0p[kPItems];
+  float dxp[kPItems], dyq[kPItems], dxq[kPItems], dyp[kPItems];
+#pragma unroll
+  for (int it = 0; it < kPItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    int mu, pq, p, q;
+    divmod(f < size_ ? f : 0, N2, mu, pq);
+    divmod(pq, N, p, q);
+    const int ip = mu * N + p, iq = mu * N + q;
+    x0p[it] = X0[ip];
+    y0q[it] = Y0[iq];
+    x0q[it] = X0[iq];
+    y0p[it] = Y0[ip];
+    dxp[it] = dX[ip];
+    dyq[it] = dY[iq];
+    dxq[it] = dX[iq];
+    dyp[it] = dY[ip];
+  }
+#pragma unroll
+  for (int it = 0; it < kPItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < size_) {
+      P[f] = x0p[it] * dyq[it] + dxp[it] * (y0q[it] + (double)dyq[it]);
+      PT[f] = x0q[it] * dyp[it] + dxq[it] * (y0p[it] + (double)dyp[it]);
+    }
+  }
 }
 
 // t (x) 0 for the corrections of active molecules

@@ -515,7 +515,7 @@ static void allocate(Batch& b, bool keep_mp2) {
   HCHECK(hipMemset(b.dact, 0, b.nm * sizeof(int)));
 
   b.rgNN = make_ragged(b, sz_nn);
-  b.rgN3 = make_ragged(b, sz_n3);
+  b.rgN3 = make_ragged(b, sz_n3, kPItems);
   b.rgT2 = make_ragged(b, sz_t2);
   b.rgOcc = make_ragged(b, [](int, int o, int) { return (long)o * kE2Split * kEwThreads; });   // kE2Split blocks per i
   // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
