@@ -10,6 +10,7 @@
 // results are bitwise reproducible run to run.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "hcc_batch.h"
 #include "hcc_ops.h"
@@ -161,18 +162,33 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter(const Mol* __restrict__ m
   }
 }
 
-// u = 2 t2 - t2(a<->b)
+// u = 2 t2 - t2(a<->b); kEwItems elements per thread, loads before stores
 template <class MT>
 __global__ __launch_bounds__(kEwThreads) void k_u(const MT* __restrict__ mols, const State* __restrict__ st,
                                                  Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
-  if (e >= size_ || !st[mol].active) return;
-  const int v = M.v;
-  int q, ab, a, b;
-  divmod(e, v * v, q, ab);
-  divmod(ab, v, a, b);
-  const long base = e - ab;
-  M.u[e] = 2.0 * M.t2[e] - (double)M.t2[base + (long)b * v + a];
+  if (!st[mol].active) return;
+  using V = std::remove_pointer_t<decltype(M.t2)>;
+  const int v = M.v, vv = v * v;
+  const gptr<V> t2 = global_view(M.t2);
+  const gwptr<V> u = global_out(M.u);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+  V x[kEwItems], y[kEwItems];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    const long g = f < size_ ? f : 0;
+    int q, ab, a, b;
+    divmod(g, vv, q, ab);
+    divmod(ab, v, a, b);
+    x[it] = t2[g];
+    y[it] = t2[g - ab + (long)b * v + a];
+  }
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < size_) u[f] = 2.0 * x[it] - (double)y[it];
+  }
 }
 
 // Xvv = Ft_vv, Xoo = Ft_oo (the Fock parts of the E2 intermediates)
@@ -198,17 +214,34 @@ template <class MT>
 __global__ __launch_bounds__(kEwThreads) void k_assemble(const MT* __restrict__ mols, const State* __restrict__ st,
                                                         Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
-  if (e >= size_ || !st[mol].active) return;
+  if (!st[mol].active) return;
+  using V = std::remove_pointer_t<decltype(M.w2)>;
   const int o = M.o, v = M.v;
-  int i, j, a, b;
-  ijab_of(e, o, v, i, j, a, b);
-  const long ij = ((long)i * o + j) * v * v, ji = ((long)j * o + i) * v * v;
-  const long ab = (long)a * v + b, ba = (long)b * v + a;
-  const auto* Z = M.Zc;
-  double val = (double)M.x[e] + M.x[ji + ba];
-  val -= 0.5 * (Z[ij + ab] + Z[ji + ba]);
-  val -= Z[ji + ab] + Z[ij + ba];
-  M.w2[e] += val;
+  const gptr<V> x = global_view(M.x), Z = global_view(M.Zc), w2in = global_view(M.w2);
+  const gwptr<V> w2 = global_out(M.w2);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+  // kEwItems elements per thread, loads before stores
+  double val[kEwItems];
+  V w[kEwItems];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    const long g = f < size_ ? f : 0;
+    int i, j, a, b;
+    ijab_of(g, o, v, i, j, a, b);
+    const long ij = ((long)i * o + j) * v * v, ji = ((long)j * o + i) * v * v;
+    const long ab = (long)a * v + b, ba = (long)b * v + a;
+    double s = (double)x[g] + x[ji + ba];
+    s -= 0.5 * (Z[ij + ab] + Z[ji + ba]);
+    s -= Z[ji + ab] + Z[ij + ba];
+    val[it] = s;
+    w[it] = w2in[g];
+  }
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < size_) w2[f] = w[it] + val[it];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -918,10 +951,16 @@ __global__ __launch_bounds__(kEwThreads) void k_pouter_d(const Mol* __restrict__
 __global__ __launch_bounds__(kEwThreads) void k_zero_delta(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                           const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
-  if (e >= size_ || !st[mol].active) return;
+  if (!st[mol].active) return;
   const long n2 = (long)M.o * M.o * M.v * M.v;
-  if (e < n2) dmols[mol].t2[e] = 0.0;
-  else dmols[mol].t1[e - n2] = 0.0;
+  const gwptr<float> t2 = global_out(dmols[mol].t2), t1 = global_out(dmols[mol].t1);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < n2) t2[f] = 0.f;
+    else if (f < size_) t1[f - n2] = 0.f;
+  }
 }
 
 // t1 = 0 for every molecule (a kernel, not per-molecule memsets)
@@ -934,10 +973,29 @@ __global__ __launch_bounds__(kEwThreads) void k_zero_t1(const Mol* __restrict__ 
 __global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                             const State* __restrict__ st, Ragged rg, int nm) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v + (long)mols[mol].o * mols[mol].v);
-  if (e >= size_ || st[mol].status != 0) return;
+  if (st[mol].status != 0) return;
   const long n2 = (long)M.o * M.o * M.v * M.v;
-  if (e < n2) M.t2[e] += dmols[mol].t2[e];
-  else M.t1[e - n2] += dmols[mol].t1[e - n2];
+  const MolD& D = dmols[mol];
+  // t2 and t1 are one index range: element f < n2 of t2, else f - n2 of t1
+  const gptr<double> t2in = global_view(M.t2), t1in = global_view(M.t1);
+  const gptr<float> d2 = global_view(D.t2), d1 = global_view(D.t1);
+  const gwptr<double> t2 = global_out(M.t2), t1 = global_out(M.t1);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+  double t[kEwItems];
+  float d[kEwItems];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    const long g = f < size_ ? f : 0;
+    t[it] = g < n2 ? t2in[g] : t1in[g - n2];
+    d[it] = g < n2 ? d2[g] : d1[g - n2];
+  }
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f < n2) t2[f] = t[it] + d[it];
+    else if (f < size_) t1[f - n2] = t[it] + d[it];
+  }
 }
 
 // Before t0 += dt: the stored inner DIIS amplitude vectors are corrections

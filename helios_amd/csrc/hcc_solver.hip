@@ -125,6 +125,7 @@ struct Batch {
   int* dact = nullptr;
   std::vector<void*> allocs;
   Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL, rgOcc;
+  Ragged rgT2x, rgT2Sx;   // rgT2, rgT2S with kEwItems elements per thread
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
   std::vector<GemmTable> alt;   // second (other tile edge) parts of split contraction tables
@@ -521,6 +522,8 @@ static void allocate(Batch& b, bool keep_mp2) {
   // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
   b.rgPairL = make_ragged(b, [](int, int o, int v) { const long ne = (long)o * v; return ne * (ne + 1) / 2; }, 4);
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
+  b.rgT2x = make_ragged(b, sz_t2, kEwItems);
+  b.rgT2Sx = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; }, kEwItems);
   b.rgOV = make_ragged(b, sz_ov);
   b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
   b.rgN = make_ragged(b, [](int n, int, int) { return (long)n; });
@@ -855,12 +858,12 @@ static void residual(hcc_ctx* c) {
   gemm(c, b.it[G1_Q], G1_Q);
   gemm(c, b.it[G2A], G2A);
   gemm(c, b.it[G2B], G2B);
-  ew_u(b.dm, b.dst, b.rgT2, b.nm, s);
+  ew_u(b.dm, b.dst, b.rgT2x, b.nm, s);
   for (int g = G3; g <= G20; ++g) gemm(c, b.it[g], g);
   ew_init_xx(b.dm, b.dst, b.rgXX, b.nm, s);
   for (int g = G21; g <= G24; ++g) gemm(c, b.it[g], g);
   ew_e2(b.dm, b.dst, b.rgOcc, b.nm, s);
-  ew_assemble(b.dm, b.dst, b.rgT2, b.nm, s);
+  ew_assemble(b.dm, b.dst, b.rgT2x, b.nm, s);
   HCHECK(hipGetLastError());
 }
 
@@ -908,12 +911,12 @@ static void residual_delta(hcc_ctx* c) {
   dgemm(c, b.dit[G1_Q], 32 + G1_Q);
   dgemm(c, b.dit[G2A], 32 + G2A);
   dgemm(c, b.dit[G2B], 32 + G2B);
-  ew_u(b.ddm, b.dst, b.rgT2, b.nm, s);
+  ew_u(b.ddm, b.dst, b.rgT2x, b.nm, s);
   for (int g = G3; g <= G20; ++g) dgemm(c, b.dit[g], 32 + g);
   ew_init_xx(b.ddm, b.dst, b.rgXX, b.nm, s);
   for (int g = G21; g <= G24; ++g) dgemm(c, b.dit[g], 32 + g);
   mixed_e2(b.dm, b.ddm, b.dst, b.rgOcc, b.nm, s);
-  ew_assemble(b.ddm, b.dst, b.rgT2, b.nm, s);
+  ew_assemble(b.ddm, b.dst, b.rgT2x, b.nm, s);
   HCHECK(hipGetLastError());
 }
 
@@ -1450,7 +1453,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     if (alive() == 0) break;
     set_phase(b.dst, b.dact, b.nm, 1, s);
     if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
-    mixed_zero(b.dm, b.ddm, b.dst, b.rgT2S, b.nm, s);
+    mixed_zero(b.dm, b.ddm, b.dst, b.rgT2Sx, b.nm, s);
     first_step();
     for (int inner = 1; inner < ctx->max_inner;) {
       if (graphs && inner + kInnerBlock <= ctx->max_inner) {
@@ -1464,7 +1467,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       }
       if (alive() == 0) break;
     }
-    mixed_commit(b.dm, b.ddm, b.dst, b.rgT2S, b.rgP2, b.nm, prm.keep_diis && prm.diis, s);
+    mixed_commit(b.dm, b.ddm, b.dst, b.rgT2Sx, b.rgP2, b.nm, prm.keep_diis && prm.diis, s);
   }
   set_phase(b.dst, b.dact, b.nm, 0, s);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
