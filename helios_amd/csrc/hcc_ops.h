@@ -7,7 +7,9 @@ namespace hcc {
 constexpr int kEwThreads = 256;
 constexpr int kDotItems = 8;   // elements per thread of a DIIS dot-product workgroup
 constexpr int kPItems = 4;     // elements per thread of the P / PT builders (table rgN3)
-constexpr int kEwItems = 4;    // elements per thread of k_u, k_assemble (rgT2x), k_zero_delta, k_commit_delta (rgT2Sx)
+// elements per thread of k_u, k_assemble, k_update2, k_update2_mixed (table rgT2x, whose
+// block prefix also indexes their max|d| partials) and k_zero_delta, k_commit_delta (rgT2Sx)
+constexpr int kEwItems = 4;
 constexpr int kMaxDiis = 32;
 
 void ew_dress(const Mol* mols, const State* st, Ragged rgNN, int nm, hipStream_t s);

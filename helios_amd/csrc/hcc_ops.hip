@@ -347,43 +347,75 @@ __global__ __launch_bounds__(kEwThreads) void k_update1(const Mol* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
+// Index set of element g of an o x o x v x v array for the pair updates: the
+// pair's representative r ((i,a) <= (j,b)), the representative rx of the pair
+// with a <-> b, the mirror element of g and the packed DIIS index of the pair
+struct PairIx {
+  long r, rx, mir, pk;
+  int I, J, A, B;
+  bool rep;   // g is its pair's representative
+};
+__device__ __forceinline__ PairIx pair_ix(long g, int o, int v) {
+  PairIx q;
+  int i, j, a, b;
+  ijab_of(g, o, v, i, j, a, b);
+  const long ne = (long)o * v, mm = (long)i * v + a, nn = (long)j * v + b;
+  q.rep = mm <= nn;
+  q.I = i; q.J = j; q.A = a; q.B = b;
+  if (!q.rep) { q.I = j; q.J = i; q.A = b; q.B = a; }
+  q.r = (((long)q.I * o + q.J) * v + q.A) * v + q.B;
+  const long mb = (long)q.I * v + q.B, na = (long)q.J * v + q.A;
+  q.rx = mb <= na ? (((long)q.I * o + q.J) * v + q.B) * v + q.A : (((long)q.J * o + q.I) * v + q.A) * v + q.B;
+  q.mir = (((long)j * o + i) * v + b) * v + a;
+  q.pk = ne + packed_index(q.rep ? mm : nn, q.rep ? nn : mm, ne);
+  return q;
+}
+
 // Doubles, from the (i,a)<=(j,b) representative (I,J,A,B) of each pair, so
 // that t2 stays exactly pair-symmetric:
 //   d2 = 2 Omega - Omega(a<->b),  t2 += s2 (2 d2 + d2(a<->b)) / D2 = 3 s2 Omega / D2
+// kEwItems elements per thread (table rgT2x), loads before stores.  The
+// representative of each pair updates both of its elements, so a
+// non-representative element reads no amplitude (its load goes to w2, which
+// no thread writes here).
 __global__ __launch_bounds__(kEwThreads) void k_update2(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                        Ragged rg, int nm, Params prm, double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (!st[mol].active) return;
+  const int N = M.n, o = M.o, v = M.v;
+  const gptr<double> w2 = global_view(M.w2), F = global_view(M.F), t2in = global_view(M.t2);
+  const gwptr<double> t2 = global_out(M.t2), sd = global_out(M.sd), stv = global_out(M.st);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+  PairIx q[kEwItems];
+  double om[kEwItems], omx[kEwItems], dd[kEwItems], t[kEwItems];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    const long g = f < size_ ? f : 0;
+    q[it] = pair_ix(g, o, v);
+    om[it] = w2[q[it].r];
+    omx[it] = w2[q[it].rx];
+    dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
+             F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
+    t[it] = *(q[it].rep ? t2in + g : w2 + q[it].r);
+  }
   double amax = 0.0;
-  if (e < size_) {
-    const int N = M.n, o = M.o, v = M.v;
-    const long ne = (long)o * v;
-    int i, j, a, b;
-    ijab_of(e, o, v, i, j, a, b);
-    const long mm = (long)i * v + a, nn = (long)j * v + b;
-    int I = i, J = j, A = a, B = b;
-    if (mm > nn) { I = j; J = i; A = b; B = a; }
-    const long r = (((long)I * o + J) * v + A) * v + B;
-    // Omega(a<->b) at the representative of the pair (I,B),(J,A)
-    const long mb = (long)I * v + B, na = (long)J * v + A;
-    const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
-    const double om = M.w2[r];
-    const double d2 = 2.0 * om - M.w2[rx];
-    amax = fabs(d2);
-    const double D = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
-                     M.F[(long)(o + B) * N + o + B];
-    const double num = prm.projtype == 2 ? 3.0 * om : d2;
-    if (mm <= nn) {
-      // the representative updates both elements of its pair; no other thread touches them
-      const double tn = M.t2[e] + prm.s2 * num / D;
-      M.t2[e] = tn;
-      M.t2[(((long)j * o + i) * v + b) * v + a] = tn;
+  const State& S = st[mol];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f >= size_) continue;
+    const double d2 = 2.0 * om[it] - omx[it];
+    amax = nmax(amax, fabs(d2));
+    const double num = prm.projtype == 2 ? 3.0 * om[it] : d2;
+    if (q[it].rep) {
+      const double tn = t[it] + prm.s2 * num / dd[it];
+      t2[f] = tn;
+      t2[q[it].mir] = tn;
       if (prm.diis) {
-        const State& S = st[mol];
-        const long p = ne + packed_index(mm, nn, ne);
-        M.sd[(long)S.diis_next * M.nd + p] = num;
-        M.st[(long)S.diis_next * M.nd + p] = tn;
+        sd[(long)S.diis_next * M.nd + q[it].pk] = num;
+        stv[(long)S.diis_next * M.nd + q[it].pk] = tn;
       }
     }
   }
@@ -1155,35 +1187,51 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
   __shared__ double sh[kEwThreads / 64];
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (!st[mol].active) return;
+  const MolD& Dm = dmols[mol];
+  const int N = M.n, o = M.o, v = M.v;
+  const gptr<double> w2 = global_view(M.w2), F = global_view(M.F), t2 = global_view(M.t2);
+  const gptr<float> dw2 = global_view(Dm.w2), dt2in = global_view(Dm.t2);
+  const gwptr<float> dt2 = global_out(Dm.t2), sd = global_out(Dm.sd), stv = global_out(Dm.st);
+  const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
+  // kEwItems elements per thread, loads before stores (as in k_update2; a
+  // non-representative element reads no correction)
+  PairIx q[kEwItems];
+  double om[kEwItems], omx[kEwItems], dd[kEwItems], t0[kEwItems];
+  float dw[kEwItems], dwx[kEwItems], dt[kEwItems];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    const long g = f < size_ ? f : 0;
+    q[it] = pair_ix(g, o, v);
+    om[it] = w2[q[it].r];
+    omx[it] = w2[q[it].rx];
+    // at the first step of an inner solve (dt = 0) the correction residual
+    // arrays hold the previous solve's values and are not used
+    dw[it] = dw2[q[it].r];
+    dwx[it] = dw2[q[it].rx];
+    dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
+             F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
+    t0[it] = t2[g];
+    dt[it] = *(q[it].rep ? dt2in + g : dw2 + q[it].r);
+  }
   double amax = 0.0;
-  if (e < size_) {
-    const MolD& Dm = dmols[mol];
-    const int N = M.n, o = M.o, v = M.v;
-    const long ne = (long)o * v;
-    int i, j, a, b;
-    ijab_of(e, o, v, i, j, a, b);
-    const long mm = (long)i * v + a, nn = (long)j * v + b;
-    int I = i, J = j, A = a, B = b;
-    if (mm > nn) { I = j; J = i; A = b; B = a; }
-    const long r = (((long)I * o + J) * v + A) * v + B;
-    const long mb = (long)I * v + B, na = (long)J * v + A;
-    const long rx = mb <= na ? (((long)I * o + J) * v + B) * v + A : (((long)J * o + I) * v + A) * v + B;
-    const double om = M.w2[r] + (prm.outer_only ? 0.0 : (double)Dm.w2[r]);
-    const double d2 = 2.0 * om - (M.w2[rx] + (prm.outer_only ? 0.0 : (double)Dm.w2[rx]));
-    amax = fabs(d2);
-    const double Dn = M.F[(long)I * N + I] + M.F[(long)J * N + J] - M.F[(long)(o + A) * N + o + A] -
-                      M.F[(long)(o + B) * N + o + B];
-    const double num = prm.projtype == 2 ? 3.0 * om : d2;
-    if (mm <= nn) {
-      const double tn = M.t2[e] + Dm.t2[e] + prm.s2 * num / Dn;
-      const float dt = (float)(tn - M.t2[e]);
-      Dm.t2[e] = dt;
-      Dm.t2[(((long)j * o + i) * v + b) * v + a] = dt;
+  const State& S = st[mol];
+#pragma unroll
+  for (int it = 0; it < kEwItems; ++it) {
+    const long f = f0 + (long)it * kEwThreads;
+    if (f >= size_) continue;
+    const double om_ = om[it] + (prm.outer_only ? 0.0 : (double)dw[it]);
+    const double d2 = 2.0 * om_ - (omx[it] + (prm.outer_only ? 0.0 : (double)dwx[it]));
+    amax = nmax(amax, fabs(d2));
+    const double num = prm.projtype == 2 ? 3.0 * om_ : d2;
+    if (q[it].rep) {
+      const double tn = t0[it] + dt[it] + prm.s2 * num / dd[it];
+      const float dtn = (float)(tn - t0[it]);
+      dt2[f] = dtn;
+      dt2[q[it].mir] = dtn;
       if (prm.diis) {
-        const State& S = st[mol];
-        const long p = ne + packed_index(mm, nn, ne);
-        Dm.sd[(long)S.diis_next * M.nd + p] = (float)num;
-        Dm.st[(long)S.diis_next * M.nd + p] = dt;
+        sd[(long)S.diis_next * M.nd + q[it].pk] = (float)num;
+        stv[(long)S.diis_next * M.nd + q[it].pk] = dtn;
       }
     }
   }

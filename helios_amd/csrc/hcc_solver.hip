@@ -871,8 +871,8 @@ static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
   residual(c);
-  ew_update(b.dm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
-  ctl_after_update(b.dm, b.dst, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
+  ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm, b.part1, b.part2, s);
+  ctl_after_update(b.dm, b.dst, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
   if (prm.diis) diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
   ctl_after_diis(b.dm, b.dst, b.dact, b.nm, prm, stop, s);
   HCHECK(hipGetLastError());
@@ -1419,8 +1419,8 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   };
   auto inner_iteration = [&]() {
     residual_delta(ctx);
-    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm, b.part1, b.part2, s);
-    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm, b.part1, b.part2, s);
+    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     if (prm.diis) {
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
@@ -1432,8 +1432,8 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   Params prm1 = prm;
   prm1.outer_only = 1;
   auto first_step = [&]() {
-    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2, b.nm, prm1, b.part1, b.part2, s);
-    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm1, b.part1, b.part2, s);
+    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     if (prm1.diis) {
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
@@ -1448,8 +1448,8 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
     set_phase(b.dst, b.dact, b.nm, 0, s);
     residual(ctx);
-    ew_update(b.dm, b.dst, b.rgOV, b.rgT2, b.nm, prm0, b.part1, b.part2, s);
-    mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2, b.part2, s);
+    ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm0, b.part1, b.part2, s);
+    mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     if (alive() == 0) break;
     set_phase(b.dst, b.dact, b.nm, 1, s);
     if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
