@@ -25,6 +25,7 @@ void ew_assemble(const Mol* mols, const State* st, Ragged rgT2, int nm, hipStrea
 // per occupied orbital
 constexpr int kE2Max = 32;
 constexpr int kE2Split = 4;
+constexpr int kE2Slots = (kE2Max * kE2Max + kEwThreads - 1) / kEwThreads;   // staged elements per thread
 void ew_e2(const Mol* mols, const State* st, Ragged rgOcc, int nm, hipStream_t s);
 void mixed_e2(const Mol* m0, const MolD* md, const State* st, Ragged rgOcc, int nm, hipStream_t s);
 void ew_assemble(const MolD* mols, const State* st, Ragged rgT2, int nm, hipStream_t s);

@@ -1125,20 +1125,46 @@ __global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mol
     sOd[r] = D.Xoo[q];
     sO1[r] = (float)(M.Xoo[q] + (double)D.Xoo[q]);
   }
+  // global loads (generic ones would share the LDS wait counter); the next
+  // a's slice is in flight while the current one is computed
+  const gptr<double> t2 = global_view(M.t2);
+  const gptr<float> dt2 = global_view(D.t2), xin = global_view(D.x);
+  const gwptr<float> x = global_out(D.x);
+  const int ov = o * v;
+  float n0[kE2Slots], nd[kE2Slots];
+  auto fetch = [&](int a) {
+#pragma unroll
+    for (int r = 0; r < kE2Slots; ++r) {
+      const int q = threadIdx.x + r * kEwThreads;
+      const long e = (((long)i * o + (q < ov ? q : 0) / v) * v + a) * v + (q < ov ? q : 0) % v;
+      n0[r] = (float)t2[e];
+      nd[r] = dt2[e];
+    }
+  };
+  if (a0 < a1) fetch(a0);
   for (int a = a0; a < a1; ++a) {
     __syncthreads();
-    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
-      const long e = (((long)i * o + q / v) * v + a) * v + q % v;
-      sS0[q] = (float)M.t2[e];
-      sSd[q] = D.t2[e];
+#pragma unroll
+    for (int r = 0; r < kE2Slots; ++r) {
+      const int q = threadIdx.x + r * kEwThreads;
+      if (q < ov) {
+        sS0[q] = n0[r];
+        sSd[q] = nd[r];
+      }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
+    if (a + 1 < a1) fetch(a + 1);
+#pragma unroll
+    for (int r = 0; r < kE2Slots; ++r) {
+      const int q = threadIdx.x + r * kEwThreads;
+      if (q >= ov) continue;
       const int j = q / v, b = q % v;
+      const long ox = (((long)i * o + j) * v + a) * v + b;
+      const float x0 = xin[ox];
       float s = 0.f;
       for (int c = 0; c < v; ++c) s += sS0[j * v + c] * sXd[b * (v + 1) + c] + sSd[j * v + c] * sX1[b * (v + 1) + c];
       for (int k = 0; k < o; ++k) s -= sS0[k * v + b] * sOd[k * (o + 1) + j] + sSd[k * v + b] * sO1[k * (o + 1) + j];
-      D.x[(((long)i * o + j) * v + a) * v + b] += s;
+      x[ox] = x0 + s;
     }
   }
 }
