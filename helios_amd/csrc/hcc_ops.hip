@@ -375,9 +375,8 @@ __device__ __forceinline__ PairIx pair_ix(long g, int o, int v) {
 // that t2 stays exactly pair-symmetric:
 //   d2 = 2 Omega - Omega(a<->b),  t2 += s2 (2 d2 + d2(a<->b)) / D2 = 3 s2 Omega / D2
 // kEwItems elements per thread (table rgT2x), loads before stores.  The
-// representative of each pair updates both of its elements, so a
-// non-representative element reads no amplitude (its load goes to w2, which
-// no thread writes here).
+// representative of each pair updates both of its elements and contributes the
+// pair's max|d|; a non-representative element reads nothing.
 __global__ __launch_bounds__(kEwThreads) void k_update2(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                        Ragged rg, int nm, Params prm, double* __restrict__ part) {
   __shared__ double sh[kEwThreads / 64];
@@ -394,18 +393,22 @@ __global__ __launch_bounds__(kEwThreads) void k_update2(const Mol* __restrict__ 
     const long f = f0 + (long)it * kEwThreads;
     const long g = f < size_ ? f : 0;
     q[it] = pair_ix(g, o, v);
-    om[it] = w2[q[it].r];
-    omx[it] = w2[q[it].rx];
-    dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
-             F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
-    t[it] = *(q[it].rep ? t2in + g : w2 + q[it].r);
+    om[it] = omx[it] = dd[it] = t[it] = 0.0;
+    if (q[it].rep) {
+      om[it] = w2[q[it].r];
+      omx[it] = w2[q[it].rx];
+      dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
+               F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
+      t[it] = t2in[g];
+    }
   }
   double amax = 0.0;
   const State& S = st[mol];
 #pragma unroll
   for (int it = 0; it < kEwItems; ++it) {
     const long f = f0 + (long)it * kEwThreads;
-    if (f >= size_) continue;
+    // a non-representative element's residual is its representative's
+    if (f >= size_ || !q[it].rep) continue;
     const double d2 = 2.0 * om[it] - omx[it];
     amax = nmax(amax, fabs(d2));
     const double num = prm.projtype == 2 ? 3.0 * om[it] : d2;
@@ -1219,8 +1222,7 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
   const gptr<float> dw2 = global_view(Dm.w2), dt2in = global_view(Dm.t2);
   const gwptr<float> dt2 = global_out(Dm.t2), sd = global_out(Dm.sd), stv = global_out(Dm.st);
   const long f0 = (long)lb * kEwItems * kEwThreads + threadIdx.x;
-  // kEwItems elements per thread, loads before stores (as in k_update2; a
-  // non-representative element reads no correction)
+  // kEwItems elements per thread, loads before stores (as in k_update2)
   PairIx q[kEwItems];
   double om[kEwItems], omx[kEwItems], dd[kEwItems], t0[kEwItems];
   float dw[kEwItems], dwx[kEwItems], dt[kEwItems];
@@ -1229,23 +1231,28 @@ __global__ __launch_bounds__(kEwThreads) void k_update2_mixed(const Mol* __restr
     const long f = f0 + (long)it * kEwThreads;
     const long g = f < size_ ? f : 0;
     q[it] = pair_ix(g, o, v);
-    om[it] = w2[q[it].r];
-    omx[it] = w2[q[it].rx];
-    // at the first step of an inner solve (dt = 0) the correction residual
-    // arrays hold the previous solve's values and are not used
-    dw[it] = dw2[q[it].r];
-    dwx[it] = dw2[q[it].rx];
-    dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
-             F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
-    t0[it] = t2[g];
-    dt[it] = *(q[it].rep ? dt2in + g : dw2 + q[it].r);
+    om[it] = omx[it] = dd[it] = t0[it] = 0.0;
+    dw[it] = dwx[it] = dt[it] = 0.f;
+    if (q[it].rep) {
+      om[it] = w2[q[it].r];
+      omx[it] = w2[q[it].rx];
+      // at the first step of an inner solve (dt = 0) the correction residual
+      // arrays hold the previous solve's values and are not used
+      dw[it] = dw2[q[it].r];
+      dwx[it] = dw2[q[it].rx];
+      dd[it] = F[(long)q[it].I * N + q[it].I] + F[(long)q[it].J * N + q[it].J] -
+               F[(long)(o + q[it].A) * N + o + q[it].A] - F[(long)(o + q[it].B) * N + o + q[it].B];
+      t0[it] = t2[g];
+      dt[it] = dt2in[g];
+    }
   }
   double amax = 0.0;
   const State& S = st[mol];
 #pragma unroll
   for (int it = 0; it < kEwItems; ++it) {
     const long f = f0 + (long)it * kEwThreads;
-    if (f >= size_) continue;
+    // a non-representative element's residual is its representative's
+    if (f >= size_ || !q[it].rep) continue;
     const double om_ = om[it] + (prm.outer_only ? 0.0 : (double)dw[it]);
     const double d2 = 2.0 * om_ - (omx[it] + (prm.outer_only ? 0.0 : (double)dwx[it]));
     amax = nmax(amax, fabs(d2));
