@@ -436,9 +436,12 @@ struct Stage {
 };
 constexpr int kDualBK = 32;   // 64-deep slabs measured slower (K padding of the short contractions, fewer waves)
 // TS: output tile edge, 32 (2 waves of 32x16), 64 (4 waves of 32x32) or 128
-// (8 waves of 32x64)
-template <bool AKF, bool BNF, bool HA, bool HB, int TS>
-__global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(4, 8))) void gemm_dual_bf16(
+// (8 waves of 32x64).  SK (short K, at most two slabs): the next slab is loaded
+// after the current one's matrix-core work instead of during it; the staging
+// registers are then dead during that work, which lets six waves share a SIMD
+// (a one- or two-slab tile is mostly fixed latency, hidden by more tiles per CU)
+template <bool AKF, bool BNF, bool HA, bool HB, int TS, bool SK>
+__global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(SK ? 6 : 4, 8))) void gemm_dual_bf16(
     const GemmProb* __restrict__ probs, const int* __restrict__ tile_prob, int total_tiles,
     const int* __restrict__ active) {
   constexpr int BK = kDualBK, BM = TS, BN = TS, T = TS * 4, LD = BK + 8, NE = 8;
@@ -543,7 +546,7 @@ __global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         }
       }
       __syncthreads();
-      load(min(kl + BK, clen));   // the last prefetch reads only padding
+      if (!SK) load(min(kl + BK, clen));   // the last prefetch reads only padding
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 32) {
         const int kq = kk + 8 * (lane >> 4);
@@ -564,6 +567,7 @@ __global__ __launch_bounds__(TS * 4) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         if (HA) product(dAs, Bfs);
       }
       __syncthreads();
+      if (SK && kl + BK < clen) load(kl + BK);
     }
   }
   // epilogue: V_MFMA_F32_16X16X32_BF16 leaves D[row][col] of a 16x16 block in
@@ -609,23 +613,27 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tp, int total_tile
   if (total_tiles <= 0) return hipSuccess;
   const int grid = ((total_tiles + 7) / 8) * 8;
   if (bf16) {
-#define HCC_DUAL16(AK, BN, TS)                                                                          \
+#define HCC_DUAL16(AK, BN, TS, SK)                                                                      \
   switch ((flags >> 2) & 3) {                                                                           \
-    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
-    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
-    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true, TS>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
+    case 1: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, false, TS, SK>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
+    case 2: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, false, true, TS, SK>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
+    case 3: hipLaunchKernelGGL((gemm_dual_bf16<AK, BN, true, true, TS, SK>), dim3(grid), dim3(4 * TS), 0, s, probs, tp, total_tiles, active); break; \
     default: return hipErrorInvalidValue; /* both operands constant: not a correction product */     \
   }
-#define HCC_DUAL16_T(TS)                           \
-    switch (flags & 3) {                           \
-      case 0: HCC_DUAL16(false, false, TS) break;  \
-      case 1: HCC_DUAL16(true, false, TS) break;   \
-      case 2: HCC_DUAL16(false, true, TS) break;   \
-      default: HCC_DUAL16(true, true, TS) break;   \
+#define HCC_DUAL16_T(TS, SK)                           \
+    switch (flags & 3) {                               \
+      case 0: HCC_DUAL16(false, false, TS, SK) break;  \
+      case 1: HCC_DUAL16(true, false, TS, SK) break;   \
+      case 2: HCC_DUAL16(false, true, TS, SK) break;   \
+      default: HCC_DUAL16(true, true, TS, SK) break;   \
     }
-    if (tile == 64) { HCC_DUAL16_T(64) }
-    else if (tile == 32) { HCC_DUAL16_T(32) }
-    else if (tile == 256) { HCC_DUAL16_T(128) }
+    // flag 16: every problem of the table has K <= 2 slabs (short-K variant)
+    const bool sk = (flags & 16) != 0;
+    if (tile == 64 && sk) { HCC_DUAL16_T(64, true) }
+    else if (tile == 64) { HCC_DUAL16_T(64, false) }
+    else if (tile == 32 && sk) { HCC_DUAL16_T(32, true) }
+    else if (tile == 32) { HCC_DUAL16_T(32, false) }
+    else if (tile == 256) { HCC_DUAL16_T(128, false) }
     else return hipErrorInvalidValue;
 #undef HCC_DUAL16_T
 #undef HCC_DUAL16

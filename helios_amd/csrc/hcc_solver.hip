@@ -58,7 +58,7 @@ struct GemmTable {
   int red_blocks = 0;
   int tile = 64;               // output tile edge (64 or 32)
   int alt = -1;                // second part of the same contraction (other tile edge), index in Batch::alt
-  int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3
+  int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3 | (K <= 64) << 4
 };
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
@@ -244,6 +244,10 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
       if ((double)probs[q].M * probs[q].N * probs[q].K > (double)probs[big].M * probs[big].N * probs[big].K) big = q;
     t.flags = probs[big].a_kfast | (probs[big].b_nfast << 1) | ((probs[big].dA != nullptr) << 2) |
               ((probs[big].dB != nullptr) << 3);
+    // short-K variant of the correction GEMM when no problem has more than two 32-deep slabs
+    int kmax = 0;
+    for (const GemmProb& p : probs) kmax = std::max(kmax, p.kchunk);
+    if (kmax <= 64) t.flags |= 16;
   }
   if (ws_total > 0) {
     double* ws = b.dalloc<double>(ws_total);
@@ -1793,7 +1797,7 @@ int hcc_selftest_gemm_correction(int m, int n, int k, int a_kfast, int b_nfast, 
     p.ksplit = 1;
     p.kchunk = k;
     const int tiles = ((m + tile - 1) / tile) * p.tiles_n;
-    const int flags = (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (dA ? 4 : 0) | (dB ? 8 : 0);
+    const int flags = (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (dA ? 4 : 0) | (dB ? 8 : 0) | (k <= 64 && tile != 128 ? 16 : 0);
     HCHECK(hipMalloc(&dp, sizeof(GemmProb)));
     HCHECK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
     HCHECK(hipMalloc(&dtp, sizeof(int) * tiles));
