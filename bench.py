@@ -35,8 +35,8 @@ PEAK_BF16 = 2.5e15
 # --no-cpu --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  The GEMM
 # kernels never touch the DIIS arrays, so the figures hold at any DIIS depth.  A counter
 # pass cannot run inside this process, so other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 2.573e9        # gemm_f64_kernel, FP64 solve
-DUAL_GEMM_TRAFFIC_BATCH4096 = 1.282e9   # gemm_dual_bf16, mixed solve
+GEMM_TRAFFIC_BATCH4096 = 2.577e9        # gemm_f64_kernel, FP64 solve
+DUAL_GEMM_TRAFFIC_BATCH4096 = 1.328e9   # gemm_dual_bf16, mixed solve
 
 
 def _prep(k):
