@@ -32,6 +32,7 @@ __device__ __forceinline__ int block_mol(const int* __restrict__ prefix, int nm,
   const auto& M = mols[mol];                                         \
   const long e = (long)lb * kEwThreads + threadIdx.x;                \
   const long size_ = (SIZE);                                         \
+  (void)e;                                                           \
   (void)size_;
 
 // Index decompositions in 32-bit unsigned arithmetic (64-bit division is a
