@@ -29,6 +29,7 @@ from helios_amd import autotune, capi, learned_guess, molecules, ppp  # noqa: E4
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
+PEAK_HBM = 8.0e12        # HBM3E bytes/s
 # HBM bytes per dispatch of the dominant kernels on the default 4096-molecule batch
 # with the default solver settings: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --diis 8 --steps 1 --warmup 0
@@ -46,6 +47,11 @@ def _prep(k):
 
 
 def prepare_batch(nmol, workers):
+    """Host SCF of every molecule.  workers = 0 prepares them in this process
+    (for profiler runs: no fork after a profiler library has started the
+    runtime); otherwise a pool forked before any GPU call."""
+    if workers <= 0:
+        return [_prep(k) for k in range(nmol)]
     with mp.Pool(workers) as pool:
         return pool.map(_prep, range(nmol), chunksize=16)
 
@@ -53,6 +59,41 @@ def prepare_batch(nmol, workers):
 def f_iter(o, v):
     """Algorithmic contraction flops per molecule-iteration: 2o^2v^4 + 4o^4v^2 + 12o^3v^3."""
     return 2.0 * o * o * v ** 4 + 4.0 * o ** 4 * v * v + 12.0 * o ** 3 * v ** 3
+
+
+def f_int(o, v):
+    """Algorithmic flops of one build of the six MO integral classes:
+    sum over (P, Q) of 2 P N^2 + 2 P Q N, (P, Q) in {(o^2,o^2), (o^2,ov), (o^2,v^2),
+    (ov,ov), (ov,v^2), (v^2,v^2)}."""
+    n = o + v
+    pq = [(o * o, o * o), (o * o, o * v), (o * o, v * v), (o * v, o * v), (o * v, v * v), (v * v, v * v)]
+    return sum(2.0 * P * n * n + 2.0 * P * Q * n for P, Q in pq)
+
+
+def diis_bytes(o, v, iters, m):
+    """Algorithmic DIIS bytes of a solve: 8 nd (2 h + 4) per iteration that
+    extrapolates (every iteration but the converged last one), h the history
+    length in use (min(iteration, m)), nd = ov(ov+1)/2 + ov."""
+    ov = o * v
+    nd = ov * (ov + 1) / 2 + ov
+    total = np.zeros(len(o))
+    for k in range(len(o)):
+        h = np.minimum(np.arange(1, int(iters[k])), m)
+        total[k] = (8.0 * nd[k] * (2 * h + 4)).sum()
+    return total
+
+
+def copy_calibration():
+    """A plain device copy of 1 GiB, three times, to calibrate the HBM traffic
+    counters of a profiler pass (FETCH_SIZE, WRITE_SIZE per dispatch)."""
+    import torch
+    n = (1 << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {1 << 30} bytes x 3", flush=True)
 
 
 def cpu_baseline(mols, budget_s):
@@ -103,7 +144,13 @@ def main():
     ap.add_argument("--kappa", type=float, default=0.05)
     ap.add_argument("--max-inner", type=int, default=5)
     ap.add_argument("--max-outer", type=int, default=30)
+    ap.add_argument("--no-profile", action="store_true", help="skip the profiled solves (kernel attribution)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB three times (HBM counter calibration for profiler passes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark runs on one MI355X; "
               "across GPUs the batch would shard by molecule", file=sys.stderr)
@@ -141,28 +188,41 @@ def main():
         ctx.batch_load(mols)
         for _ in range(args.warmup):
             ctx.batch_solve(guess, 2000, args.eps)
-        ctx.profile(True)
+        # timed steps: the shipped path (iteration blocks replayed as HIP graphs,
+        # no per-launch events)
         t_steps = []
         for _ in range(args.steps):
             t1 = time.perf_counter()
             ctx.batch_solve(guess, 2000, args.eps)   # returns after a device synchronize
             t_steps.append(time.perf_counter() - t1)
         e, iters, st = ctx.batch_results()
-        gemm_ms, launches = ctx.profile_read()
-        ops_ms = ctx.profile_ops()
-        fl_it, fl_pre = ctx.gemm_flops(len(mols))
         outer = inner = None
         fl_corr = None
         if precision == capi.HCC_MIXED:
             outer, inner = ctx.batch_statistics()
             fl_corr = ctx.gemm_flops_correction(len(mols))
+        # one more solve, profiled (every launch group bracketed by HIP events on
+        # the context's stream, graphs off): per-kernel-class device time
+        gemm_ms = launches = prof_ms = None
+        ops_ms = np.zeros(96)
+        if not args.no_profile:
+            ctx.profile(True)
+            t1 = time.perf_counter()
+            ctx.batch_solve(guess, 2000, args.eps)
+            prof_ms = (time.perf_counter() - t1) * 1e3
+            gemm_ms, launches = ctx.profile_read()
+            ops_ms = ctx.profile_ops()
+            ctx.profile(False)
+            ep, itp, stp = ctx.batch_results()
+            assert np.array_equal(itp, iters) and np.array_equal(stp, st), "profiled solve differs"
+        fl_it, fl_pre = ctx.gemm_flops(len(mols))
         ctx.close()
         de = np.abs(e[:, 0] - e_ref[:, 0]) if not args.skip_reference else np.zeros(len(mols))
         ok = (de < 1e-6) & (st == 1) & (st_ref == 1)
         step_s = float(np.mean(t_steps))
         return dict(e=e, iters=iters, st=st, de=de, ok=ok, step_s=step_s, gemm_ms=gemm_ms, launches=launches,
                     ops_ms=ops_ms, fl_it=fl_it, fl_pre=fl_pre, outer=outer, inner=inner, fl_corr=fl_corr,
-                    solves=ok.sum() / step_s)
+                    solves=ok.sum() / step_s, profiled_solve_ms=prof_ms)
 
     r64 = timed(capi.HCC_FP64, capi.GUESS_MP2)
     print(f"# fp64: {r64['solves']:.1f} solves/s, {r64['step_s']*1e3:.1f} ms/step, "
@@ -183,33 +243,69 @@ def main():
                   file=sys.stderr, flush=True)
     head = rmx if rmx is not None else r64
 
-    default_workload = (args.nmol == 4096 and args.eps == 2e-7 and args.kappa == 0.05
+    default_workload = (args.nmol == 4096 and args.eps == 2e-7 and args.kappa == 0.05 and args.diis == 20
                         and args.max_inner == 5 and args.max_outer == 30 and not args.no_tuning)
-    # roofline of the FP64 contraction GEMM kernel (the FP64 solver's dominant kernel)
-    gemm_flops = args.steps * float((r64["fl_it"] * r64["iters"]).sum() + r64["fl_pre"].sum())
-    achieved = gemm_flops / (r64["gemm_ms"] * 1e-3)
-    falg = float((f_iter(o, v) * r64["iters"]).sum())
-
-    rfp64 = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-             "traffic": GEMM_TRAFFIC_BATCH4096 if default_workload else None,
-             "kernel": "gemm_f64_kernel (FP64 solver)", "launches": int(r64["launches"]),
-             "kernel_ms": r64["gemm_ms"], "gemm_flops": gemm_flops}
-    roofline = rfp64
-    if rmx is not None:
-        # the headline solver's dominant kernel: the BF16 correction GEMM of the inner iterations
-        corr_ms = float(np.sum(rmx["ops_ms"][32:57]))
-        # every inner solve starts with a step taken from the outer residual, without
-        # correction GEMMs: a molecule evaluates them inner - (outer - 1) times
-        corr_evals = rmx["inner"] - (rmx["outer"] - 1)
-        corr_flops = args.steps * float((rmx["fl_corr"] * corr_evals).sum())
-        ach = corr_flops / (corr_ms * 1e-3)
-        roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_BF16, "traffic": DUAL_GEMM_TRAFFIC_BATCH4096 if default_workload else None,
-                    "kernel": "gemm_dual_bf16 (mixed solver inner iterations; bound in practice by operand "
-                              "delivery, see roofline_fp64_solver for the FP64 GEMM)",
-                    "kernel_ms": corr_ms, "gemm_flops": corr_flops}
-
+    rooflines = {}
+    if not args.no_profile:
+        ms = r64["ops_ms"]
+        its = r64["iters"].astype(np.float64)
+        # FP64 contraction GEMM: executed flops (2MNK per launched problem) and the
+        # algorithmic count F_iter per molecule-iteration, over the profiled solve
+        gemm_exec = float((r64["fl_it"] * its).sum() + r64["fl_pre"].sum())
+        gemm_ms = float(r64["gemm_ms"])
+        falg64 = float((f_iter(o, v) * its).sum())
+        rfp64 = {"bound": "mfma", "achieved": falg64 / (gemm_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                 "unit": "TFLOP/s", "frac": falg64 / (gemm_ms * 1e-3) / PEAK_FP64,
+                 "executed_tflops": gemm_exec / (gemm_ms * 1e-3) / 1e12,
+                 "traffic": GEMM_TRAFFIC_BATCH4096 if default_workload else None,
+                 "kernel": "gemm_f64_kernel (FP64 solver, all contractions)", "launches": int(r64["launches"]),
+                 "kernel_ms": gemm_ms, "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per molecule-iteration"}
+        rooflines["roofline_fp64_solver"] = rfp64
+        # integral build: the dressed integral blocks are rebuilt every iteration
+        # (GEMM tables g1, g3, g5, g8, g9, g11, g14, g17, g18 and the set-up (kc|ld))
+        int_ids = [0, 3, 5, 8, 9, 11, 14, 17, 18]
+        int_ms = float(ms[int_ids].sum() + ms[62] + ms[63])
+        fint = float((f_int(o, v) * (its + 1)).sum())
+        rooflines["roofline_integrals"] = {
+            "bound": "mfma", "achieved": fint / (int_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": fint / (int_ms * 1e-3) / PEAK_FP64, "kernel_ms": int_ms,
+            "work": "F_int = sum (2PN^2 + 2PQN) over the six classes, per build (one per iteration + set-up)"}
+        upd_ms = float(ms[64])
+        upd_b = float((24.0 * (o * v) ** 2 * its).sum())
+        rooflines["roofline_update"] = {
+            "bound": "hbm", "achieved": upd_b / (upd_ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": upd_b / (upd_ms * 1e-3) / PEAK_HBM, "kernel_ms": upd_ms,
+            "kernel": "k_update1 + k_update2 (FP64 solver)", "work": "24 o^2 v^2 bytes per molecule-iteration"}
+        diis_ms = float(ms[65])
+        dbytes = float(diis_bytes(o, v, its, args.diis).sum())
+        rooflines["roofline_diis"] = {
+            "bound": "hbm", "achieved": dbytes / (diis_ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": dbytes / (diis_ms * 1e-3) / PEAK_HBM, "kernel_ms": diis_ms,
+            "kernel": "k_diis_dot + k_diis_solve + k_diis_extrap (FP64 solver)",
+            "work": "8 nd (2h + 4) bytes per molecule-iteration, h = history in use"}
+        roofline = rfp64
+        if rmx is not None:
+            mm = rmx["ops_ms"]
+            corr_ms = float(mm[32:57].sum())
+            # every inner solve starts with a step taken from the outer residual,
+            # without correction GEMMs: a molecule evaluates them inner - (outer - 1) times
+            corr_evals = (rmx["inner"] - (rmx["outer"] - 1)).astype(np.float64)
+            falg = float((f_iter(o, v) * corr_evals).sum())
+            fexec = float((rmx["fl_corr"] * corr_evals).sum())
+            roofline = {"bound": "mfma", "achieved": falg / (corr_ms * 1e-3) / 1e12, "peak": PEAK_BF16 / 1e12,
+                        "unit": "TFLOP/s", "frac": falg / (corr_ms * 1e-3) / PEAK_BF16,
+                        "executed_tflops": fexec / (corr_ms * 1e-3) / 1e12,
+                        "traffic": DUAL_GEMM_TRAFFIC_BATCH4096 if default_workload else None,
+                        "kernel": "gemm_dual_bf16 (mixed solver, correction contractions of the inner iterations)",
+                        "kernel_ms": corr_ms,
+                        "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per correction evaluation"}
+            rooflines["time_split_mixed_ms"] = {
+                "correction_gemm": corr_ms, "outer_fp64_gemm": float(mm[0:25].sum()),
+                "setup_gemm": float(mm[60:64].sum()),
+                "elementwise": float(mm[64:96].sum()),
+                "profiled_solve": rmx["profiled_solve_ms"]}
+    else:
+        roofline = None
     line = {
         "metric": "converged CCSD solves/s at |dE|<1e-6 eV",
         "value": float(head["solves"]),
@@ -234,15 +330,16 @@ def main():
                    "fp64_mean_iterations": float(r64["iters"].mean()),
                    "speedup_vs_fp64": float(head["solves"] / r64["solves"]) if rmx is not None else 1.0},
         "roofline": roofline,
-        "roofline_fp64_solver": rfp64,
-        "f_iter_rate_tflops": falg / r64["step_s"] / 1e12,
+        "f_iter_rate_tflops": float((f_iter(o, v) * r64["iters"]).sum()) / r64["step_s"] / 1e12,
     }
+    line.update(rooflines)
     if rmx is not None:
         line["config"].update({"mixed_mean_outer": float(rmx["outer"].mean()),
                                "mixed_mean_inner": float(rmx["inner"].mean()),
                                "mixed_mean_correction_evaluations":
                                    float((rmx["inner"] - (rmx["outer"] - 1)).mean()),
                                "mixed_gemm_ms": rmx["gemm_ms"]})
+        line["config"]["mixed_max_abs_dE_per_molecule_eV"] = float(rmx["de"].max())
     if ablation:
         line["config"]["ablation"] = ablation
     if not args.no_cpu:

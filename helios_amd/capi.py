@@ -32,6 +32,8 @@ SIGNATURES = {
     "hcc_create": (_i, [_P(_vp), _i, _i, _i]),
     "hcc_set_options": (_i, [_vp, _d, _d, _i, _i]),
     "hcc_set_mixed": (_i, [_vp, _i, _d, _i, _i]),
+    "hcc_set_iterator": (_i, [_vp, _i, _d, _i, _d, _i, _i]),
+    "hcc_request_stop": (_i, [_vp]),
     "hcc_batch_statistics": (_i, [_vp, _P(_i), _P(_i)]),
     "hcc_set_molecule": (_i, [_vp, _i, _i, _P(_d), _P(_d), _P(_d), _d]),
     "hcc_guess": (_i, [_vp, _i]),
@@ -147,6 +149,17 @@ class Context:
         self._check(self.lib.hcc_set_options(self.h, t1_step, t2_step, projection_type, int(diis)),
                     "hcc_set_options")
 
+    def set_iterator(self, check_divergence=True, divergence=1e3, check_stagnation=True, stagnation=60.0,
+                     stagnation_after=200, enable_stop=True):
+        """The reference's iteration block (definebd.f90:53-58)."""
+        self._check(self.lib.hcc_set_iterator(self.h, int(check_divergence), float(divergence),
+                                              int(check_stagnation), float(stagnation), int(stagnation_after),
+                                              int(enable_stop)), "hcc_set_iterator")
+
+    def request_stop(self):
+        """Asks a running solve (from another thread) to stop with status -4."""
+        self._check(self.lib.hcc_request_stop(self.h), "hcc_request_stop")
+
     def set_mixed(self, bf16=True, kappa=0.05, max_inner=5, max_outer=30):
         self._check(self.lib.hcc_set_mixed(self.h, int(bf16), kappa, max_inner, max_outer), "hcc_set_mixed")
 
@@ -234,7 +247,8 @@ class Context:
         return ms.value, n.value
 
     def profile_ops(self):
-        ms = np.zeros(64)
+        """Profiled device ms per slot (96 slots, see hcc_profile_ops)."""
+        ms = np.zeros(96)
         self._check(self.lib.hcc_profile_ops(self.h, _dp(ms)), "hcc_profile_ops")
         return ms
 

@@ -254,6 +254,19 @@ __device__ __forceinline__ long packed_index(long mm, long nn, long ne) {
 }
 
 
+// (mm, nn), mm <= nn, of packed index p (the inverse of packed_index): the
+// row is the smaller root of mm^2 - (2 ne + 1) mm + 2 p = 0 rounded down,
+// corrected by one either way against the exact integer row starts
+__device__ __forceinline__ void packed_pair(long p, long ne, int& mm, int& nn) {
+  const double b = 2.0 * (double)ne + 1.0;
+  long r = (long)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+  r = r < 0 ? 0 : (r >= ne ? ne - 1 : r);
+  if (r * ne - r * (r - 1) / 2 > p) --r;
+  else if (r + 1 < ne && (r + 1) * ne - (r + 1) * r / 2 <= p) ++r;
+  mm = (int)r;
+  nn = (int)(r + (p - (r * ne - r * (r - 1) / 2)));
+}
+
 // maximum that propagates NaN (fmax drops it), so that a NaN residual can
 // never read as a converged one
 __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
@@ -497,7 +510,7 @@ __global__ void k_control2(const Mol* __restrict__ mols, State* __restrict__ st,
   int status = 0;
   if (S.status == 1) status = 1;
   else if (stop != nullptr && stop[mol]) status = -4;
-  else if (prm.check_divergence && !(S.cur <= prm.divergence)) status = -2;   // NaN diverges too
+  else if (!(S.cur == S.cur) || (prm.check_divergence && S.cur > prm.divergence)) status = -2;   // NaN diverges always
   else if (S.iter + 1 > prm.maxiter) status = -1;
   else if (prm.check_stagnation && S.iter > prm.stagnation_after &&
            100.0 * S.successful / S.iter <= prm.stagnation_pct) status = -3;
@@ -634,13 +647,13 @@ __global__ __launch_bounds__(64) void k_diis_solve(const MT* __restrict__ mols, 
   for (int k = 0; k < cnt; ++k) S.coef[k] = x[k];
 }
 
-// t = sum_k c_k st_k.  Grid: singles, then the (mm, nn) square of excitation
-// pairs; each upper-triangle element reads its packed entry once (coalesced
-// along nn) and writes both mirror elements of t2.
+// t = sum_k c_k st_k.  Grid: one thread per packed DIIS element (singles,
+// then the pairs mm <= nn in packed order, read coalesced); each pair writes
+// both mirror elements of t2.
 template <class MT>
 __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict__ mols, const State* __restrict__ st,
                                                            Ragged rg, int nm) {
-  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v * (1L + (long)mols[mol].o * mols[mol].v));
+  RAGGED_PROLOGUE((long)mols[mol].nd);
   const State& S = st[mol];
   if (e >= size_ || !S.active || !S.staged || S.diis_count < 2) return;
   const int o = M.o, v = M.v;
@@ -649,11 +662,9 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_extrap(const MT* __restrict
     M.t1[e] = diis_combine(S, M.st, M.nd, e);
     return;
   }
-  const long f = e - ne;
+  const long p = e;
   int mm, nn, i, a, j, b;
-  divmod(f, (int)ne, mm, nn);
-  if (mm > nn) return;
-  const long p = ne + packed_index(mm, nn, ne);
+  packed_pair(e - ne, ne, mm, nn);
   const double s = diis_combine(S, M.st, M.nd, p);
   divmod(mm, v, i, a);
   divmod(nn, v, j, b);
@@ -786,6 +797,18 @@ __global__ void k_reset_state(State* __restrict__ st, int* __restrict__ act, int
   act[mol] = 1;
 }
 
+// stop request: every running molecule ends with status -4 (interrupted)
+__global__ void k_stop_running(State* __restrict__ st, int* __restrict__ act, int nm) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  State& S = st[mol];
+  if (S.status != 0) return;
+  S.status = -4;
+  S.active = 0;
+  S.inner = 0;
+  act[mol] = 0;
+}
+
 // re-arm molecules for the single-step interface (the caller runs the iterator)
 __global__ void k_activate(State* __restrict__ st, int* __restrict__ act, int nm) {
   const int mol = blockIdx.x * blockDim.x + threadIdx.x;
@@ -885,6 +908,9 @@ void reset_state(State* st, int* act, int nm, hipStream_t s) {
 }
 void activate(State* st, int* act, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_activate, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
+}
+void stop_running(State* st, int* act, int nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_stop_running, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1039,22 +1065,18 @@ __global__ __launch_bounds__(kEwThreads) void k_commit_delta(const Mol* __restri
 // layout as in k_diis_extrap.
 __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restrict__ mols, const State* __restrict__ st,
                                                           Ragged rg, int nm) {
-  RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].v * (1L + (long)mols[mol].o * mols[mol].v));
+  RAGGED_PROLOGUE((long)mols[mol].nd);
   const State& S = st[mol];
   if (e >= size_ || S.status != 0 || S.diis_count == 0) return;
   const int o = M.o, v = M.v;
   const long ne = (long)o * v;
-  long p;
+  const long p = e;
   double d;
   if (e < ne) {
-    p = e;
     d = M.t1[e];
   } else {
-    const long f = e - ne;
     int mm, nn, i, a, j, b;
-    divmod(f, (int)ne, mm, nn);
-    if (mm > nn) return;
-    p = ne + packed_index(mm, nn, ne);
+    packed_pair(e - ne, ne, mm, nn);
     divmod(mm, v, i, a);
     divmod(nn, v, j, b);
     d = M.t2[(((long)i * o + j) * v + a) * v + b];
@@ -1302,7 +1324,8 @@ __global__ __launch_bounds__(kEwThreads) void k_outer_control(const Mol* __restr
   }
   S.staged = 0;
   if (S.acc0 < prm.eps) { S.status = 1; act[mol] = 0; return; }
-  if (!(S.acc0 < prm.divergence)) { S.status = -2; act[mol] = 0; return; }
+  if (prm.check_divergence && !(S.acc0 <= prm.divergence)) { S.status = -2; act[mol] = 0; return; }
+  if (!(S.acc0 == S.acc0)) { S.status = -2; act[mol] = 0; return; }   // NaN diverges whatever the checks
   if (S.outer_its >= max_outer || S.iter + 1 > prm.maxiter) { S.status = -1; act[mol] = 0; return; }
   S.inner = 1;
 }

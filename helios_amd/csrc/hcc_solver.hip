@@ -23,6 +23,8 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
+#include <atomic>
 #include <array>
 #include <string>
 #include <vector>
@@ -531,8 +533,11 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgOV = make_ragged(b, sz_ov);
   b.rgXX = make_ragged(b, [](int, int o, int v) { return (long)v * v + (long)o * o; });
   b.rgN = make_ragged(b, [](int n, int, int) { return (long)n; });
-  // DIIS extrapolation grid: singles, then the (mm, nn) square of pair pairs (upper triangle used)
-  b.rgP2 = make_ragged(b, [](int, int o, int v) { return (long)o * v + (long)o * v * o * v; });
+  // DIIS extrapolation / shift grid: one thread per packed element (singles, then pairs mm <= nn)
+  b.rgP2 = make_ragged(b, [](int, int o, int v) {
+    const long ov = (long)o * v;
+    return ov + ov * (ov + 1) / 2;
+  });
   b.rgND = make_ragged(b, [](int, int o, int v) {
     long ov = (long)o * v;
     return ov + ov * (ov + 1) / 2;
@@ -753,6 +758,26 @@ enum ItOp {
 
 using namespace hcc;
 
+// Profile slots (hcc_profile_ops): 0..24 iteration GEMM tables, 32..56 their
+// correction forms, 60..63 set-up GEMMs, then the elementwise groups below.
+constexpr int kProfSlots = 96;
+enum ProfGroup {
+  kProfUpdate = 64,        // k_update1 + k_update2 (FP64 step, DIIS push fused)
+  kProfDiis = 65,          // k_diis_dot + k_diis_solve + k_diis_extrap (FP64)
+  kProfUpdateMixed = 66,   // k_update1_mixed + k_update2_mixed (inner step)
+  kProfDiisMixed = 67,     // DIIS over the FP32 corrections (inner)
+  kProfE2 = 68,            // k_e2 (FP64)
+  kProfE2Mixed = 69,       // k_e2_d (correction form)
+  kProfResidualEw = 70,    // dressing, Fock, P/PT, u, X init, assembly (FP64)
+  kProfResidualEwMixed = 71,
+  kProfOuterNorm = 72,     // residual norms at the outer point (mixed solver)
+  kProfCommit = 73,        // t0 += dt, DIIS shift, correction zeroing, BF16 copies
+  kProfGuess = 74,         // guess (+ learned correction)
+  kProfEnergy = 75,
+  kProfPrecomputeEw = 76,
+  kProfControl = 77,       // per-molecule iterator control kernels
+};
+
 struct hcc_ctx {
   int device = 0;
   int precision = HCC_FP64;
@@ -779,14 +804,25 @@ struct hcc_ctx {
   size_t prof_used = 0;
   double prof_ms = 0.0;
   long prof_launches = 0;
-  double prof_op_ms[64] = {0};
+  double prof_op_ms[kProfSlots] = {0};
   // learned-guess network (FP32 parameters on the device), null until set
   float* guess_net = nullptr;
   // replay blocks of iterations as captured HIP graphs (off while profiling)
   int graphs = 1;
   // measured GEMM tile / K-split choices, (op, n_occ, n_virt) -> (tile edge, ksplit)
   std::map<std::array<int, 3>, std::array<int, 2>> tuning;
+  // the reference's iteration block (src/input/definebd.f90:53-58)
+  int chk_divergence = 1, chk_stagnation = 1, enable_stop = 1;
+  double divergence = 1e3;        // divergence-threshold 3, potentiated: 10^3
+  double stagnation_pct = 60.0;   // stagnation-threshold
+  int stagnation_after = 200;     // stagnation-check-after
+  // stop request (the reference's SIGUSR1 -> chkStopIteration(2),
+  // src/modules/mod_helios.f90:441): set from any thread or a signal handler,
+  // read by the whole-solve loops between iteration blocks
+  std::atomic<int> stop_req{0};
 };
+
+static_assert(std::atomic<int>::is_always_lock_free, "hcc_request_stop must be async-signal-safe");
 
 namespace hcc {
 
@@ -804,33 +840,41 @@ static int table_parts(const hcc_ctx* c, const GemmTable& t) {
   return n;
 }
 
-static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
-  if (t.nprob == 0) return;
-  if (c->prof) {
-    c->prof_launches += table_parts(c, t);
-    if (c->prof_used == c->prof_events.size()) {
-      hipEvent_t a, b;
-      HCHECK(hipEventCreate(&a));
-      HCHECK(hipEventCreate(&b));
-      c->prof_events.emplace_back(a, b);
-      c->prof_ids.push_back(0);
-    }
-    c->prof_ids[c->prof_used] = id;
-    auto& ev = c->prof_events[c->prof_used++];
-    HCHECK(hipEventRecord(ev.first, c->stream));
-    launch_table(c, t);
-    HCHECK(hipEventRecord(ev.second, c->stream));
-    return;
+// While profiling, the launches between prof_open and prof_close are bracketed
+// by a HIP event pair on the context's stream and their time is added to
+// profile slot `id` (no-ops otherwise).
+static void prof_open(hcc_ctx* c, int id) {
+  if (!c->prof) return;
+  if (c->prof_used == c->prof_events.size()) {
+    hipEvent_t a, b;
+    HCHECK(hipEventCreate(&a));
+    HCHECK(hipEventCreate(&b));
+    c->prof_events.emplace_back(a, b);
+    c->prof_ids.push_back(0);
   }
-  launch_table(c, t);
+  c->prof_ids[c->prof_used] = id;
+  HCHECK(hipEventRecord(c->prof_events[c->prof_used].first, c->stream));
+}
+static void prof_close(hcc_ctx* c) {
+  if (!c->prof) return;
+  HCHECK(hipEventRecord(c->prof_events[c->prof_used++].second, c->stream));
 }
 
-// folds the recorded launch intervals into prof_ms (call after a synchronize)
+static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
+  if (t.nprob == 0) return;
+  if (c->prof) c->prof_launches += table_parts(c, t);
+  prof_open(c, id);
+  launch_table(c, t);
+  prof_close(c);
+}
+
+// folds the recorded intervals into the profile slots (call after a
+// synchronize); prof_ms sums the GEMM slots only
 static void prof_collect(hcc_ctx* c) {
   for (size_t q = 0; q < c->prof_used; ++q) {
     float ms = 0.f;
     HCHECK(hipEventElapsedTime(&ms, c->prof_events[q].first, c->prof_events[q].second));
-    c->prof_ms += ms;
+    if (c->prof_ids[q] < 64) c->prof_ms += ms;
     c->prof_op_ms[c->prof_ids[q]] += ms;
   }
   c->prof_used = 0;
@@ -839,35 +883,57 @@ static void prof_collect(hcc_ctx* c) {
 static Params params(const hcc_ctx* c, double eps, int maxiter, bool iterator_rules) {
   Params p{};
   p.eps = eps;
-  p.divergence = 1e3;
+  p.divergence = c->divergence;
   p.s1 = c->s1;
   p.s2 = c->s2;
   p.projtype = c->projtype;
   p.diis = c->diis;
   p.maxiter = iterator_rules ? maxiter : 0x7fffffff;
-  p.stagnation_after = 200;
-  p.stagnation_pct = 60.0;
-  p.check_divergence = iterator_rules ? 1 : 0;
-  p.check_stagnation = iterator_rules ? 1 : 0;
+  p.stagnation_after = c->stagnation_after;
+  p.stagnation_pct = c->stagnation_pct;
+  p.check_divergence = iterator_rules ? c->chk_divergence : 0;
+  p.check_stagnation = iterator_rules ? c->chk_stagnation : 0;
   return p;
+}
+
+// A pending stop request ends every running molecule with status -4 at this
+// host check (the reference's iterator tests chkStopIteration(2) once per
+// iteration, gIterator.f90:165-170; here the check runs between blocks of
+// iterations).  Returns true when the solve must stop.
+static bool take_stop(hcc_ctx* c) {
+  if (!c->enable_stop || c->stop_req.exchange(0) == 0) return false;
+  Batch& b = *c->batch;
+  stop_running(b.dst, b.dact, b.nm, c->stream);
+  HCHECK(hipGetLastError());
+  return true;
 }
 
 // Omega1 -> w1 (without the F~_ai term), Omega2 -> w2
 static void residual(hcc_ctx* c) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
+  prof_open(c, kProfResidualEw);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
   ew_fock_m(b.dm, b.dst, b.rgN, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  prof_close(c);
   gemm(c, b.it[G1_Q], G1_Q);
   gemm(c, b.it[G2A], G2A);
   gemm(c, b.it[G2B], G2B);
+  prof_open(c, kProfResidualEw);
   ew_u(b.dm, b.dst, b.rgT2x, b.nm, s);
+  prof_close(c);
   for (int g = G3; g <= G20; ++g) gemm(c, b.it[g], g);
+  prof_open(c, kProfResidualEw);
   ew_init_xx(b.dm, b.dst, b.rgXX, b.nm, s);
+  prof_close(c);
   for (int g = G21; g <= G24; ++g) gemm(c, b.it[g], g);
+  prof_open(c, kProfE2);
   ew_e2(b.dm, b.dst, b.rgOcc, b.nm, s);
+  prof_close(c);
+  prof_open(c, kProfResidualEw);
   ew_assemble(b.dm, b.dst, b.rgT2x, b.nm, s);
+  prof_close(c);
   HCHECK(hipGetLastError());
 }
 
@@ -875,52 +941,60 @@ static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
   residual(c);
+  prof_open(c, kProfUpdate);
   ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm, b.part1, b.part2, s);
+  prof_close(c);
+  prof_open(c, kProfControl);
   ctl_after_update(b.dm, b.dst, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
-  if (prm.diis) diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
+  prof_close(c);
+  if (prm.diis) {
+    prof_open(c, kProfDiis);
+    diis_step(b.dm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
+    prof_close(c);
+  }
+  prof_open(c, kProfControl);
   ctl_after_diis(b.dm, b.dst, b.dact, b.nm, prm, stop, s);
+  prof_close(c);
   HCHECK(hipGetLastError());
 }
 
 static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
   if (t.nprob == 0) return;
-  hipEvent_t* ev = nullptr;
-  if (c->prof) {
-    c->prof_launches += table_parts(c, t);
-    if (c->prof_used == c->prof_events.size()) {
-      hipEvent_t a, b2;
-      HCHECK(hipEventCreate(&a));
-      HCHECK(hipEventCreate(&b2));
-      c->prof_events.emplace_back(a, b2);
-      c->prof_ids.push_back(0);
-    }
-    c->prof_ids[c->prof_used] = id;
-    ev = &c->prof_events[c->prof_used++].first;
-    HCHECK(hipEventRecord(ev[0], c->stream));
-  }
+  if (c->prof) c->prof_launches += table_parts(c, t);
+  prof_open(c, id);
   for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr) {
     HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16,
                             p->tile));
     if (p->red_blocks > 0)
       HCHECK(launch_splitk_reduce(p->d, p->red_prefix, p->red_bprob, p->red_blocks, c->batch->dact, c->stream));
   }
-  if (ev) HCHECK(hipEventRecord(c->prof_events[c->prof_used - 1].second, c->stream));
+  prof_close(c);
 }
 
 // correction residual dOmega at (t0 + dt): every step in correction form
 static void residual_delta(hcc_ctx* c) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
+  prof_open(c, kProfResidualEwMixed);
   mixed_dress(b.dm, b.ddm, b.dst, b.rgNN, b.rgN, b.rgN3, b.nm, s);
+  prof_close(c);
   dgemm(c, b.dit[G1_Q], 32 + G1_Q);
   dgemm(c, b.dit[G2A], 32 + G2A);
   dgemm(c, b.dit[G2B], 32 + G2B);
+  prof_open(c, kProfResidualEwMixed);
   ew_u(b.ddm, b.dst, b.rgT2x, b.nm, s);
+  prof_close(c);
   for (int g = G3; g <= G20; ++g) dgemm(c, b.dit[g], 32 + g);
+  prof_open(c, kProfResidualEwMixed);
   ew_init_xx(b.ddm, b.dst, b.rgXX, b.nm, s);
+  prof_close(c);
   for (int g = G21; g <= G24; ++g) dgemm(c, b.dit[g], 32 + g);
+  prof_open(c, kProfE2Mixed);
   mixed_e2(b.dm, b.ddm, b.dst, b.rgOcc, b.nm, s);
+  prof_close(c);
+  prof_open(c, kProfResidualEwMixed);
   ew_assemble(b.ddm, b.dst, b.rgT2x, b.nm, s);
+  prof_close(c);
   HCHECK(hipGetLastError());
 }
 
@@ -928,28 +1002,36 @@ static void residual_delta(hcc_ctx* c) {
 static void precompute(hcc_ctx* c) {
   Batch& b = *c->batch;
   hipStream_t s = c->stream;
+  prof_open(c, kProfPrecomputeEw);
   reset_state(b.dst, b.dact, b.nm, s);
   // t1 = 0 -> undressed orbital products
   zero_t1(b.dm, b.rgOV, b.nm, s);
+  prof_close(c);
   gemm(c, b.pre[0], 60);
   gemm(c, b.pre[1], 61);
+  prof_open(c, kProfPrecomputeEw);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  prof_close(c);
   gemm(c, b.it[G1_Q], 62);
   gemm(c, b.pre[2], 63);
+  prof_open(c, kProfPrecomputeEw);
   make_L(b.dm, b.rgT2, b.nm, s);
+  prof_close(c);
   HCHECK(hipGetLastError());
 }
 
 // initial amplitudes of every molecule; the learned guess corrects the MP2 one
 static void apply_guess(hcc_ctx* c, int kind) {
   Batch& b = *c->batch;
+  prof_open(c, kProfGuess);
   guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind == HCC_GUESS_LEARNED ? HCC_GUESS_MP2 : kind, c->stream);
   if (kind == HCC_GUESS_LEARNED) {
     mlp_overlaps(b.dm, b.rgNN, b.nm, c->stream);
     mlp_guess(b.dm, b.rgPairL, b.nm, c->guess_net, c->stream);
     HCHECK(hipGetLastError());
   }
+  prof_close(c);
 }
 
 bool batch_view(hcc_ctx* c, BatchView* v) {
@@ -964,12 +1046,17 @@ bool batch_view(hcc_ctx* c, BatchView* v) {
   return true;
 }
 
+// Builds the new batch privately and publishes it in c->batch only when the
+// validation, the allocations, the tables and the upload have all succeeded:
+// a rejected load leaves the context with no batch (HCC_E_STATE afterwards),
+// never with a half-built one.
 static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const double* C, const double* F,
                  const double* G, const double* eref, bool keep_mp2) {
   delete c->batch;
   c->batch = nullptr;
-  Batch* b = new Batch();
-  c->batch = b;
+  c->have_guess = false;
+  std::unique_ptr<Batch> owner(new Batch());
+  Batch* b = owner.get();
   b->nm = nmol;
   b->m = c->diis_steps;
   b->mixed = c->precision == HCC_MIXED;
@@ -1016,7 +1103,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
     }
     HCHECK(hipMemcpy(base, h.data(), (size_t)span * sizeof(double), hipMemcpyHostToDevice));
   }
-  c->have_guess = false;
+  c->batch = owner.release();
 }
 
 }  // namespace hcc
@@ -1071,6 +1158,29 @@ int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection
   ctx->s2 = t2_step;
   ctx->projtype = projection_type;
   ctx->diis = diis_enabled ? 1 : 0;
+  return HCC_OK;
+}
+
+int hcc_set_iterator(hcc_ctx* ctx, int check_divergence, double divergence_threshold, int check_stagnation,
+                     double stagnation_threshold, int stagnation_after, int enable_stop) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (!(divergence_threshold > 0.0) || !(stagnation_threshold >= 0.0 && stagnation_threshold <= 100.0) ||
+      stagnation_after < 0) {
+    ctx->err = "invalid iterator options";
+    return HCC_E_ARG;
+  }
+  ctx->chk_divergence = check_divergence ? 1 : 0;
+  ctx->divergence = divergence_threshold;
+  ctx->chk_stagnation = check_stagnation ? 1 : 0;
+  ctx->stagnation_pct = stagnation_threshold;
+  ctx->stagnation_after = stagnation_after;
+  ctx->enable_stop = enable_stop ? 1 : 0;
+  return HCC_OK;
+}
+
+int hcc_request_stop(hcc_ctx* ctx) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  ctx->stop_req.store(1);
   return HCC_OK;
 }
 
@@ -1338,6 +1448,7 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   const int check_every = 4;
   int it = 0;
   double iters_total = 0.0;
+  ctx->stop_req.store(0);   // each iterator run starts unflagged (gIterator.f90:57)
   // blocks of check_every iterations replay one captured graph: every kernel
   // reads its molecule's state on the device, so the launches never change
   GraphExec block;
@@ -1358,9 +1469,11 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
     if (ctx->prof) prof_collect(ctx);
     int alive = 0;
     for (int a : act) alive += a;
-    if (alive == 0) break;
+    if (alive == 0 || take_stop(ctx)) break;
   }
+  prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
+  prof_close(ctx);
   HCHECK(hipEventRecord(e1, s));
   HCHECK(hipEventSynchronize(e1));
   if (ctx->prof) prof_collect(ctx);
@@ -1413,21 +1526,33 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   prm0.s1 = prm0.s2 = 0.0;
   prm0.diis = 0;
   std::vector<int> act(b.nm);
+  ctx->stop_req.store(0);   // each iterator run starts unflagged (gIterator.f90:57)
+  bool stopped = false;
   auto alive = [&]() {
     HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
     HCHECK(hipStreamSynchronize(s));
     if (ctx->prof) prof_collect(ctx);
     int n = 0;
     for (int a : act) n += a;
+    if (n > 0 && take_stop(ctx)) {
+      stopped = true;
+      return 0;
+    }
     return n;
   };
   auto inner_iteration = [&]() {
     residual_delta(ctx);
+    prof_open(ctx, kProfUpdateMixed);
     mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm, b.part1, b.part2, s);
+    prof_close(ctx);
+    prof_open(ctx, kProfControl);
     mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
+    prof_close(ctx);
     if (prm.diis) {
+      prof_open(ctx, kProfDiisMixed);
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
+      prof_close(ctx);
     }
   };
   // The first step of every inner solve starts from dt = 0, whose correction
@@ -1436,11 +1561,17 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   Params prm1 = prm;
   prm1.outer_only = 1;
   auto first_step = [&]() {
+    prof_open(ctx, kProfUpdateMixed);
     mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm1, b.part1, b.part2, s);
+    prof_close(ctx);
+    prof_open(ctx, kProfControl);
     mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2x, b.part2, s);
+    prof_close(ctx);
     if (prm1.diis) {
+      prof_open(ctx, kProfDiisMixed);
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
       mixed_inner_advance(b.dm, b.dst, b.nm, s);
+      prof_close(ctx);
     }
   };
   // blocks of kInnerBlock inner iterations replay one captured graph (see
@@ -1452,12 +1583,16 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
     set_phase(b.dst, b.dact, b.nm, 0, s);
     residual(ctx);
+    prof_open(ctx, kProfOuterNorm);
     ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm0, b.part1, b.part2, s);
     mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2x, b.part2, s);
-    if (alive() == 0) break;
+    prof_close(ctx);
+    if (alive() == 0 || stopped) break;
+    prof_open(ctx, kProfCommit);
     set_phase(b.dst, b.dact, b.nm, 1, s);
     if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2Sx, b.nm, s);
+    prof_close(ctx);
     first_step();
     for (int inner = 1; inner < ctx->max_inner;) {
       if (graphs && inner + kInnerBlock <= ctx->max_inner) {
@@ -1471,10 +1606,15 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       }
       if (alive() == 0) break;
     }
+    if (stopped) break;
+    prof_open(ctx, kProfCommit);
     mixed_commit(b.dm, b.ddm, b.dst, b.rgT2Sx, b.rgP2, b.nm, prm.keep_diis && prm.diis, s);
+    prof_close(ctx);
   }
   set_phase(b.dst, b.dact, b.nm, 0, s);
+  prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
+  prof_close(ctx);
   HCHECK(hipEventRecord(e1, s));
   HCHECK(hipEventSynchronize(e1));
   if (ctx->prof) prof_collect(ctx);
@@ -1560,9 +1700,9 @@ int hcc_profile(hcc_ctx* ctx, int enable) {
   return HCC_OK;
 }
 
-int hcc_profile_ops(hcc_ctx* ctx, double* ms64) {
-  if (ctx == nullptr || ms64 == nullptr) return HCC_E_ARG;
-  for (int k = 0; k < 64; ++k) ms64[k] = ctx->prof_op_ms[k];
+int hcc_profile_ops(hcc_ctx* ctx, double* ms96) {
+  if (ctx == nullptr || ms96 == nullptr) return HCC_E_ARG;
+  for (int k = 0; k < kProfSlots; ++k) ms96[k] = ctx->prof_op_ms[k];
   return HCC_OK;
 }
 

@@ -57,34 +57,37 @@ def _prep(k):
     return dict(C=p["C"], F=p["F"], G=p["G"], nocc=p["nocc"], eref=p["eref"], n=m.n)
 
 
-def collect(trainer, count, chunk=2048, per_molecule=256, eps=1e-9, workers=16, log=print):
+def collect(trainer, count, pool, chunk=2048, per_molecule=256, eps=1e-9, log=print):
     """Solve `count` training molecules in chunks with the FP64 batched solver
-    and append per_molecule samples of each converged one to the trainer."""
+    and append per_molecule samples of each converged one to the trainer.
+    `pool` is a multiprocessing pool created before this process touched the
+    GPU (forking after the HIP runtime is up is not supported)."""
     t0 = time.time()
     solved = 0
-    with mp.Pool(workers) as pool:
-        for c0 in range(0, count, chunk):
-            ks = range(c0, min(count, c0 + chunk))
-            mols = pool.map(_prep, ks, chunksize=16)
-            ctx = capi.Context(0, capi.HCC_FP64, 8)
-            ctx.batch_load(mols)
-            ctx.batch_solve(capi.GUESS_MP2, 500, eps)
-            _, it, st = ctx.batch_results()
-            trainer.collect(ctx, per_molecule, seed=c0)
-            ctx.close()
-            solved += int((st == 1).sum())
-            log(f"# training molecules {c0 + len(ks)}/{count}: converged {int((st == 1).sum())}/{len(ks)}, "
-                f"mean iterations {it.mean():.1f}, {time.time() - t0:.1f} s")
+    for c0 in range(0, count, chunk):
+        ks = range(c0, min(count, c0 + chunk))
+        mols = pool.map(_prep, ks, chunksize=16)
+        ctx = capi.Context(0, capi.HCC_FP64, 8)
+        ctx.batch_load(mols)
+        ctx.batch_solve(capi.GUESS_MP2, 500, eps)
+        _, it, st = ctx.batch_results()
+        trainer.collect(ctx, per_molecule, seed=c0)
+        ctx.close()
+        solved += int((st == 1).sum())
+        log(f"# training molecules {c0 + len(ks)}/{count}: converged {int((st == 1).sum())}/{len(ks)}, "
+            f"mean iterations {it.mean():.1f}, {time.time() - t0:.1f} s")
     return solved
 
 
 def train(count=20000, per_molecule=256, epochs=60, batch=4096, lr=2e-3, lr_final=5e-5, seed=0,
           out=PARAMS_PATH, workers=16, log=print):
-    trainer = capi.Trainer(count * per_molecule)
-    trainer.set_params(initial_params(seed))
-    t0 = time.time()
-    solved = collect(trainer, count, per_molecule=per_molecule, workers=workers, log=log)
-    t_collect = time.time() - t0
+    # the workers fork here, before the trainer initialises the GPU
+    with mp.Pool(workers) as pool:
+        trainer = capi.Trainer(count * per_molecule)
+        trainer.set_params(initial_params(seed))
+        t0 = time.time()
+        solved = collect(trainer, count, pool, per_molecule=per_molecule, log=log)
+        t_collect = time.time() - t0
     loss0 = trainer.evaluate()
     t1 = time.time()
     hist = trainer.train(epochs, batch, lr, lr_final, seed)

@@ -53,6 +53,26 @@ int hcc_create(hcc_ctx** ctx, int device, int precision, int diis_steps);
 int hcc_set_options(hcc_ctx* ctx, double t1_step, double t2_step, int projection_type,
                     int diis_enabled);
 
+/* The reference's iteration block (src/input/definebd.f90:53-58, read by
+ * iterator, src/calculation/gIterator.f90:165-206) for hcc_solve and
+ * hcc_batch_solve: enable-divergence-check with divergence-threshold (the
+ * potentiated value, default 1e3), enable-stagnation-check with
+ * stagnation-threshold (percent of improving iterations, default 60) and
+ * stagnation-check-after (default 200), enable-stop-iteration (default 1).
+ * A NaN residual ends a solve with status -2 whatever the checks. */
+int hcc_set_iterator(hcc_ctx* ctx, int check_divergence, double divergence_threshold, int check_stagnation,
+                     double stagnation_threshold, int stagnation_after, int enable_stop);
+
+/* Stop request, the counterpart of the reference's SIGUSR1 handler setting
+ * chkStopIteration(2) (src/modules/mod_helios.f90:405-452): the running
+ * hcc_solve / hcc_batch_solve ends every molecule still iterating with status
+ * -4 at its next host check (between blocks of at most 4 iterations) and
+ * returns energies of the amplitudes reached.  Async-signal-safe (it stores
+ * one lock-free atomic) and callable from any thread.  Each solve starts with
+ * the request cleared, as the reference's iterator does (gIterator.f90:57);
+ * with enable-stop-iteration off the request is ignored. */
+int hcc_request_stop(hcc_ctx* ctx);
+
 /* Options of the mixed-precision solver (precision HCC_MIXED; no reference
  * counterpart -- the reference is FP64 only).  The solve is a defect
  * correction: an FP64 residual at the outer point t0 decides convergence;
@@ -150,10 +170,16 @@ int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup)
 /* Matrix-core flops of one inner iteration of the mixed solver per loaded
  * molecule (HCC_MIXED contexts; 0 otherwise): 2*M*N*K per correction product. */
 int hcc_gemm_flops_correction(hcc_ctx* ctx, double* per_inner_iteration);
-/* Per-contraction split of the profiled GEMM time: ms64[k] for GEMM table k
- * of the iteration (0..N-1 in launch order), 32+k for its correction form
- * (mixed solver) and 60..63 for the set-up GEMMs. */
-int hcc_profile_ops(hcc_ctx* ctx, double* ms64);
+/* Split of the profiled device time into 96 slots: ms96[k] for GEMM table k
+ * of the iteration (0..24 in launch order), 32+k for its correction form
+ * (mixed solver), 60..63 for the set-up GEMMs; then elementwise groups:
+ * 64 FP64 update (k_update1/2 with the DIIS push), 65 FP64 DIIS (dots, solve,
+ * extrapolation), 66 mixed inner update, 67 mixed inner DIIS, 68 E2 pass,
+ * 69 its correction form, 70 residual elementwise (dressing, Fock, P/PT, u,
+ * assembly), 71 its correction form, 72 outer-point residual norms, 73 commit
+ * and BF16 copies, 74 guess, 75 energy, 76 set-up elementwise, 77 iterator
+ * control kernels. */
+int hcc_profile_ops(hcc_ctx* ctx, double* ms96);
 
 /* Iteration blocks (4 iterations of hcc_solve / hcc_batch_solve, 4 inner
  * iterations of the mixed solver) are captured once per solve as HIP graphs

@@ -334,6 +334,7 @@ class DIIS:
             M[:it, :it] /= np.max(np.diag(M)[:it])
         lam, V = np.linalg.eigh(M)
         c = (V[:it, :] * V[it, :][None, :] / lam[None, :]).sum(1)
+        self.coef = c          # newest vector first, as stored
         t1 = (self.st1[:, :it] @ c).reshape(self.o, self.v)
         t2 = self.unpack(self.st2[:, :it] @ c)
         return t1, t2

@@ -1,0 +1,195 @@
+"""Parity of the HIP solver with the oracle at the workloads' full sizes.
+
+* Residuals at random amplitudes for N = 24, 36, 48 (o = v = 12, 18, 24: the
+  multi-tile, split-K and 128x32 tile kinds of the batch's larger molecules),
+  once with the cost heuristic and once with the measured tile table, against
+  the reference-form projections of ``oracle.ccsd`` (1e-11 relative).
+* One batch molecule per even N from 10 to 48, solved together by the batched
+  FP64 solver, against ``oracle.ccsd.solve`` (energy 1e-8 eV, T2 1e-7).
+* The 150-site honeycomb flake (o = v = 75: the E2 terms run as GEMMs):
+  residuals at random amplitudes against the GEMM-form oracle
+  ``oracle.dressed`` (itself pinned against ``oracle.ccsd`` at small N).
+* The 200-site polyene of the plain-FP64 workload (SCF 1e-10, CCSD 1e-8): the
+  GPU's converged amplitudes satisfy the oracle's equations (max |d| of the
+  oracle residual there) and the oracle's energy expression gives the GPU's
+  energy; the N^8 oracle solve itself is out of reach at this size.
+* The mixed-precision workload's two molecules (the polyene and the flake):
+  mixed minus FP64 energy below 1e-6 eV for each, printed per molecule.
+* The 4096-molecule batch, mixed precision with the learned guess: every
+  solve converges, within 1e-6 eV of a tight FP64 solve of the same batch.
+"""
+import multiprocessing as mp
+
+import numpy as np
+import pytest
+
+from helios_amd import autotune, capi, learned_guess, molecules as M, ppp
+from oracle import ccsd, dressed
+from tests._molprep import prepare, random_amplitudes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("tuned", [False, True])
+@pytest.mark.parametrize("n", [24, 36, 48])
+def test_residual_parity_larger_molecules(n, tuned):
+    p = prepare(M.random_fragment(n, seed=100 + n))
+    o, v = p["nocc"], n - p["nocc"]
+    ctx = capi.Context(0, capi.HCC_FP64, 20)
+    if tuned:
+        ctx.set_gemm_tuning(autotune.load())
+    ctx.set_molecule(p["C"], p["F"], p["G"], o, p["eref"])
+    B = ccsd.Blocks(ccsd.rint(p["C"], p["G"]), o)
+    for seed in (0, 1):
+        t1, t2 = random_amplitudes(o, v, seed)
+        ctx.set_amplitudes(t1, t2)
+        d1, d2 = ctx.residual()
+        r1 = ccsd.singles(p["F"], B, t1, t2)
+        r2 = ccsd.doubles(p["F"], B, t1, t2)
+        s1, s2 = max(1.0, np.abs(r1).max()), max(1.0, np.abs(r2).max())
+        say(n, tuned, seed, np.abs(d1 - r1).max() / s1, np.abs(d2 - r2).max() / s2)
+        assert np.abs(d1 - r1).max() < 1e-11 * s1
+        assert np.abs(d2 - r2).max() < 1e-11 * s2
+    ctx.close()
+
+
+SIZES = list(range(10, 50, 2))
+
+
+def _first_member_of_size(n, limit=400):
+    for k in range(limit):
+        m = M.batch_member(k)
+        if m.n == n:
+            return k, m
+    raise RuntimeError(f"no batch member with {n} sites among the first {limit}")
+
+
+@pytest.fixture(scope="module")
+def per_size_batch():
+    """One batch member per even N, solved together (FP64, batched)."""
+    picked = [_first_member_of_size(n) for n in SIZES]
+    ps = [prepare(m) for _, m in picked]
+    ctx = capi.Context(0, capi.HCC_FP64, 20)
+    ctx.set_gemm_tuning(autotune.load())
+    ctx.batch_load(ps)
+    ctx.batch_solve(capi.GUESS_REFERENCE, 2000, 1e-10)
+    e, it, st = ctx.batch_results()
+    ctx.close()
+    return dict(ps=ps, ks=[k for k, _ in picked], e=e, it=it, st=st)
+
+
+@pytest.mark.parametrize("idx", range(len(SIZES)), ids=[f"n{n}" for n in SIZES])
+def test_solve_parity_each_size(per_size_batch, idx):
+    b = per_size_batch
+    p = b["ps"][idx]
+    ref = ccsd.solve(p["C"], p["F"], p["G"], p["nocc"], p["eref"], eps=1e-10, diis_scale=True)
+    say(f"n={p['n']} member {b['ks'][idx]}: GPU E {b['e'][idx, 0]:.12f} ({b['it'][idx]} it), "
+        f"oracle {ref['energy'][0]:.12f} ({ref['iterations']} it)")
+    assert ref["status"] == 1 and b["st"][idx] == 1
+    assert abs(b["e"][idx, 0] - ref["energy"][0]) < 1e-8
+
+
+def test_flake150_residual_parity():
+    """o = v = 75 > 32: the E2 contractions take the GEMM path (g23, g24)."""
+    mol = M.honeycomb_flake(150)
+    p = ppp.prepare(mol)
+    o, v = p["nocc"], mol.n - p["nocc"]
+    t1, t2 = random_amplitudes(o, v, 3, s1=0.02, s2=0.02)
+    ctx = capi.Context(0, capi.HCC_FP64, 8)
+    ctx.set_molecule(p["C"], p["F"], p["G"], o, p["eref"])
+    ctx.set_amplitudes(t1, t2)
+    d1, d2 = ctx.residual()
+    ctx.close()
+    say("flake: GPU residual done, oracle running")
+    w1, w2 = dressed.omega(p["C"], p["F"], p["G"], o, t1, t2)
+    r2 = dressed.contravariant(w2)
+    s1, s2 = max(1.0, np.abs(w1).max()), max(1.0, np.abs(r2).max())
+    say("flake150", np.abs(d1 - w1).max() / s1, np.abs(d2 - r2).max() / s2)
+    assert np.abs(d1 - w1).max() < 1e-11 * s1
+    assert np.abs(d2 - r2).max() < 1e-11 * s2
+
+
+def _oracle_energy(p, t1, t2):
+    """energyCC (mod_cc.f90:1916-1952) from the undressed (ia|jb) alone:
+    E = eref + 2 sum F_ia t_ia + sum_ijab L_iajb (t_ij^ab + t_ia t_jb)."""
+    o = p["nocc"]
+    X, Y = dressed.dress(p["C"], o, np.zeros_like(t1))
+    g = dressed.gt(X, Y, p["G"], "ovov")                    # (ia|jb)
+    L = 2 * g - g.transpose(0, 3, 2, 1)
+    tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
+    return p["eref"] + 2 * float((p["F"][:o, o:] * t1).sum()) + float(np.einsum("iajb,ijab->", L, tau))
+
+
+@pytest.mark.timeout(600)
+def test_polyene200_fp64_fixed_point():
+    mol = M.polyene(200)
+    p = ppp.prepare(mol)          # SCF to a commutator of 1e-11 (tighter than the workload's 1e-10)
+    o = p["nocc"]
+    ctx = capi.Context(0, capi.HCC_FP64, 20)
+    ctx.set_molecule(p["C"], p["F"], p["G"], o, p["eref"])
+    e, it, st = ctx.solve(capi.GUESS_REFERENCE, 200, 1e-8)
+    t1, t2 = ctx.amplitudes()
+    ctx.close()
+    say(f"polyene200 FP64: status {st}, {it} iterations, E {e[0]:.10f}; oracle residual running")
+    assert st == 1
+    w1, w2 = dressed.omega(p["C"], p["F"], p["G"], o, t1, t2)
+    r1, r2 = np.abs(w1).max(), np.abs(dressed.contravariant(w2)).max()
+    eo = _oracle_energy(p, t1, t2)
+    say(f"polyene200: oracle max|d1| {r1:.2e} max|d2| {r2:.2e}; E GPU - oracle {e[0] - eo:.2e}")
+    assert max(r1, r2) < 2e-8
+    assert abs(e[0] - eo) < 1e-8
+
+
+@pytest.mark.timeout(600)
+def test_mixed_precision_large_molecules_per_molecule_error():
+    mols = [M.polyene(200), M.honeycomb_flake(150)]
+    for mol in mols:
+        p = ppp.prepare(mol)
+        out = {}
+        for prec in (capi.HCC_FP64, capi.HCC_MIXED):
+            ctx = capi.Context(0, prec, 20)
+            ctx.set_molecule(p["C"], p["F"], p["G"], p["nocc"], p["eref"])
+            out[prec] = ctx.solve(capi.GUESS_MP2, 300, 1e-8)
+            ctx.close()
+        e64, it64, st64 = out[capi.HCC_FP64]
+        emx, itmx, stmx = out[capi.HCC_MIXED]
+        say(f"{mol.name}: FP64 E {e64[0]:.10f} ({it64} it), mixed E {emx[0]:.10f} ({itmx} steps), "
+            f"mixed - FP64 {emx[0] - e64[0]:.3e} eV")
+        assert st64 == 1 and stmx == 1
+        assert abs(emx[0] - e64[0]) < 1e-6
+
+
+def _prep(k):
+    p = ppp.prepare(M.batch_member(k))
+    return dict(C=p["C"], F=p["F"], G=p["G"], nocc=p["nocc"], eref=p["eref"])
+
+
+@pytest.mark.timeout(600)
+def test_batch4096_mixed_learned_all_converged():
+    # spawned (not forked) workers: this process has initialised the GPU
+    with mp.get_context("spawn").Pool(16) as pool:
+        mols = pool.map(_prep, range(4096), chunksize=32)
+    say("batch prepared")
+    ref = capi.Context(0, capi.HCC_FP64, 20)
+    ref.batch_load(mols)
+    ref.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+    e64, _, st64 = ref.batch_results()
+    ref.close()
+    assert (st64 == 1).all()
+    ctx = capi.Context(0, capi.HCC_MIXED, 20)
+    ctx.set_gemm_tuning(autotune.load())
+    ctx.set_guess_network(learned_guess.load_params())
+    ctx.batch_load(mols)
+    ctx.batch_solve(capi.GUESS_LEARNED, 2000, 2e-7)
+    e, it, st = ctx.batch_results()
+    outer, inner = ctx.batch_statistics()
+    ctx.close()
+    de = np.abs(e[:, 0] - e64[:, 0])
+    say(f"4096 mixed+learned: converged {(st == 1).sum()}, max|dE| {de.max():.3e} eV, "
+        f"outer {outer.mean():.2f} inner {inner.mean():.2f}")
+    assert (st == 1).all()
+    assert de.max() < 1e-6

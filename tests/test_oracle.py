@@ -210,3 +210,74 @@ def test_diis_scaling_keeps_coefficients():
         a.iterate(it, 0.0)
         b.iterate(it, 0.0)
     assert np.abs(a.t2 - b.t2).max() < 1e-10
+
+
+def _constrained_min_residual(R):
+    """argmin_c ||R c|| subject to sum(c) = 1, by least squares over the
+    null space of the constraint: c = e_0 + N y with N's columns e_j - e_0."""
+    k = R.shape[1]
+    if k == 1:
+        return np.ones(1)
+    N = np.zeros((k, k - 1))
+    N[0, :] = -1.0
+    N[1:, :] = np.eye(k - 1)
+    y, *_ = np.linalg.lstsq(R @ N, -R[:, 0], rcond=None)
+    return np.eye(k)[0] + N @ y
+
+
+@pytest.mark.parametrize("scale", [False, True])
+def test_diis_is_constrained_minimal_residual(scale):
+    """DIIS (mod_cc.f90:1390-1423 push, 1514-1555 extrapolation) on a linear
+    fixed-point problem r(x) = b - A x with Richardson steps x~ = x + r: the
+    extrapolated vector must be sum_k c_k x~_k with c the constrained minimiser
+    of ||sum_k c_k r_k|| over the last nd residuals, computed here without the
+    bordered system.  Eight iterations on a 4-deep history also exercise the
+    shift of B once the history is full."""
+    o, v, nd = 2, 3, 4
+    dg = ccsd.DIIS(o, v, nd, scale=scale)
+    ne = o * v
+    dim = ne + dg.np2
+    rng = np.random.default_rng(11)
+    A = np.eye(dim) + 0.3 * rng.standard_normal((dim, dim)) / np.sqrt(dim)
+    b = rng.standard_normal(dim)
+    x = rng.standard_normal(dim)
+    split = lambda z: (z[:ne].reshape(o, v), dg.unpack(z[ne:]))   # noqa: E731
+    hist_r, hist_x = [], []
+    for it in range(1, 9):
+        r = b - A @ x
+        xt = x + r
+        t1, t2 = split(xt)
+        d1, d2 = split(r)
+        dg.push(t1, t2, d1, d2, projtype=1)
+        hist_r.insert(0, r)
+        hist_x.insert(0, xt)
+        res = dg.extrapolate(min(it, nd))
+        if it == 1:
+            assert res is None
+            x = xt
+            continue
+        k = min(it, nd)
+        R = np.array(hist_r[:k]).T
+        c = _constrained_min_residual(R)
+        assert np.abs(dg.coef - c).max() < 1e-9 * max(1.0, np.abs(c).max())
+        xn = np.array(hist_x[:k]).T @ c
+        assert np.abs(res[0].ravel() - xn[:ne]).max() < 1e-9
+        assert np.abs(dg.pack(res[1]) - xn[ne:]).max() < 1e-9
+        # the extrapolated residual is the smallest in the affine hull
+        assert np.linalg.norm(R @ dg.coef) <= np.linalg.norm(hist_r[0]) + 1e-12
+        x = xn
+
+
+def test_diis_error_vector_is_the_update_direction():
+    """pushCCVectors stores, for r-ccsd with the default projection, the same
+    doubles vector that stepCC divides by the denominators (mod_cc.f90:1007-1010,
+    1418-1420): (t2_new - t2_old) D2 / s2, packed."""
+    p = prepare(M.polyene(6))
+    cc = ccsd.CCSD(p["C"], p["F"], p["G"], 3, p["eref"], s2=0.7)
+    D1, D2 = ccsd.denominators(p["F"], 3)
+    t2_old = cc.t2.copy()
+    t1_old = cc.t1.copy()
+    cc.iterate(1, 0.0)
+    dg = cc.diis
+    assert np.abs(dg.sd2[:, 0] - dg.pack((cc.t2 - t2_old) * D2 / 0.7)).max() < 1e-10
+    assert np.abs(dg.sd1[:, 0] - ((cc.t1 - t1_old) * D1).ravel()).max() < 1e-10
