@@ -96,32 +96,63 @@ def copy_calibration():
     print(f"copied {1 << 30} bytes x 3", flush=True)
 
 
-def cpu_baseline(mols, budget_s):
-    """The oracle (numpy restatement of the reference equations) on the first
-    molecules of the batch, in batch order, until budget_s of CPU work."""
-    try:
-        from threadpoolctl import threadpool_info, threadpool_limits
-    except ImportError:      # pragma: no cover
-        threadpool_limits = None
-    from oracle import ccsd
-    cores = os.cpu_count() or 1
-    if threadpool_limits is not None:
-        ctx = threadpool_limits(limits=cores)
-        info = threadpool_info()
-        cores = max([i.get("num_threads", 1) for i in info] or [1])
-    t0 = time.perf_counter()
-    done = []
-    for p in mols:
-        ccsd.solve(p["C"], p["F"], p["G"], p["nocc"], p["eref"], eps=1e-6, diis_scale=True)
-        done.append(p["n"])
-        if time.perf_counter() - t0 > budget_s:
+def cpu_baseline(mols, iters_fp64, threads_all):
+    """The reference CPU path (oracle/refloops.c: the reference's loop nests and
+    OpenMP pattern, O(o^4 v^4) per iteration) on the host cores, as
+    BASELINE.md plans it: on one batch member per even N from 10 to 24, the R
+    build and one iteration are timed with 1 thread and with every host
+    thread (and N = 10, 12 solved in full, whose time the parts must predict);
+    every molecule of the batch is then costed as build + (its FP64 GPU
+    iteration count) x iteration, with the o^8 fit of the largest sampled
+    sizes beyond N = 24 -- an extrapolation, labelled as such."""
+    from oracle import refloops
+    sample_n = list(range(10, 26, 2))
+    reps = {}
+    for k, m in enumerate(mols):
+        if m["n"] in sample_n and m["n"] not in reps:
+            reps[m["n"]] = k
+        if len(reps) == len(sample_n):
             break
-    dt = time.perf_counter() - t0
-    return {"value": len(done) / dt, "unit": "solves/s", "cores": int(cores), "kind": "port",
-            "sample": (f"first {len(done)} molecules of the batch in batch order (N = "
-                       f"{min(done)}..{max(done)}, mean {np.mean(done):.1f}; batch mean "
-                       f"{np.mean([m['n'] for m in mols]):.1f}), numpy oracle restating the reference "
-                       f"r-ccsd equations with scaled DIIS, residual threshold 1e-6, {dt:.1f} s")}
+    out = {}
+    t_cpu = 0.0
+    check = {}
+    for th in (1, threads_all):
+        parts = {}
+        for n in sorted(reps):
+            p = mols[reps[n]]
+            tb, ti = refloops.time_parts(p["C"], p["F"], p["G"], p["nocc"], iters=1, threads=th)
+            parts[n] = (tb, ti)
+            t_cpu += (tb + ti) * th
+        for n in (10, 12):
+            p = mols[reps[n]]
+            t0 = time.perf_counter()
+            r = refloops.solve(p["C"], p["F"], p["G"], p["nocc"], p["eref"], eps=2e-7, threads=th)
+            dt = time.perf_counter() - t0
+            t_cpu += dt * th
+            pred = parts[n][0] + r["iterations"] * parts[n][1]
+            check[(th, n)] = (dt, pred, r["iterations"])
+        big = [n for n in sorted(parts)][-3:]
+        c_it = np.mean([parts[n][1] / (n // 2) ** 8 for n in big])
+        c_b = np.mean([parts[n][0] / n ** 6 for n in big])
+        total = 0.0
+        for k, m in enumerate(mols):
+            n = m["n"]
+            tb, ti = parts[n] if n in parts else (c_b * n ** 6, c_it * (n // 2) ** 8)
+            total += tb + float(iters_fp64[k]) * ti
+        out[th] = (len(mols) / total, total, parts)
+    v_all, tot_all, parts_all = out[threads_all]
+    v_one, tot_one, _ = out[1]
+    return {"value": v_all, "unit": "solves/s", "cores": int(threads_all), "kind": "port",
+            "value_1_thread": v_one,
+            "batch_seconds_extrapolated": tot_all, "batch_seconds_extrapolated_1_thread": tot_one,
+            "full_solve_check": {f"n{n}_threads{th}": {"measured_s": round(a, 4), "predicted_s": round(b, 4),
+                                                      "iterations": it}
+                                 for (th, n), (a, b, it) in check.items()},
+            "sample": (f"C restatement of the reference loop nests (oracle/refloops.c, gcc -O3 -fopenmp); "
+                       f"R build + 1 iteration timed on one batch member per N in {sorted(reps)} with 1 and "
+                       f"{threads_all} threads, full solves at N = 10, 12 to check the parts; the 4096 batch "
+                       f"costed per molecule with the FP64 GPU solver's iteration counts and an o^8 fit "
+                       f"above N = 24 (extrapolation); {t_cpu:.1f} thread-seconds of CPU work")}
 
 
 def main():
@@ -134,7 +165,6 @@ def main():
     ap.add_argument("--diis", type=int, default=20,
                     help="DIIS vectors of both solvers (the reference's default diis-steps)")
     ap.add_argument("--workers", type=int, default=16)
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--ablation", action="store_true", help="also time FP64+learned and mixed+MP2")
@@ -343,7 +373,8 @@ def main():
     if ablation:
         line["config"]["ablation"] = ablation
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mols, args.cpu_budget)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(mols, r64["iters"], threads)
     print(json.dumps(line), flush=True)
 
 
