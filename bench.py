@@ -329,11 +329,18 @@ def main():
                         "kernel": "gemm_dual_bf16 (mixed solver, correction contractions of the inner iterations)",
                         "kernel_ms": corr_ms,
                         "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per correction evaluation"}
+            groups = ["update", "diis", "update_inner", "diis_inner", "e2", "e2_correction", "residual_ew",
+                      "residual_ew_correction", "outer_norm", "commit", "guess", "energy", "setup_ew", "control"]
             rooflines["time_split_mixed_ms"] = {
                 "correction_gemm": corr_ms, "outer_fp64_gemm": float(mm[0:25].sum()),
                 "setup_gemm": float(mm[60:64].sum()),
                 "elementwise": float(mm[64:96].sum()),
+                "elementwise_groups": {g: float(mm[64 + k]) for k, g in enumerate(groups)},
                 "profiled_solve": rmx["profiled_solve_ms"]}
+            rooflines["time_split_fp64_ms"] = {
+                "gemm": gemm_ms, "elementwise_groups": {g: float(ms[64 + k]) for k, g in enumerate(groups)
+                                                        if ms[64 + k] > 0},
+                "profiled_solve": r64["profiled_solve_ms"]}
     else:
         roofline = None
     line = {

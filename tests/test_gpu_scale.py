@@ -6,9 +6,10 @@
   the reference-form projections of ``oracle.ccsd`` (1e-11 relative).
 * One batch molecule per even N from 10 to 48, solved together by the batched
   FP64 solver, against ``oracle.ccsd.solve`` (energy 1e-8 eV, T2 1e-7).
-* The 150-site honeycomb flake (o = v = 75: the E2 terms run as GEMMs):
-  residuals at random amplitudes against the GEMM-form oracle
-  ``oracle.dressed`` (itself pinned against ``oracle.ccsd`` at small N).
+* The 150-site honeycomb flake (o = v = 75: the E2 terms run as GEMMs) and
+  the 200-site polyene (o = v = 100): residuals at random amplitudes against
+  the GEMM-form oracle ``oracle.dressed`` (itself pinned against
+  ``oracle.ccsd`` at small N).
 * The 200-site polyene of the plain-FP64 workload (SCF 1e-10, CCSD 1e-8): the
   GPU's converged amplitudes satisfy the oracle's equations (max |d| of the
   oracle residual there) and the oracle's energy expression gives the GPU's
@@ -113,6 +114,27 @@ def test_flake150_residual_parity():
     assert np.abs(d2 - r2).max() < 1e-11 * s2
 
 
+@pytest.mark.timeout(600)
+def test_polyene200_residual_parity():
+    """The 200-site polyene (o = v = 100) at random amplitudes."""
+    mol = M.polyene(200)
+    p = ppp.prepare(mol)
+    o, v = p["nocc"], mol.n - p["nocc"]
+    t1, t2 = random_amplitudes(o, v, 4, s1=0.02, s2=0.02)
+    ctx = capi.Context(0, capi.HCC_FP64, 8)
+    ctx.set_molecule(p["C"], p["F"], p["G"], o, p["eref"])
+    ctx.set_amplitudes(t1, t2)
+    d1, d2 = ctx.residual()
+    ctx.close()
+    say("polyene200: GPU residual done, oracle running")
+    w1, w2 = dressed.omega(p["C"], p["F"], p["G"], o, t1, t2)
+    r2 = dressed.contravariant(w2)
+    s1, s2 = max(1.0, np.abs(w1).max()), max(1.0, np.abs(r2).max())
+    say("polyene200", np.abs(d1 - w1).max() / s1, np.abs(d2 - r2).max() / s2)
+    assert np.abs(d1 - w1).max() < 1e-11 * s1
+    assert np.abs(d2 - r2).max() < 1e-11 * s2
+
+
 def _oracle_energy(p, t1, t2):
     """energyCC (mod_cc.f90:1916-1952) from the undressed (ia|jb) alone:
     E = eref + 2 sum F_ia t_ia + sum_ijab L_iajb (t_ij^ab + t_ia t_jb)."""
@@ -139,8 +161,13 @@ def test_polyene200_fp64_fixed_point():
     w1, w2 = dressed.omega(p["C"], p["F"], p["G"], o, t1, t2)
     r1, r2 = np.abs(w1).max(), np.abs(dressed.contravariant(w2)).max()
     eo = _oracle_energy(p, t1, t2)
+    # the iterator stops once the residual checked before the last update is
+    # below 1e-8, and the amplitudes returned are those after that update
+    # (stepCC runs before the convergence test, mod_cc.f90:874-888): with the
+    # x3 update of the '2-1' projection the residual there is about
+    # (1 - 3) times the checked one, so below 3e-8
     say(f"polyene200: oracle max|d1| {r1:.2e} max|d2| {r2:.2e}; E GPU - oracle {e[0] - eo:.2e}")
-    assert max(r1, r2) < 2e-8
+    assert max(r1, r2) < 3e-8
     assert abs(e[0] - eo) < 1e-8
 
 
