@@ -53,6 +53,8 @@ SIGNATURES = {
     "hcc_gemm_flops_correction": (_i, [_vp, _P(_d)]),
     "hcc_profile_ops": (_i, [_vp, _P(_d)]),
     "hcc_destroy": (None, [_vp]),
+    "hcc_solve_noalloc": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _i, _i, _d, _i, _P(_d), _P(_i), _P(_i),
+                               _P(_d), _P(_d), ctypes.c_char_p, _i]),
     "hcc_last_error": (ctypes.c_char_p, [_vp]),
     "hcc_selftest_gemm": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _d, _i]),
     "hcc_selftest_gemm_correction": (_i, [_i, _i, _i, _i, _i, _P(_d), _P(_f), _P(_d), _P(_f), _P(_f), _d, _d, _i,
@@ -377,6 +379,30 @@ class Trainer:
         y = np.zeros(X.shape[0], dtype=np.float32)
         self._check(self.lib.hcc_mlp_predict(self.h, _fp(X), X.shape[0], _fp(y)), "hcc_mlp_predict")
         return y
+
+
+def solve_noalloc(C, F, G, nocc, eref, guess=GUESS_MP2, maxiter=2000, eps=1e-10, diis_steps=20, device=0,
+                  amplitudes=True):
+    """hcc_solve_noalloc: (energy[5], iterations, status, t1, t2); raises HccError
+    with the library's message on a nonzero return."""
+    lib = load()
+    n = C.shape[0]
+    Cf, Ff, Gf = fortran(C), fortran(F), fortran(G)
+    e = np.zeros(5)
+    it = ctypes.c_int()
+    st = ctypes.c_int()
+    o, v = nocc, n - nocc
+    t1 = np.zeros((o, v), order="F") if amplitudes else None
+    t2 = np.zeros((o, o, v, v), order="F") if amplitudes else None
+    err = ctypes.create_string_buffer(256)
+    rc = lib.hcc_solve_noalloc(device, n, nocc, _dp(Cf), _dp(Ff), _dp(Gf), float(eref), guess, maxiter, eps,
+                               diis_steps, _dp(e), ctypes.byref(it), ctypes.byref(st),
+                               _dp(t1) if amplitudes else None, _dp(t2) if amplitudes else None, err, 256)
+    if rc != 0:
+        raise HccError(f"hcc_solve_noalloc failed ({rc}): {err.value.decode()}")
+    if amplitudes:
+        t1, t2 = np.ascontiguousarray(t1), np.ascontiguousarray(t2)
+    return e, it.value, st.value, t1, t2
 
 
 def selftest_gemm_correction(A0, dA, B0, dB, Cd, alpha=1.0, beta=0.0, a_kfast=True, b_nfast=True, tile=64, reps=0):

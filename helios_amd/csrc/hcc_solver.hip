@@ -24,6 +24,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <atomic>
 #include <array>
 #include <string>
@@ -151,10 +152,18 @@ struct Batch {
   std::vector<std::function<GemmProb(const Mol&)>> ops;
   std::vector<GemmTable> dit;
 
+  // device memory: hipMalloc per buffer, or -- when `arena` is set (the
+  // no-allocation entry point) -- consecutive 256-byte aligned pieces of a
+  // caller-owned region of arena_cap bytes, never freed here
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+
   ~Batch() { release(); }
   void release() {
-    for (void* p : allocs) (void)hipFree(p);
+    if (arena == nullptr)
+      for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
+    arena_used = 0;
     dm = nullptr;
     dst = nullptr;
     dact = nullptr;
@@ -163,6 +172,16 @@ struct Batch {
   T* dalloc(size_t count) {
     void* p = nullptr;
     if (count == 0) count = 1;
+    if (arena != nullptr) {
+      const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+      if (bytes > arena_cap - arena_used)
+        throw std::string("molecule too large for the no-allocation scratch region (") +
+            std::to_string(arena_cap >> 20) + " MiB): needs more than " + std::to_string((arena_used + bytes) >> 20) +
+            " MiB";
+      p = arena + arena_used;
+      arena_used += bytes;
+      return static_cast<T*>(p);
+    }
     HCHECK(hipMalloc(&p, count * sizeof(T)));
     allocs.push_back(p);
     return static_cast<T*>(p);
@@ -820,6 +839,11 @@ struct hcc_ctx {
   // src/modules/mod_helios.f90:441): set from any thread or a signal handler,
   // read by the whole-solve loops between iteration blocks
   std::atomic<int> stop_req{0};
+  // no-allocation contexts: device memory from this region (see Batch::arena),
+  // no timing events
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  bool timing = true;
 };
 
 static_assert(std::atomic<int>::is_always_lock_free, "hcc_request_stop must be async-signal-safe");
@@ -1057,6 +1081,8 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   c->have_guess = false;
   std::unique_ptr<Batch> owner(new Batch());
   Batch* b = owner.get();
+  b->arena = c->arena;
+  b->arena_cap = c->arena_cap;
   b->nm = nmol;
   b->m = c->diis_steps;
   b->mixed = c->precision == HCC_MIXED;
@@ -1442,10 +1468,12 @@ static void capture(GraphExec& ge, hipStream_t s, const std::function<void()>& b
 static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool build_integrals) {
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
-  hipEvent_t e0, e1;
-  HCHECK(hipEventCreate(&e0));
-  HCHECK(hipEventCreate(&e1));
-  HCHECK(hipEventRecord(e0, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ctx->timing) {
+    HCHECK(hipEventCreate(&e0));
+    HCHECK(hipEventCreate(&e1));
+    HCHECK(hipEventRecord(e0, s));
+  }
   if (build_integrals) precompute(ctx);
   apply_guess(ctx, guess_kind);
   Params prm = params(ctx, eps, maxiter, true);
@@ -1479,12 +1507,16 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
   prof_close(ctx);
-  HCHECK(hipEventRecord(e1, s));
-  HCHECK(hipEventSynchronize(e1));
   float ms = 0.f;
-  HCHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (ctx->timing) {
+    HCHECK(hipEventRecord(e1, s));
+    HCHECK(hipEventSynchronize(e1));
+    HCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } else {
+    HCHECK(hipStreamSynchronize(s));
+  }
   if (ctx->prof) prof_collect(ctx);
   ctx->last_ms = ms;
   std::vector<State> S(b.nm);
@@ -1848,6 +1880,109 @@ void hcc_destroy(hcc_ctx* ctx) {
 const char* hcc_last_error(const hcc_ctx* ctx) {
   if (ctx == nullptr) return "null context";
   return ctx->err.c_str();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Solve without device allocation (hcc_solve_noalloc).  Every device buffer of
+// the solve -- the molecule's arenas, descriptors, block tables, GEMM tables
+// and partial results -- is carved, 256-byte aligned, from the file-scope
+// array `scratch`, whose address on each device comes from
+// hipGetSymbolAddress once per device (the array's name is only a host-side
+// stand-in for it).  Calls on one device share the array and a context, so
+// they must not overlap; each call rebuilds everything it reads, and the state
+// and the partial results it accumulates are cleared by the solve itself.
+// ---------------------------------------------------------------------------
+// (no `used` attribute: it would keep clang from exporting the array under the
+// name the host registers, and the runtime's lookup by that name would fail)
+constexpr size_t kNoAllocBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocDevice {
+  char* ptr = nullptr;       // this device's copy of `scratch`
+  hcc_ctx* ctx = nullptr;    // null stream, no graphs, no timing events
+};
+std::mutex g_noalloc_mu;
+std::map<int, NoAllocDevice> g_noalloc;
+
+void noalloc_message(char* err, int len, const std::string& m) {
+  if (err != nullptr && len > 0) snprintf(err, (size_t)len, "%s", m.c_str());
+}
+}  // namespace
+
+extern "C" {
+
+int hcc_solve_noalloc(int device, int n, int n_occ, const double* C, const double* F_mo, const double* gamma,
+                      double e_ref, int guess_kind, int maxiter, double eps, int diis_steps, double energy_out[5],
+                      int* iterations, int* status, double* t1, double* t2, char* err, int err_len) {
+  noalloc_message(err, err_len, "");
+  if (C == nullptr || F_mo == nullptr || gamma == nullptr || energy_out == nullptr) {
+    noalloc_message(err, err_len, "null input or energy array");
+    return HCC_E_ARG;
+  }
+  if (guess_kind != HCC_GUESS_REFERENCE && guess_kind != HCC_GUESS_MP2) {
+    noalloc_message(err, err_len, "guess must be HCC_GUESS_REFERENCE or HCC_GUESS_MP2");
+    return HCC_E_ARG;
+  }
+  if (maxiter < 1 || diis_steps < 2 || diis_steps > kMaxDiis) {
+    noalloc_message(err, err_len, "maxiter must be positive and diis_steps within 2..32");
+    return HCC_E_ARG;
+  }
+  NoAllocDevice* d = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    d = &g_noalloc[device];
+    if (d->ptr == nullptr) {
+      if (hipSetDevice(device) != hipSuccess) {
+        noalloc_message(err, err_len, "hipSetDevice failed");
+        return HCC_E_HIP;
+      }
+      void* p = nullptr;
+      const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (r != hipSuccess || p == nullptr) {
+        noalloc_message(err, err_len, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+        return HCC_E_HIP;
+      }
+      hcc_ctx* c = new hcc_ctx();
+      c->device = device;
+      c->stream = nullptr;
+      c->graphs = 0;
+      c->timing = false;
+      c->arena = static_cast<char*>(p);
+      c->arena_cap = kNoAllocBytes;
+      d->ptr = static_cast<char*>(p);
+      d->ctx = c;
+    }
+  }
+  hcc_ctx* ctx = d->ctx;
+  ctx->diis_steps = diis_steps;
+  int rc = HCC_OK;
+  try {
+    HCHECK(hipSetDevice(device));
+    load(ctx, 1, &n, &n_occ, C, F_mo, gamma, &e_ref, false);
+    precompute(ctx);
+    run_solve(ctx, guess_kind, maxiter, eps, false);
+    State S;
+    HCHECK(hipMemcpy(&S, ctx->batch->dst, sizeof(State), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 5; ++k) energy_out[k] = S.energy[k];
+    if (iterations) *iterations = S.iter;
+    if (status) *status = S.status;
+    if (t1 || t2) copy_amplitudes_out(ctx, t1, t2);
+  } catch (const HipError& he) {
+    noalloc_message(err, err_len, std::string(he.what) + ": " + hipGetErrorString(he.e));
+    rc = HCC_E_HIP;
+  } catch (const std::string& se) {
+    noalloc_message(err, err_len, se);
+    rc = HCC_E_ARG;
+  } catch (const std::bad_alloc&) {
+    noalloc_message(err, err_len, "host allocation failed");
+    rc = HCC_E_MEMORY;
+  }
+  delete ctx->batch;   // its pieces of the scratch region are not freed
+  ctx->batch = nullptr;
+  return rc;
 }
 
 int hcc_selftest_gemm(int m, int n, int k, const double* A, const double* B, double* C, double alpha, double beta,

@@ -205,6 +205,23 @@ int hcc_gemm_shape(hcc_ctx* ctx, int op, int mol, int* mnk);
 void hcc_destroy(hcc_ctx* ctx);
 const char* hcc_last_error(const hcc_ctx* ctx);
 
+/* Whole FP64 solve of one molecule without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc).  Inputs, outputs
+ * and the meaning of every argument are those of hcc_create(device, HCC_FP64,
+ * diis_steps) + hcc_set_molecule + hcc_solve(guess, maxiter, eps) +
+ * hcc_get_amplitudes (t1, t2 may be null; guess: HCC_GUESS_REFERENCE or
+ * HCC_GUESS_MP2), with the default options of hcc_set_options and
+ * hcc_set_iterator.  Every device buffer of the solve comes from a 128 MiB
+ * region that the library reserves statically on each device it runs on
+ * (held until the process ends): a molecule of up to 48 sites at 20 DIIS
+ * vectors fits; a larger one is rejected with HCC_E_ARG and a message in
+ * err (err_len bytes, may be null).  The call makes the device current and
+ * uses the null stream.  Calls on one device share the region: calls that
+ * overlap on one device race; each call sets up everything it reads. */
+int hcc_solve_noalloc(int device, int n, int n_occ, const double* C, const double* F_mo, const double* gamma,
+                      double e_ref, int guess, int maxiter, double eps, int diis_steps, double energy[5],
+                      int* iterations, int* status, double* t1, double* t2, char* err, int err_len);
 
 /* ------------------------------------------------------------------------
  * Learned T2 guess (no reference counterpart: it replaces the guess of
