@@ -43,7 +43,8 @@ DUAL_GEMM_TRAFFIC_BATCH4096 = 1.328e9   # gemm_dual_bf16, mixed solve
 def _prep(k):
     m = molecules.batch_member(k)
     p = ppp.prepare(m)
-    return dict(C=p["C"], F=p["F"], G=p["G"], nocc=p["nocc"], eref=p["eref"], n=m.n)
+    _, _, hk = ppp.model(m.coords, m.bonds, m.alpha, m.U, 1)
+    return dict(C=p["C"], F=p["F"], G=p["G"], nocc=p["nocc"], eref=p["eref"], n=m.n, core=p["core"], huckel=hk)
 
 
 def prepare_batch(nmol, workers):
@@ -204,6 +205,25 @@ def main():
         ctx.close()
         print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
               f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
+
+    # the batch's RHF on the device (the step before the path; its time is
+    # reported beside the headline, which starts from the MO inputs)
+    device_scf = None
+    if not args.skip_reference:
+        ctx = capi.Context(0, capi.HCC_FP64, args.diis)
+        ctx.batch_load_scf(mols, accuracy=1e-11, maxiter=400)      # warm-up (code object load)
+        t1 = time.perf_counter()
+        er, its_scf, st_scf = ctx.batch_load_scf(mols, accuracy=1e-11, maxiter=400)
+        scf_s = time.perf_counter() - t1
+        ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+        e_dev, _, st_dev = ctx.batch_results()
+        ctx.close()
+        device_scf = {"ms_incl_upload": scf_s * 1e3, "mean_iterations": float(its_scf.mean()),
+                      "converged": int((st_scf == 1).sum()),
+                      "max_abs_dE_ccsd_vs_host_scf_eV": float(np.abs(e_dev[:, 0] - e_ref[:, 0]).max())}
+        print(f"# device SCF: {scf_s * 1e3:.1f} ms for {len(mols)} molecules, mean iterations "
+              f"{its_scf.mean():.1f}, CCSD from it vs host SCF max|dE| "
+              f"{device_scf['max_abs_dE_ccsd_vs_host_scf_eV']:.2e}", file=sys.stderr, flush=True)
 
     net = learned_guess.load_params()
     tuning = None if args.no_tuning else autotune.load()
@@ -379,6 +399,8 @@ def main():
         line["config"]["mixed_max_abs_dE_per_molecule_eV"] = float(rmx["de"].max())
     if ablation:
         line["config"]["ablation"] = ablation
+    if device_scf is not None:
+        line["config"]["device_scf"] = device_scf
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(mols, r64["iters"], threads)

@@ -45,6 +45,8 @@ SIGNATURES = {
     "hcc_solve": (_i, [_vp, _i, _i, _d, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_load": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_solve": (_i, [_vp, _i, _i, _d]),
+    "hcc_batch_get_orbitals": (_i, [_vp, _i, _P(_d), _P(_d), _P(_d)]),
+    "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_profile": (_i, [_vp, _i]),
@@ -228,6 +230,32 @@ class Context:
         self._check(self.lib.hcc_batch_load(self.h, len(mols), _ip(n), _ip(no), _dp(C), _dp(F), _dp(G), _dp(er)),
                     "hcc_batch_load")
         self.nmol = len(mols)
+
+    def batch_load_scf(self, mols, accuracy=1e-10, maxiter=400):
+        """mols: list of dicts with core, G, huckel (n x n) and nocc; the RHF runs
+        on the device.  Returns (e_ref, SCF iterations, SCF status) per molecule."""
+        n = np.array([m["core"].shape[0] for m in mols], dtype=np.int32)
+        no = np.array([m["nocc"] for m in mols], dtype=np.int32)
+        core = np.concatenate([fortran(m["core"]).ravel(order="F") for m in mols])
+        G = np.concatenate([fortran(m["G"]).ravel(order="F") for m in mols])
+        hk = np.concatenate([fortran(m["huckel"]).ravel(order="F") for m in mols])
+        er = np.zeros(len(mols))
+        it = np.zeros(len(mols), dtype=np.int32)
+        st = np.zeros(len(mols), dtype=np.int32)
+        self._check(self.lib.hcc_batch_load_scf(self.h, len(mols), _ip(n), _ip(no), _dp(core), _dp(G), _dp(hk),
+                                                float(accuracy), int(maxiter), _dp(er), _ip(it), _ip(st)),
+                    "hcc_batch_load_scf")
+        self.nmol = len(mols)
+        return er, it, st
+
+    def batch_orbitals(self, mol, n):
+        """(C, F_mo, e_ref) of loaded molecule `mol` with n sites."""
+        C = np.zeros((n, n), order="F")
+        F = np.zeros((n, n), order="F")
+        er = ctypes.c_double()
+        self._check(self.lib.hcc_batch_get_orbitals(self.h, mol, _dp(C), _dp(F), ctypes.byref(er)),
+                    "hcc_batch_get_orbitals")
+        return np.ascontiguousarray(C), np.ascontiguousarray(F), er.value
 
     def batch_solve(self, guess=GUESS_MP2, maxiter=200, eps=1e-7):
         self._check(self.lib.hcc_batch_solve(self.h, guess, maxiter, eps), "hcc_batch_solve")

@@ -34,6 +34,7 @@
 #include "hcc_batch.h"
 #include "hcc_ops.h"
 #include "hcc_mlp.h"
+#include "hcc_scf.h"
 
 namespace hcc {
 
@@ -1107,12 +1108,14 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   build_tables(*b);
   if (b->mixed) build_delta_tables(*b);
   // upload, transposing the column-major inputs to row-major: one staging
-  // image per arena (molecule blocks at their arena offsets), one copy each
+  // image per arena (molecule blocks at their arena offsets), one copy each;
+  // a null C or F is computed on the device afterwards (hcc_batch_load_scf)
   double* dsts0[3] = {b->hm[0].C, b->hm[0].F, b->hm[0].G};
   const double* srcs[3] = {C, F, G};
   const long nlast = (long)n[nmol - 1] * n[nmol - 1];
   std::vector<double> h;
   for (int q = 0; q < 3; ++q) {
+    if (srcs[q] == nullptr) continue;
     double* base = dsts0[q];
     double* const* last = q == 0 ? &b->hm[nmol - 1].C : (q == 1 ? &b->hm[nmol - 1].F : &b->hm[nmol - 1].G);
     const long span = (*last - base) + nlast;
@@ -1278,6 +1281,89 @@ int hcc_batch_load(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   load(ctx, nmol, n, n_occ, C, F_mo, gamma, e_ref, false);
+  API_END
+}
+
+int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* core,
+                       const double* gamma, const double* huckel, double scf_accuracy, int scf_maxiter,
+                       double* e_ref, int* scf_iterations, int* scf_status) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (nmol < 1 || !n || !n_occ || !core || !gamma || !huckel || !(scf_accuracy > 0.0) || scf_maxiter < 1) {
+    ctx->err = "invalid SCF batch arguments";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  int nmax = 0;
+  for (int k = 0; k < nmol; ++k) nmax = std::max(nmax, n[k]);
+  int lds_max = 0;
+  HCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  if (nmax > 64 || scf_lds_bytes(nmax) > (size_t)lds_max)
+    throw std::string("the device SCF takes molecules of up to 64 sites whose three n x n matrices fit in LDS (") +
+        std::to_string(lds_max) + " bytes per workgroup)";
+  std::vector<double> zero_eref(nmol, 0.0);
+  load(ctx, nmol, n, n_occ, nullptr, nullptr, gamma, zero_eref.data(), false);
+  Batch& b = *ctx->batch;
+  // core and Hueckel matrices (row-major; column-major inputs transposed) and work blocks
+  long total = 0;
+  std::vector<long> off(nmol);
+  for (int k = 0; k < nmol; ++k) {
+    off[k] = total;
+    total += (long)n[k] * n[k];
+  }
+  double* dcore = b.dalloc<double>(total);
+  double* dhk = b.dalloc<double>(total);
+  double* dwork = b.dalloc<double>(total);
+  std::vector<double> hc(total), hh(total);
+  for (int k = 0; k < nmol; ++k)
+    for (int r = 0; r < n[k]; ++r)
+      for (int c = 0; c < n[k]; ++c) {
+        hc[off[k] + (long)r * n[k] + c] = core[off[k] + r + (long)c * n[k]];
+        hh[off[k] + (long)r * n[k] + c] = huckel[off[k] + r + (long)c * n[k]];
+      }
+  HCHECK(hipMemcpy(dcore, hc.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(dhk, hh.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<ScfInput> in(nmol);
+  for (int k = 0; k < nmol; ++k) in[k] = ScfInput{dcore + off[k], dhk + off[k], dwork + off[k]};
+  ScfInput* din = b.dalloc<ScfInput>(nmol);
+  ScfResult* dres = b.dalloc<ScfResult>(nmol);
+  HCHECK(hipMemcpy(din, in.data(), nmol * sizeof(ScfInput), hipMemcpyHostToDevice));
+  HCHECK(launch_rhf(b.dm, din, nmol, nmax, scf_accuracy, scf_maxiter, 0.05, dres, ctx->stream));
+  std::vector<ScfResult> res(nmol);
+  HCHECK(hipMemcpyAsync(res.data(), dres, nmol * sizeof(ScfResult), hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < nmol; ++k) {
+    b.eref[k] = res[k].eref;
+    b.hm[k].eref = res[k].eref;
+    if (e_ref) e_ref[k] = res[k].eref;
+    if (scf_iterations) scf_iterations[k] = res[k].iterations;
+    if (scf_status) scf_status[k] = res[k].status;
+  }
+  API_END
+}
+
+static int need_batch(hcc_ctx* ctx);
+
+int hcc_batch_get_orbitals(hcc_ctx* ctx, int mol, double* C, double* F_mo, double* e_ref) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  Batch& b = *ctx->batch;
+  if (mol < 0 || mol >= b.nm) {
+    ctx->err = "invalid molecule index";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  const int N = b.n[mol];
+  std::vector<double> h((size_t)N * N);
+  double* src[2] = {b.hm[mol].C, b.hm[mol].F};
+  double* dst[2] = {C, F_mo};
+  for (int q = 0; q < 2; ++q) {
+    if (dst[q] == nullptr) continue;
+    HCHECK(hipMemcpy(h.data(), src[q], h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < N; ++r)
+      for (int c = 0; c < N; ++c) dst[q][r + (size_t)c * N] = h[(size_t)r * N + c];
+  }
+  if (e_ref) *e_ref = b.eref[mol];
   API_END
 }
 

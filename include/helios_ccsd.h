@@ -149,6 +149,27 @@ int hcc_solve(hcc_ctx* ctx, int guess, int maxiter, double eps, double energy[5]
 int hcc_batch_load(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* C,
                    const double* F_mo, const double* gamma, const double* e_ref);
 int hcc_batch_solve(hcc_ctx* ctx, int guess, int maxiter, double eps);
+
+/* hcc_batch_load with the SCF on the device: from the n[k] x n[k]
+ * column-major core Hamiltonian (mol%core), gamma (mol%G) and Hueckel matrix
+ * (mol%huckelCore) of every molecule, the reference's RHF -- Hueckel guess,
+ * then damped-density steps of 0.05 until max|Fmin| < scf_accuracy or
+ * scf_maxiter iterations (src/methods/mod_scf.f90:106-136, 351-386, 446-494)
+ * -- runs one workgroup per molecule and leaves C, the MO Fock matrix and the
+ * reference energy in the batch, as initCC prepares them after the SCF
+ * (src/methods/coupledCluster/mod_cc.f90:538-545, 2311-2504); MO columns
+ * ascend in energy, each with its largest-|C| entry positive.  Outputs per
+ * molecule (each may be null): e_ref, SCF iterations, status (1 converged,
+ * -1 iteration cap, -2 NaN).  Molecules of up to 64 sites. */
+int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, const double* core,
+                       const double* gamma, const double* huckel, double scf_accuracy, int scf_maxiter,
+                       double* e_ref, int* scf_iterations, int* scf_status);
+
+/* MO coefficients C (n x n column-major, = hV), MO Fock matrix and reference
+ * energy of loaded molecule `mol` (either array may be null): the values the
+ * solve starts from (getSCFResult(vectors=hV), src/methods/mod_scf.f90:204-277,
+ * and prepareFock, src/methods/coupledCluster/mod_cc.f90:2336-2396). */
+int hcc_batch_get_orbitals(hcc_ctx* ctx, int mol, double* C, double* F_mo, double* e_ref);
 int hcc_batch_results(hcc_ctx* ctx, double* energies, int* iterations, int* status);
 
 /* Device timing of the last hcc_solve / hcc_batch_solve in milliseconds, and
