@@ -46,6 +46,7 @@ SIGNATURES = {
     "hcc_batch_load": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_solve": (_i, [_vp, _i, _i, _d]),
     "hcc_batch_get_orbitals": (_i, [_vp, _i, _P(_d), _P(_d), _P(_d)]),
+    "hcc_batch_triples": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
@@ -247,6 +248,14 @@ class Context:
                     "hcc_batch_load_scf")
         self.nmol = len(mols)
         return er, it, st
+
+    def batch_triples(self):
+        """(energies (nmol x 5) as the reference's energyCC('spin-r-ccsd(t)'),
+        standard (E[T], E[ST]) (nmol x 2)) of the converged amplitudes."""
+        e = np.zeros((self.nmol, 5))
+        et = np.zeros((self.nmol, 2))
+        self._check(self.lib.hcc_batch_triples(self.h, _dp(e), _dp(et)), "hcc_batch_triples")
+        return e, et
 
     def batch_orbitals(self, mol, n):
         """(C, F_mo, e_ref) of loaded molecule `mol` with n sites."""

@@ -48,6 +48,7 @@ void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s);
 void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 void stop_running(State* st, int* act, int nm, hipStream_t s);
+void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s);
 
 // mixed-precision solver (correction form)
 void mixed_dress(const Mol* m0, const MolD* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,

@@ -909,6 +909,23 @@ void reset_state(State* st, int* act, int nm, hipStream_t s) {
 void activate(State* st, int* act, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_activate, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
 }
+// post-solve passes over the whole batch: arm every molecule, keeping its
+// active flag in save[] (statuses untouched); then restore the flags
+__global__ void k_arm_all(State* __restrict__ st, int* __restrict__ act, int* __restrict__ save, int nm, int restore) {
+  const int mol = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol >= nm) return;
+  if (restore) {
+    st[mol].active = save[mol];
+    act[mol] = save[mol];
+  } else {
+    save[mol] = st[mol].active;
+    st[mol].active = 1;
+    act[mol] = 1;
+  }
+}
+void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s) {
+  hipLaunchKernelGGL(k_arm_all, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, save, nm, restore ? 1 : 0);
+}
 void stop_running(State* st, int* act, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_stop_running, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);
 }

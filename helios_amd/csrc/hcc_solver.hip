@@ -35,6 +35,7 @@
 #include "hcc_ops.h"
 #include "hcc_mlp.h"
 #include "hcc_scf.h"
+#include "hcc_triples.h"
 
 namespace hcc {
 
@@ -152,6 +153,15 @@ struct Batch {
   MolD* ddm = nullptr;
   std::vector<std::function<GemmProb(const Mol&)>> ops;
   std::vector<GemmTable> dit;
+  // (T): undressed integral blocks (xp|yd), (ql|rz), their GEMM tables and the work items
+  std::vector<double*> tA, tB;
+  std::vector<GemmTable> tpre;
+  TripMol* dtm = nullptr;
+  long* dtprefix = nullptr;
+  int *dtri = nullptr, *dtsave = nullptr;
+  double *dtw = nullptr, *dtpart = nullptr, *dtout = nullptr;
+  long ttotal = 0, twstride = 0;
+  int tgrid = 0;
 
   // device memory: hipMalloc per buffer, or -- when `arena` is set (the
   // no-allocation entry point) -- consecutive 256-byte aligned pieces of a
@@ -1343,6 +1353,117 @@ int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, c
 }
 
 static int need_batch(hcc_ctx* ctx);
+
+// Buffers, GEMM tables and work items of the (T) correction (once per batch).
+static void setup_triples(Batch& b) {
+  const int nm = b.nm;
+  b.tA.assign(nm, nullptr);
+  b.tB.assign(nm, nullptr);
+  int omax = 0, vmax = 0;
+  std::vector<long> prefix(nm + 1, 0);
+  for (int k = 0; k < nm; ++k) {
+    const long o = b.o[k], v = b.v[k];
+    b.tA[k] = b.dalloc<double>(o * v * v * v);
+    b.tB[k] = b.dalloc<double>(o * o * o * v);
+    omax = std::max(omax, b.o[k]);
+    vmax = std::max(vmax, b.v[k]);
+    prefix[k + 1] = prefix[k] + o * (o + 1) * (o + 2) / 6;
+  }
+  b.tpre.clear();
+  // (xp|yd) at [p][x][y][d]: rows (p, x) of P, columns (y, d) of Q = gamma P (undressed)
+  b.tpre.push_back(make_table(b, [&](int k) {
+    const Mol& M = b.hm[k];
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N;
+    return prob(cmp(M.P + o * N, v, 1, N, 1, N2, 0), cmp(M.Q + o * N + o, 1, N2, 0, v, N, 1), rm(b.tA[k], v * v),
+                o * v, v * v, N, 1.0, 0.0);
+  }));
+  // (ql|rz) at [q][l][r][z]: rows (q, l) of P, columns (r, z) of Q
+  b.tpre.push_back(make_table(b, [&](int k) {
+    const Mol& M = b.hm[k];
+    const int N = M.n, o = M.o, v = M.v, N2 = N * N;
+    return prob(cmp(M.P, o, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1), rm(b.tB[k], o * v), o * o, o * v, N,
+                1.0, 0.0);
+  }));
+  std::vector<int> tri;
+  for (int i = 0; i < omax; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int k = 0; k <= j; ++k) tri.push_back(i | (j << 8) | (k << 16));
+  std::vector<TripMol> tm(nm);
+  for (int k = 0; k < nm; ++k) {
+    const Mol& M = b.hm[k];
+    tm[k] = TripMol{M.n, M.o, M.v, 0, b.tA[k], b.tB[k], M.Gklcd, M.t1, M.t2, M.F};
+  }
+  b.ttotal = prefix[nm];
+  b.twstride = ((long)vmax * vmax * vmax + 31) / 32 * 32;
+  b.tgrid = (int)std::max<long>(1, std::min<long>(b.ttotal, 1024));
+  b.dtm = b.dalloc<TripMol>(nm);
+  b.dtprefix = b.dalloc<long>(nm + 1);
+  b.dtri = b.dalloc<int>(tri.size());
+  b.dtw = b.dalloc<double>((size_t)b.tgrid * b.twstride);
+  b.dtpart = b.dalloc<double>(2 * std::max<long>(1, b.ttotal));
+  b.dtout = b.dalloc<double>(2 * (long)nm);
+  b.dtsave = b.dalloc<int>(nm);
+  HCHECK(hipMemcpy(b.dtm, tm.data(), nm * sizeof(TripMol), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(b.dtprefix, prefix.data(), (nm + 1) * sizeof(long), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(b.dtri, tri.data(), tri.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  Batch& b = *ctx->batch;
+  for (int k = 0; k < b.nm; ++k)
+    if (b.o[k] > kTripMaxOV || b.v[k] > kTripMaxOV) {
+      ctx->err = "the (T) correction takes molecules with up to 32 occupied and 32 virtual orbitals";
+      return HCC_E_ARG;
+    }
+  if (!ctx->have_guess) {
+    ctx->err = "(T) needs amplitudes: solve first";
+    return HCC_E_STATE;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  if (b.dtm == nullptr) setup_triples(b);
+  // undressed orbital products (t1 = 0), with t1 kept in w1 meanwhile: the
+  // t1 and w1 arenas have the same per-molecule offsets
+  const Mol& first = b.hm[0];
+  const Mol& last = b.hm[b.nm - 1];
+  const size_t span = (size_t)(last.t1 - first.t1) + (size_t)last.o * last.v;
+  HCHECK(hipMemcpyAsync(first.w1, first.t1, span * sizeof(double), hipMemcpyDeviceToDevice, s));
+  arm_all(b.dst, b.dact, b.dtsave, b.nm, false, s);
+  zero_t1(b.dm, b.rgOV, b.nm, s);
+  ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
+  ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  gemm(ctx, b.it[G1_Q], 62);
+  gemm(ctx, b.tpre[0], 60);
+  gemm(ctx, b.tpre[1], 61);
+  HCHECK(hipMemcpyAsync(first.t1, first.w1, span * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HCHECK(launch_triples(b.dtm, b.nm, b.dtprefix, b.ttotal, b.dtri, b.dtw, b.twstride, b.tgrid, b.dtpart, b.dtout, s));
+  arm_all(b.dst, b.dact, b.dtsave, b.nm, true, s);
+  std::vector<double> out(2 * (size_t)b.nm);
+  std::vector<State> S(b.nm);
+  HCHECK(hipMemcpyAsync(out.data(), b.dtout, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HCHECK(hipMemcpyAsync(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost, s));
+  HCHECK(hipStreamSynchronize(s));
+  for (int k = 0; k < b.nm; ++k) {
+    const double eT = out[2 * k], eST = out[2 * k + 1];
+    if (e_t) {
+      e_t[2 * k] = eT;
+      e_t[2 * k + 1] = eST;
+    }
+    if (energies) {
+      // energyCC('spin-r-ccsd(t)'): c1 = -E[T], c2 = -E[ST]; c3 = sum t2 F_kc = 0 (F_kc = 0 for the RHF reference)
+      const double c1 = -eT, c2 = -eST;
+      energies[5 * k + 0] = S[k].energy[0] + c1 + c2;
+      energies[5 * k + 1] = S[k].energy[1];
+      energies[5 * k + 2] = S[k].energy[2];
+      energies[5 * k + 3] = S[k].energy[3];
+      energies[5 * k + 4] = c1;
+    }
+  }
+  API_END
+}
 
 int hcc_batch_get_orbitals(hcc_ctx* ctx, int mol, double* C, double* F_mo, double* e_ref) {
   if (ctx == nullptr) return HCC_E_ARG;

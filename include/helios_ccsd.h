@@ -165,6 +165,20 @@ int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, c
                        const double* gamma, const double* huckel, double scf_accuracy, int scf_maxiter,
                        double* e_ref, int* scf_iterations, int* scf_status);
 
+/* (T) triples correction of the converged amplitudes of every loaded molecule
+ * (after hcc_solve / hcc_batch_solve), the counterpart of the reference's
+ * energyCC('spin-r-ccsd(t)') (src/methods/coupledCluster/mod_cc.f90:2028-2111)
+ * applied to these closed-shell amplitudes as convertSpatialToSpinCC turns
+ * them into spin orbitals (mod_cc.f90:2601-2678).  energies (5 per molecule,
+ * may be null) is that routine's energy(1..5): total = CCSD energy + c1 + c2
+ * + c3, reference, singles, doubles, c1.  The reference's c1 and c2 are the
+ * negatives of the standard E[T] and E[ST] (its triples raise the energy);
+ * c3 = sum amp t2 F_kc vanishes for the canonical RHF reference (F_kc = 0).
+ * e_t (2 per molecule, may be null) gives the standard E[T] and E[ST], whose
+ * sum is the usual (T) correction.  Molecules with up to 32 occupied and 32
+ * virtual orbitals. */
+int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t);
+
 /* MO coefficients C (n x n column-major, = hV), MO Fock matrix and reference
  * energy of loaded molecule `mol` (either array may be null): the values the
  * solve starts from (getSCFResult(vectors=hV), src/methods/mod_scf.f90:204-277,
