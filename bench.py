@@ -84,6 +84,38 @@ def diis_bytes(o, v, iters, m):
     return total
 
 
+def f_triples(o, v):
+    """(T) flops per molecule: the connected triples w over all ordered occupied
+    triples, each a v^3 x (v + o) multiply-add (the six permutations of an
+    i >= j >= k item cover the ordered triples once)."""
+    o = np.asarray(o, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    return 2.0 * o ** 3 * v ** 3 * (o + v)
+
+
+def triples_leg(ctx, o, v):
+    """(T) of the converged batch: one untimed call, one timed, one profiled."""
+    ctx.batch_triples()
+    t1 = time.perf_counter()
+    _, et = ctx.batch_triples()
+    wall = time.perf_counter() - t1
+    ctx.profile(True)
+    ctx.batch_triples()
+    ops = ctx.profile_ops()
+    ctx.profile(False)
+    kms = float(ops[78])
+    fl = float(f_triples(o, v).sum())
+    out = {"bound": "fp64", "achieved": fl / (kms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+           "frac": fl / (kms * 1e-3) / PEAK_FP64, "kernel": "k_triples (FP64 vector FMA)", "kernel_ms": kms,
+           "integral_gemm_ms": float(ops[58] + ops[59] + ops[62]), "setup_ew_ms": float(ops[79]),
+           "wall_ms_incl_readback": wall * 1e3, "molecules": int(len(o)),
+           "mean_e_t_eV": float(et.sum(1).mean()),
+           "work": "2 o^3 v^3 (o + v) flops per molecule (connected triples over ordered i, j, k)"}
+    print(f"# (T): kernel {kms:.1f} ms ({out['achieved']:.1f} TFLOP/s, {out['frac']:.3f} of FP64 peak), "
+          f"wall {wall * 1e3:.1f} ms for {len(o)} molecules", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB, three times, to calibrate the HBM traffic
     counters of a profiler pass (FETCH_SIZE, WRITE_SIZE per dispatch)."""
@@ -194,6 +226,7 @@ def main():
     v = np.array([m["n"] - m["nocc"] for m in mols])
 
     # FP64 reference energies (untimed, tight threshold)
+    triples = None
     if args.skip_reference:
         e_ref = np.zeros((len(mols), 5))
         st_ref = np.ones(len(mols), dtype=np.int32)
@@ -202,6 +235,7 @@ def main():
         ctx.batch_load(mols)
         ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
         e_ref, it_ref, st_ref = ctx.batch_results()
+        triples = triples_leg(ctx, o, v)
         ctx.close()
         print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
               f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
@@ -401,6 +435,8 @@ def main():
         line["config"]["ablation"] = ablation
     if device_scf is not None:
         line["config"]["device_scf"] = device_scf
+    if triples is not None:
+        line["roofline_triples"] = triples
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(mols, r64["iters"], threads)

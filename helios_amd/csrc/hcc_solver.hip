@@ -806,6 +806,8 @@ enum ProfGroup {
   kProfEnergy = 75,
   kProfPrecomputeEw = 76,
   kProfControl = 77,       // per-molecule iterator control kernels
+  kProfTriples = 78,       // k_triples + k_triples_final ((T) correction)
+  kProfTriplesEw = 79,     // (T) set-up: flags, undressed products, t1 save/restore
 };
 
 struct hcc_ctx {
@@ -1430,22 +1432,31 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
   const Mol& first = b.hm[0];
   const Mol& last = b.hm[b.nm - 1];
   const size_t span = (size_t)(last.t1 - first.t1) + (size_t)last.o * last.v;
+  prof_open(ctx, kProfTriplesEw);
   HCHECK(hipMemcpyAsync(first.w1, first.t1, span * sizeof(double), hipMemcpyDeviceToDevice, s));
   arm_all(b.dst, b.dact, b.dtsave, b.nm, false, s);
   zero_t1(b.dm, b.rgOV, b.nm, s);
   ew_dress(b.dm, b.dst, b.rgNN, b.nm, s);
   ew_pouter(b.dm, b.dst, b.rgN3, b.nm, s);
+  prof_close(ctx);
   gemm(ctx, b.it[G1_Q], 62);
-  gemm(ctx, b.tpre[0], 60);
-  gemm(ctx, b.tpre[1], 61);
+  gemm(ctx, b.tpre[0], 58);
+  gemm(ctx, b.tpre[1], 59);
+  prof_open(ctx, kProfTriplesEw);
   HCHECK(hipMemcpyAsync(first.t1, first.w1, span * sizeof(double), hipMemcpyDeviceToDevice, s));
+  prof_close(ctx);
+  prof_open(ctx, kProfTriples);
   HCHECK(launch_triples(b.dtm, b.nm, b.dtprefix, b.ttotal, b.dtri, b.dtw, b.twstride, b.tgrid, b.dtpart, b.dtout, s));
+  prof_close(ctx);
+  prof_open(ctx, kProfTriplesEw);
   arm_all(b.dst, b.dact, b.dtsave, b.nm, true, s);
+  prof_close(ctx);
   std::vector<double> out(2 * (size_t)b.nm);
   std::vector<State> S(b.nm);
   HCHECK(hipMemcpyAsync(out.data(), b.dtout, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HCHECK(hipMemcpyAsync(S.data(), b.dst, b.nm * sizeof(State), hipMemcpyDeviceToHost, s));
   HCHECK(hipStreamSynchronize(s));
+  if (ctx->prof) prof_collect(ctx);
   for (int k = 0; k < b.nm; ++k) {
     const double eT = out[2 * k], eST = out[2 * k + 1];
     if (e_t) {

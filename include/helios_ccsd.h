@@ -207,13 +207,14 @@ int hcc_gemm_flops(hcc_ctx* ctx, double* per_iteration, double* per_solve_setup)
 int hcc_gemm_flops_correction(hcc_ctx* ctx, double* per_inner_iteration);
 /* Split of the profiled device time into 96 slots: ms96[k] for GEMM table k
  * of the iteration (0..24 in launch order), 32+k for its correction form
- * (mixed solver), 60..63 for the set-up GEMMs; then elementwise groups:
+ * (mixed solver), 60..63 for the set-up GEMMs, 58 and 59 for the (T)
+ * integral blocks; then elementwise groups:
  * 64 FP64 update (k_update1/2 with the DIIS push), 65 FP64 DIIS (dots, solve,
  * extrapolation), 66 mixed inner update, 67 mixed inner DIIS, 68 E2 pass,
  * 69 its correction form, 70 residual elementwise (dressing, Fock, P/PT, u,
  * assembly), 71 its correction form, 72 outer-point residual norms, 73 commit
  * and BF16 copies, 74 guess, 75 energy, 76 set-up elementwise, 77 iterator
- * control kernels. */
+ * control kernels, 78 the (T) kernel, 79 its set-up elementwise work. */
 int hcc_profile_ops(hcc_ctx* ctx, double* ms96);
 
 /* Iteration blocks (4 iterations of hcc_solve / hcc_batch_solve, 4 inner
