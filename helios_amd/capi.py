@@ -65,6 +65,7 @@ SIGNATURES = {
     "hcc_set_guess_network": (_i, [_vp, _P(_f), _i]),
     "hcc_set_gemm_tuning": (_i, [_vp, _i, _P(_i)]),
     "hcc_set_mixed_history": (_i, [_vp, _i]),
+    "hcc_set_mixed_diis": (_i, [_vp, _i]),
     "hcc_set_graphs": (_i, [_vp, _i]),
     "hcc_tune_gemm": (_i, [_vp, _i, _i, _i, _i, _P(_d)]),
     "hcc_gemm_shape": (_i, [_vp, _i, _i, _P(_i)]),
@@ -300,6 +301,10 @@ class Context:
     def set_mixed_history(self, keep=True):
         """Mixed solver: keep the inner DIIS history across outer points."""
         self._check(self.lib.hcc_set_mixed_history(self.h, int(keep)), "hcc_set_mixed_history")
+
+    def set_mixed_diis(self, depth=0):
+        """Mixed solver: inner DIIS depth (0: diis_steps) for batches loaded afterwards."""
+        self._check(self.lib.hcc_set_mixed_diis(self.h, int(depth)), "hcc_set_mixed_diis")
 
     def set_graphs(self, enable=True):
         self._check(self.lib.hcc_set_graphs(self.h, int(enable)), "hcc_set_graphs")

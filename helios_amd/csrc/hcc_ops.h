@@ -61,9 +61,9 @@ void mixed_update(const Mol* m0, const MolD* md, const State* st, Ragged rgOV, R
                   double* part1, double* part2, hipStream_t s);
 void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, int max_outer, Ragged rgOV,
                          const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
-void mixed_inner_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
+void mixed_inner_control(const MolD* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
                          const double* part1, Ragged rgT2, const double* part2, hipStream_t s);
-void mixed_inner_advance(const Mol* mols, State* st, int nm, hipStream_t s);
+void mixed_inner_advance(const MolD* mols, State* st, int nm, hipStream_t s);
 void set_phase(State* st, int* act, int nm, int phase, hipStream_t s);
 
 }  // namespace hcc

@@ -1112,45 +1112,93 @@ __global__ __launch_bounds__(kEwThreads) void k_diis_shift(const MolD* __restric
 
 // ---------------------------------------------------------------------------
 // The two thin contractions of the doubles residual,
-//   x[i][j][a][b] += sum_c t2[i][j][a][c] Xvv[b][c] - sum_k t2[i][k][a][b] Xoo[k][j],
-// (K = v and K = o, outputs v and o wide) as one elementwise pass for
-// molecules with o, v <= kE2Max.  kE2Split workgroups per (molecule, i), each
-// over a range of a: Xvv and Xoo stay in LDS, and for each a the slab
-// S[j][c] = t2[i][j][a][c] feeds both sums (the second reads it as S[k][b]).
-// ---------------------------------------------------------------------------
+//   x[i][j][a][b] += sum_c t2[i][j][a][c] Xvv[b][c] - sum_k t2[i][k][a][b] Xoo[k][j]
+// (K = v and K = o, outputs v and o wide), as one pass for molecules with
+// o, v <= kE2Max (the others take GEMM tables g23/g24).
+typedef double e2d4 __attribute__((ext_vector_type(4)));
+typedef float e2f4 __attribute__((ext_vector_type(4)));
+
+// E2 terms (FP64 matrix cores), molecules with o, v <= kE2Max; rgOcc:
+// kE2Split workgroups per (molecule, i), each over a range of a.  For each a,
+// the o x v slice x[i][:][a][:] is computed in 16x16 tiles, one per wave:
+//   x[i][j][a][b] += sum_c S[j][c] Xvv[b][c] - sum_k Xoo[k][j] S[k][b],
+// S = t2[i][:][a][:] (LDS, zero-padded to 32 x 32, like Xvv and Xoo).
+// V_MFMA_F64_16X16X4F64: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+// D[row][col] is element row / 4 of lane col + 16 (row % 4).
+constexpr int kE2PS = 34, kE2PO = 48;   // LDS pitches: [row][K] reads are bank-conflict free
 __global__ __launch_bounds__(kEwThreads) void k_e2(const Mol* __restrict__ mols, const State* __restrict__ st,
                                                   Ragged rg, int nm) {
-  __shared__ double sX[kE2Max * (kE2Max + 1)], sO[kE2Max * (kE2Max + 1)], sS[kE2Max * kE2Max];
+  __shared__ double sX[kE2Max * kE2PS], sO[kE2Max * kE2PO], sS[kE2Max * kE2PS];
   const int mol = rg.bmol[blockIdx.x];
   const int lb = blockIdx.x - rg.prefix[mol], i = lb / kE2Split, part = lb % kE2Split;
   const Mol& M = mols[mol];
   const int o = M.o, v = M.v;
   if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
   const int na = (v + kE2Split - 1) / kE2Split, a0 = part * na, a1 = min(v, a0 + na);
-  for (int q = threadIdx.x; q < v * v; q += kEwThreads) sX[(q / v) * (v + 1) + q % v] = M.Xvv[q];
-  for (int q = threadIdx.x; q < o * o; q += kEwThreads) sO[(q / o) * (o + 1) + q % o] = M.Xoo[q];
+  for (int q = threadIdx.x; q < kE2Max * kE2Max; q += kEwThreads) {
+    const int r = q / kE2Max, c = q % kE2Max;
+    sX[r * kE2PS + c] = r < v && c < v ? M.Xvv[r * v + c] : 0.0;
+    sO[r * kE2PO + c] = r < o && c < o ? M.Xoo[r * o + c] : 0.0;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+  const int BT = (v + 15) >> 4, jt = wave / BT, bt = wave % BT;
+  const bool work = wave < ((o + 15) >> 4) * BT;
+  const int jr = 16 * jt + l16, bc = 16 * bt + l16;
+  // the next a's slice and x tile are loaded into registers while the current
+  // one is computed, so no global read waits at the barriers
+  constexpr int kSl = kE2Max * kE2Max / kEwThreads;
+  const gptr<double> t2 = global_view(M.t2), xin = global_view(M.x);
+  double ns[kSl], nx[4];
+  auto fetch = [&](int a) {
+#pragma unroll
+    for (int r = 0; r < kSl; ++r) {
+      const int q = threadIdx.x + r * kEwThreads, j = q / kE2Max, c = q % kE2Max;
+      const bool in = j < o && c < v;
+      ns[r] = in ? t2[(((long)i * o + j) * v + a) * v + c] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = 16 * jt + kq + 4 * r;
+      nx[r] = work && bc < v && j < o ? xin[(((long)i * o + j) * v + a) * v + bc] : 0.0;
+    }
+  };
+  if (a0 < a1) fetch(a0);
   for (int a = a0; a < a1; ++a) {
     __syncthreads();
-    for (int q = threadIdx.x; q < o * v; q += kEwThreads)
-      sS[q] = M.t2[(((long)i * o + q / v) * v + a) * v + q % v];
-    __syncthreads();
-    for (int q = threadIdx.x; q < o * v; q += kEwThreads) {
-      const int j = q / v, b = q % v;
-      double s = 0.0;
-      for (int c = 0; c < v; ++c) s += sS[j * v + c] * sX[b * (v + 1) + c];
-      for (int k = 0; k < o; ++k) s -= sS[k * v + b] * sO[k * (o + 1) + j];
-      M.x[(((long)i * o + j) * v + a) * v + b] += s;
+#pragma unroll
+    for (int r = 0; r < kSl; ++r) {
+      const int q = threadIdx.x + r * kEwThreads;
+      sS[(q / kE2Max) * kE2PS + q % kE2Max] = ns[r];
     }
+    double xc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xc[r] = nx[r];
+    __syncthreads();
+    if (a + 1 < a1) fetch(a + 1);
+    if (!work) continue;
+    e2d4 acc = e2d4{0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < v; c0 += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sS[jr * kE2PS + c0 + kq], sX[bc * kE2PS + c0 + kq], acc, 0, 0, 0);
+    for (int k0 = 0; k0 < o; k0 += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-sO[(k0 + kq) * kE2PO + jr], sS[(k0 + kq) * kE2PS + bc], acc, 0, 0, 0);
+    if (bc < v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = 16 * jt + kq + 4 * r;
+        if (j < o) M.x[(((long)i * o + j) * v + a) * v + bc] = xc[r] + acc[r];
+      }
   }
 }
 
-// Correction form of k_e2 (FP32): with X1 = X0 + dX,
+// Correction form of k_e2 (FP32 matrix cores): with X1 = X0 + dX,
 //   dx += sum_c (t2_0 dXvv + dt2 Xvv1) - sum_k (t2_0 dXoo + dt2 Xoo1)
-// -- the exact change of the FP64 terms up to FP32 rounding.
+// -- the exact change of the FP64 terms up to FP32 rounding.  Tiles as in
+// k_e2; V_MFMA_F32_16X16X4F32 holds A and B as the FP64 instruction does and
+// leaves D[row][col] in element row % 4 of lane col + 16 (row / 4).
 __global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mols, const MolD* __restrict__ dmols,
                                                     const State* __restrict__ st, Ragged rg, int nm) {
-  constexpr int kP = kE2Max * (kE2Max + 1);
-  __shared__ float sXd[kP], sX1[kP], sOd[kP], sO1[kP], sS0[kE2Max * kE2Max], sSd[kE2Max * kE2Max];
+  __shared__ float sXd[kE2Max * kE2PS], sX1[kE2Max * kE2PS], sOd[kE2Max * kE2PO], sO1[kE2Max * kE2PO];
+  __shared__ float sS0[kE2Max * kE2PS], sSd[kE2Max * kE2PS];
   const int mol = rg.bmol[blockIdx.x];
   const int lb = blockIdx.x - rg.prefix[mol], i = lb / kE2Split, part = lb % kE2Split;
   const Mol& M = mols[mol];
@@ -1158,57 +1206,72 @@ __global__ __launch_bounds__(kEwThreads) void k_e2_d(const Mol* __restrict__ mol
   const int o = M.o, v = M.v;
   if (!st[mol].active || o > kE2Max || v > kE2Max || i >= o) return;
   const int na = (v + kE2Split - 1) / kE2Split, a0 = part * na, a1 = min(v, a0 + na);
-  for (int q = threadIdx.x; q < v * v; q += kEwThreads) {
-    const int r = (q / v) * (v + 1) + q % v;
-    sXd[r] = D.Xvv[q];
-    sX1[r] = (float)(M.Xvv[q] + (double)D.Xvv[q]);
-  }
-  for (int q = threadIdx.x; q < o * o; q += kEwThreads) {
-    const int r = (q / o) * (o + 1) + q % o;
-    sOd[r] = D.Xoo[q];
-    sO1[r] = (float)(M.Xoo[q] + (double)D.Xoo[q]);
+  for (int q = threadIdx.x; q < kE2Max * kE2Max; q += kEwThreads) {
+    const int r = q / kE2Max, c = q % kE2Max;
+    const bool xv = r < v && c < v, xo = r < o && c < o;
+    sXd[r * kE2PS + c] = xv ? D.Xvv[r * v + c] : 0.f;
+    sX1[r * kE2PS + c] = xv ? (float)(M.Xvv[r * v + c] + (double)D.Xvv[r * v + c]) : 0.f;
+    sOd[r * kE2PO + c] = xo ? D.Xoo[r * o + c] : 0.f;
+    sO1[r * kE2PO + c] = xo ? (float)(M.Xoo[r * o + c] + (double)D.Xoo[r * o + c]) : 0.f;
   }
   // global loads (generic ones would share the LDS wait counter); the next
   // a's slice is in flight while the current one is computed
   const gptr<double> t2 = global_view(M.t2);
   const gptr<float> dt2 = global_view(D.t2), xin = global_view(D.x);
   const gwptr<float> x = global_out(D.x);
-  const int ov = o * v;
-  float n0[kE2Slots], nd[kE2Slots];
+  constexpr int kSl = kE2Max * kE2Max / kEwThreads;   // staged slice elements per thread
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+  const int BT = (v + 15) >> 4, jt = wave / BT, bt = wave % BT;
+  const bool work = wave < ((o + 15) >> 4) * BT;
+  const int jr = 16 * jt + l16, bc = 16 * bt + l16;
+  float n0[kSl], nd[kSl], nx[4];
   auto fetch = [&](int a) {
 #pragma unroll
-    for (int r = 0; r < kE2Slots; ++r) {
-      const int q = threadIdx.x + r * kEwThreads;
-      const long e = (((long)i * o + (q < ov ? q : 0) / v) * v + a) * v + (q < ov ? q : 0) % v;
-      n0[r] = (float)t2[e];
-      nd[r] = dt2[e];
+    for (int r = 0; r < kSl; ++r) {
+      const int q = threadIdx.x + r * kEwThreads, j = q / kE2Max, c = q % kE2Max;
+      const bool in = j < o && c < v;
+      const long e = (((long)i * o + (in ? j : 0)) * v + a) * v + (in ? c : 0);
+      n0[r] = in ? (float)t2[e] : 0.f;
+      nd[r] = in ? dt2[e] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = 16 * jt + 4 * kq + r;
+      nx[r] = work && bc < v && j < o ? xin[(((long)i * o + j) * v + a) * v + bc] : 0.f;
     }
   };
   if (a0 < a1) fetch(a0);
   for (int a = a0; a < a1; ++a) {
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kE2Slots; ++r) {
+    for (int r = 0; r < kSl; ++r) {
       const int q = threadIdx.x + r * kEwThreads;
-      if (q < ov) {
-        sS0[q] = n0[r];
-        sSd[q] = nd[r];
-      }
+      sS0[(q / kE2Max) * kE2PS + q % kE2Max] = n0[r];
+      sSd[(q / kE2Max) * kE2PS + q % kE2Max] = nd[r];
     }
+    float xc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xc[r] = nx[r];
     __syncthreads();
     if (a + 1 < a1) fetch(a + 1);
-#pragma unroll
-    for (int r = 0; r < kE2Slots; ++r) {
-      const int q = threadIdx.x + r * kEwThreads;
-      if (q >= ov) continue;
-      const int j = q / v, b = q % v;
-      const long ox = (((long)i * o + j) * v + a) * v + b;
-      const float x0 = xin[ox];
-      float s = 0.f;
-      for (int c = 0; c < v; ++c) s += sS0[j * v + c] * sXd[b * (v + 1) + c] + sSd[j * v + c] * sX1[b * (v + 1) + c];
-      for (int k = 0; k < o; ++k) s -= sS0[k * v + b] * sOd[k * (o + 1) + j] + sSd[k * v + b] * sO1[k * (o + 1) + j];
-      x[ox] = x0 + s;
+    if (!work) continue;
+    e2f4 acc = e2f4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < v; c0 += 4) {
+      const int c = c0 + kq;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sS0[jr * kE2PS + c], sXd[bc * kE2PS + c], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sSd[jr * kE2PS + c], sX1[bc * kE2PS + c], acc, 0, 0, 0);
     }
+    for (int k0 = 0; k0 < o; k0 += 4) {
+      const int k = k0 + kq;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-sOd[k * kE2PO + jr], sS0[k * kE2PS + bc], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-sO1[k * kE2PO + jr], sSd[k * kE2PS + bc], acc, 0, 0, 0);
+    }
+    if (bc < v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = 16 * jt + 4 * kq + r;
+        if (j < o) x[(((long)i * o + j) * v + a) * v + bc] = xc[r] + acc[r];
+      }
   }
 }
 
@@ -1348,7 +1411,7 @@ __global__ __launch_bounds__(kEwThreads) void k_outer_control(const Mol* __restr
 }
 
 // inner-iteration control: stop at acc < kappa*acc0 (or below eps/2, or at the cap)
-__global__ __launch_bounds__(kEwThreads) void k_inner_control(const Mol* __restrict__ mols, State* __restrict__ st,
+__global__ __launch_bounds__(kEwThreads) void k_inner_control(const MolD* __restrict__ mols, State* __restrict__ st,
                                                              int* __restrict__ act, int nm, Params prm,
                                                              const int* __restrict__ pre1, const double* __restrict__ part1,
                                                              const int* __restrict__ pre2, const double* __restrict__ part2) {
@@ -1384,7 +1447,7 @@ __global__ __launch_bounds__(kEwThreads) void k_inner_control(const Mol* __restr
 }
 
 // DIIS slot advance for the inner phase (k_control2 without the iterator tests)
-__global__ void k_inner_advance(const Mol* __restrict__ mols, State* __restrict__ st, int nm) {
+__global__ void k_inner_advance(const MolD* __restrict__ mols, State* __restrict__ st, int nm) {
   const int mol = blockIdx.x * blockDim.x + threadIdx.x;
   if (mol >= nm) return;
   State& S = st[mol];
@@ -1442,12 +1505,12 @@ void mixed_outer_control(const Mol* mols, State* st, int* act, int nm, const Par
   hipLaunchKernelGGL(k_outer_control, dim3(nm), dim3(kEwThreads), 0, s, mols, st, act, nm, prm, max_outer,
                      rgOV.prefix, part1, rgT2.prefix, part2);
 }
-void mixed_inner_control(const Mol* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
+void mixed_inner_control(const MolD* mols, State* st, int* act, int nm, const Params& prm, Ragged rgOV,
                          const double* part1, Ragged rgT2, const double* part2, hipStream_t s) {
   hipLaunchKernelGGL(k_inner_control, dim3(nm), dim3(kEwThreads), 0, s, mols, st, act, nm, prm, rgOV.prefix,
                      part1, rgT2.prefix, part2);
 }
-void mixed_inner_advance(const Mol* mols, State* st, int nm, hipStream_t s) {
+void mixed_inner_advance(const MolD* mols, State* st, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_inner_advance, dim3((nm + 63) / 64), dim3(64), 0, s, mols, st, nm);
 }
 void set_phase(State* st, int* act, int nm, int phase, hipStream_t s) {

@@ -122,6 +122,7 @@ struct DevBuf {
 struct Batch {
   int nm = 0;
   int m = 20;
+  int inner_m = 20;   // mixed solver: inner DIIS depth (<= m; the ring storage is sized for m)
   std::vector<int> n, o, v;
   std::vector<double> eref;
   std::vector<Mol> hm;
@@ -161,7 +162,8 @@ struct Batch {
   int *dtri = nullptr, *dtsave = nullptr;
   double *dtw = nullptr, *dtpart = nullptr, *dtout = nullptr;
   long ttotal = 0, twstride = 0;
-  int tgrid = 0;
+  int tgrid = 0, tgrid_mfma = 0;
+  bool tglobal = false;
 
   // device memory: hipMalloc per buffer, or -- when `arena` is set (the
   // no-allocation entry point) -- consecutive 256-byte aligned pieces of a
@@ -516,7 +518,7 @@ static void allocate(Batch& b, bool keep_mp2) {
       const Mol& H = b.hm[k];
       MolD& D = b.hd[k];
       memset(&D, 0, sizeof(D));
-      D.n = H.n; D.o = H.o; D.v = H.v; D.m = H.m; D.nd = H.nd; D.eref = H.eref;
+      D.n = H.n; D.o = H.o; D.v = H.v; D.m = b.inner_m; D.nd = H.nd; D.eref = H.eref;
       D.C = H.C; D.G = H.G; D.F = H.F; D.Fao = H.Fao;
       D.Gklcd = H.Gklcd; D.Lldkc = H.Lldkc; D.Lt = H.Lt;
       // FP32 DIIS vectors in the (otherwise unused) FP64 ring of the molecule
@@ -826,6 +828,7 @@ struct hcc_ctx {
   // mixed-precision solver options
   int mixed_bf16 = 1;
   int keep_diis = 1;   // mixed solver: inner DIIS history kept across outer steps
+  int mixed_diis = 0;  // mixed solver: inner DIIS depth (0: diis_steps)
   double kappa = 0.05;
   int max_inner = 5;
   int max_outer = 30;
@@ -1098,6 +1101,7 @@ static void load(hcc_ctx* c, int nmol, const int* n, const int* nocc, const doub
   b->arena_cap = c->arena_cap;
   b->nm = nmol;
   b->m = c->diis_steps;
+  b->inner_m = c->mixed_diis > 0 ? std::min(c->mixed_diis, c->diis_steps) : c->diis_steps;
   b->mixed = c->precision == HCC_MIXED;
   b->mixed_bf16 = c->mixed_bf16 != 0;
   b->tuning = c->tuning.empty() ? nullptr : &c->tuning;
@@ -1233,6 +1237,16 @@ int hcc_set_mixed_history(hcc_ctx* ctx, int keep) {
   return HCC_OK;
 }
 
+int hcc_set_mixed_diis(hcc_ctx* ctx, int depth) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (depth != 0 && (depth < 2 || depth > ctx->diis_steps)) {
+    ctx->err = "inner DIIS depth must be 0 or in [2, diis_steps]";
+    return HCC_E_ARG;
+  }
+  ctx->mixed_diis = depth;
+  return HCC_OK;
+}
+
 int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_outer) {
   if (ctx == nullptr) return HCC_E_ARG;
   if (!(kappa > 0.0 && kappa < 1.0) || max_inner < 1 || max_outer < 1) {
@@ -1357,18 +1371,17 @@ int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, c
 static int need_batch(hcc_ctx* ctx);
 
 // Buffers, GEMM tables and work items of the (T) correction (once per batch).
-static void setup_triples(Batch& b) {
+static void setup_triples(Batch& b, int device) {
   const int nm = b.nm;
   b.tA.assign(nm, nullptr);
   b.tB.assign(nm, nullptr);
-  int omax = 0, vmax = 0;
+  int omax = 0;
   std::vector<long> prefix(nm + 1, 0);
   for (int k = 0; k < nm; ++k) {
     const long o = b.o[k], v = b.v[k];
     b.tA[k] = b.dalloc<double>(o * v * v * v);
     b.tB[k] = b.dalloc<double>(o * o * o * v);
     omax = std::max(omax, b.o[k]);
-    vmax = std::max(vmax, b.v[k]);
     prefix[k + 1] = prefix[k] + o * (o + 1) * (o + 2) / 6;
   }
   b.tpre.clear();
@@ -1396,8 +1409,20 @@ static void setup_triples(Batch& b) {
     tm[k] = TripMol{M.n, M.o, M.v, 0, b.tA[k], b.tB[k], M.Gklcd, M.t1, M.t2, M.F};
   }
   b.ttotal = prefix[nm];
-  b.twstride = ((long)vmax * vmax * vmax + 31) / 32 * 32;
-  b.tgrid = (int)std::max<long>(1, std::min<long>(b.ttotal, 1024));
+  // molecules with v <= kTripLdsV go to the matrix-core kernel (W in LDS, one
+  // workgroup per CU); the others keep W in a global block per workgroup
+  int vglob = 0;
+  bool small = false;
+  for (int k = 0; k < nm; ++k) {
+    if (b.v[k] > kTripLdsV) vglob = std::max(vglob, b.v[k]);
+    else small = true;
+  }
+  b.tglobal = vglob > 0;
+  int cus = 0;
+  HCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  b.tgrid_mfma = small ? std::max(1, cus) : 0;
+  b.twstride = ((long)vglob * vglob * vglob + 31) / 32 * 32;
+  b.tgrid = b.tglobal ? (int)std::max<long>(1, std::min<long>(b.ttotal, 1024)) : 1;
   b.dtm = b.dalloc<TripMol>(nm);
   b.dtprefix = b.dalloc<long>(nm + 1);
   b.dtri = b.dalloc<int>(tri.size());
@@ -1426,7 +1451,7 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  if (b.dtm == nullptr) setup_triples(b);
+  if (b.dtm == nullptr) setup_triples(b, ctx->device);
   // undressed orbital products (t1 = 0), with t1 kept in w1 meanwhile: the
   // t1 and w1 arenas have the same per-molecule offsets
   const Mol& first = b.hm[0];
@@ -1446,7 +1471,8 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
   HCHECK(hipMemcpyAsync(first.t1, first.w1, span * sizeof(double), hipMemcpyDeviceToDevice, s));
   prof_close(ctx);
   prof_open(ctx, kProfTriples);
-  HCHECK(launch_triples(b.dtm, b.nm, b.dtprefix, b.ttotal, b.dtri, b.dtw, b.twstride, b.tgrid, b.dtpart, b.dtout, s));
+  HCHECK(launch_triples(b.dtm, b.nm, b.dtprefix, b.ttotal, b.dtri, b.dtw, b.twstride, b.tgrid, b.tgrid_mfma, b.tglobal,
+                        b.dtpart, b.dtout, s));
   prof_close(ctx);
   prof_open(ctx, kProfTriplesEw);
   arm_all(b.dst, b.dact, b.dtsave, b.nm, true, s);
@@ -1801,12 +1827,12 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm, b.part1, b.part2, s);
     prof_close(ctx);
     prof_open(ctx, kProfControl);
-    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
+    mixed_inner_control(b.ddm, b.dst, b.dact, b.nm, prm, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     prof_close(ctx);
     if (prm.diis) {
       prof_open(ctx, kProfDiisMixed);
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
-      mixed_inner_advance(b.dm, b.dst, b.nm, s);
+      mixed_inner_advance(b.ddm, b.dst, b.nm, s);
       prof_close(ctx);
     }
   };
@@ -1820,12 +1846,12 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     mixed_update(b.dm, b.ddm, b.dst, b.rgOV, b.rgT2x, b.nm, prm1, b.part1, b.part2, s);
     prof_close(ctx);
     prof_open(ctx, kProfControl);
-    mixed_inner_control(b.dm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2x, b.part2, s);
+    mixed_inner_control(b.ddm, b.dst, b.dact, b.nm, prm1, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     prof_close(ctx);
     if (prm1.diis) {
       prof_open(ctx, kProfDiisMixed);
       diis_step(b.ddm, b.dst, b.nm, b.rgND, b.rgP2, b.partD, s);
-      mixed_inner_advance(b.dm, b.dst, b.nm, s);
+      mixed_inner_advance(b.ddm, b.dst, b.nm, s);
       prof_close(ctx);
     }
   };

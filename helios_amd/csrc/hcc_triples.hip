@@ -59,6 +59,7 @@ __global__ __launch_bounds__(kTripThreads) void k_triples(const TripMol* __restr
     }
     const TripMol& M = mols[lo];
     const int o = M.o, v = M.v, n = M.n;
+    if (v <= kTripLdsV) continue;   // the matrix-core kernel takes it
     const int packed = tri[item - prefix[lo]];
     const int occ[3] = {packed & 0xff, (packed >> 8) & 0xff, (packed >> 16) & 0xff};   // i >= j >= k
     for (int s = 0; s < 6; ++s) {
@@ -135,6 +136,163 @@ __global__ __launch_bounds__(kTripThreads) void k_triples(const TripMol* __restr
   }
 }
 
+// Items of molecules with v <= kTripLdsV on the matrix cores, W resident in
+// LDS.  For permutation s the product Y[(x,y)][z] = sum_d A[p][x][y][d]
+// t2[r][q][z][d] - sum_l t2[p][l][x][y] (ql|rz) is a (v^2 x v) x (v + o)
+// FP64 MFMA product: A fragments stream from memory (rows (x,y), K = d
+// contiguous; the t2 rows are contiguous along (x,y)), the B operands
+// (t2[r][q][z][d] as [d][z], (ql|rz) as [l][z], zero-padded to 32 x 32) sit in
+// LDS; each 16 x 16 tile is added into W at the permuted positions.
+// V_MFMA_F64_16X16X4F64: lane t holds A[t % 16][t / 16] and B[t / 16][t % 16];
+// D[row][col] is element row / 4 of lane col + 16 (row % 4).
+typedef double trip_d4 __attribute__((ext_vector_type(4)));
+constexpr int kTripMfmaThreads = 512;
+constexpr int kTripKD = (kTripLdsV + 3) / 4;   // K steps over d
+constexpr int kTripKL = kTripMaxOV / 4;        // K steps over l
+// LDS (doubles): W[v^3], B operands 2 x 32 x 32 (reused for the L slices of
+// the energy pass), orbital energies and t1 rows, reduction
+constexpr int kTripLdsW = kTripLdsV * kTripLdsV * kTripLdsV;
+constexpr int kTripLdsE = 2 * kTripMaxOV + 3 * kTripLdsV;
+
+__global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol* __restrict__ mols, int nm,
+                                                                   const long* __restrict__ prefix, long total,
+                                                                   const int* __restrict__ tri,
+                                                                   double* __restrict__ part) {
+  extern __shared__ double lds[];
+  double* W = lds;
+  double* Tb = W + kTripLdsW;                 // [d][z], 32 x 32
+  double* Qb = Tb + 32 * 32;                  // [l][z], 32 x 32
+  double* E = Qb + 32 * 32;                   // e_occ (o), e_vir (v), t1 rows of i, j, k
+  double* red = E + kTripLdsE;                // kTripMfmaThreads / 64
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+  constexpr int nwave = kTripMfmaThreads / 64;
+  for (long item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const TripMol& M = mols[lo];
+    const int o = M.o, v = M.v, n = M.n;
+    if (v > kTripLdsV) continue;   // uniform over the workgroup: the global-W kernel takes it
+    const int packed = tri[item - prefix[lo]];
+    const int occ[3] = {packed & 0xff, (packed >> 8) & 0xff, (packed >> 16) & 0xff};
+    const int vv = v * v, rt_n = (vv + 15) >> 4, ct_n = (v + 15) >> 4;
+    for (int s = 0; s < 6; ++s) {
+      const int p = occ[kPerm[s][0]], q = occ[kPerm[s][1]], r = occ[kPerm[s][2]];
+      const int stride3[3] = {vv, v, 1};
+      const int sx = stride3[kPerm[s][0]], sy = stride3[kPerm[s][1]], sz = stride3[kPerm[s][2]];
+      __syncthreads();
+      for (int e = threadIdx.x; e < 32 * 32; e += kTripMfmaThreads) {
+        const int row = e >> 5, z = e & 31;
+        Tb[e] = row < v && z < v ? M.t2[(((long)r * o + q) * v + z) * v + row] : 0.0;     // t_rq^{z d}, d = row
+        Qb[e] = row < o && z < v ? M.B[(((long)q * o + row) * o + r) * v + z] : 0.0;      // (ql|rz), l = row
+      }
+      __syncthreads();
+      const double* Ap = M.A + (long)p * vv * v;     // [(x,y)][d]
+      const double* Up = M.t2 + (long)p * o * vv;    // [l][(x,y)]
+      for (int rt = wave; rt < rt_n; rt += nwave) {
+        const int row = rt * 16 + l16;
+        const bool rin = row < vv;
+        double fa[kTripKD], fu[kTripKL];
+#pragma unroll
+        for (int k = 0; k < kTripKD; ++k) {
+          const int d = 4 * k + kq;
+          fa[k] = rin && d < v ? Ap[(long)row * v + d] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kTripKL; ++k) {
+          const int l = 4 * k + kq;
+          fu[k] = rin && l < o ? -Up[(long)l * vv + row] : 0.0;
+        }
+        trip_d4 acc[2];
+        acc[0] = trip_d4{0.0, 0.0, 0.0, 0.0};
+        acc[1] = trip_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < kTripKD; ++k) {
+          const int d = 4 * k + kq;
+          acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + l16], acc[0], 0, 0, 0);
+          if (ct_n > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + 16 + l16], acc[1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kTripKL; ++k) {
+          if (4 * k >= o) break;
+          const int l = 4 * k + kq;
+          acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + l16], acc[0], 0, 0, 0);
+          if (ct_n > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + 16 + l16], acc[1], 0, 0, 0);
+        }
+        // the tile's rows (x, y) and columns z, added at the permuted position
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const int z = 16 * ct + l16;
+          if (ct >= ct_n || z >= v) continue;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int rr = rt * 16 + kq + 4 * e;
+            if (rr >= vv) continue;
+            const int idx = (rr / v) * sx + (rr % v) * sy + z * sz;
+            W[idx] = (s == 0 ? 0.0 : W[idx]) + acc[ct][e];
+          }
+        }
+      }
+    }
+    // energy of the item: L slices (jb|kc), (ia|kc), (ia|jb) and the t1 rows in LDS
+    const int i = occ[0], j = occ[1], k = occ[2];
+    __syncthreads();
+    double* Ljk = Tb;
+    double* Lik = Tb + vv;
+    double* Lij = Tb + 2 * vv;   // 3 v^2 <= 2 x 32 x 32 for v <= 26
+    for (int e = threadIdx.x; e < vv; e += kTripMfmaThreads) {
+      Ljk[e] = M.L[((long)j * o + k) * vv + e];
+      Lik[e] = M.L[((long)i * o + k) * vv + e];
+      Lij[e] = M.L[((long)i * o + j) * vv + e];
+    }
+    double* ev = E;
+    double* t1i = E + v;
+    double* t1j = t1i + v;
+    double* t1k = t1j + v;
+    for (int e = threadIdx.x; e < v; e += kTripMfmaThreads) {
+      ev[e] = M.F[(long)(o + e) * n + o + e];
+      t1i[e] = M.t1[(long)i * v + e];
+      t1j[e] = M.t1[(long)j * v + e];
+      t1k[e] = M.t1[(long)k * v + e];
+    }
+    __syncthreads();
+    const double eijk = M.F[(long)i * n + i] + M.F[(long)j * n + j] + M.F[(long)k * n + k];
+    double eT = 0.0, eST = 0.0;
+    for (int e = threadIdx.x; e < vv * v; e += kTripMfmaThreads) {
+      const int a = e / vv, b = (e / v) % v, c = e % v;
+      const double Wabc = W[e];
+      const double r3 = 4.0 * Wabc + W[(b * v + c) * v + a] + W[(c * v + a) * v + b] -
+                        2.0 * (W[(a * v + c) * v + b] + W[(b * v + a) * v + c] + W[(c * v + b) * v + a]);
+      const double D = eijk - ev[a] - ev[b] - ev[c];
+      const double V = t1i[a] * Ljk[b * v + c] + t1j[b] * Lik[a * v + c] + t1k[c] * Lij[a * v + b];
+      eT += Wabc * r3 / (3.0 * D);
+      eST += V * r3 / (3.0 * D);
+    }
+    // block sums (fixed order)
+    for (int off = 32; off > 0; off >>= 1) {
+      eT += __shfl_xor(eT, off);
+      eST += __shfl_xor(eST, off);
+    }
+    if (lane == 0) {
+      red[wave] = eT;
+      red[nwave + wave] = eST;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0, st = 0.0;
+      for (int w = 0; w < nwave; ++w) {
+        t += red[w];
+        st += red[nwave + w];
+      }
+      const double m = (i > j && j > k) ? 6.0 : ((i == j && j == k) ? 1.0 : 3.0);
+      part[2 * item] = m * t;
+      part[2 * item + 1] = m * st;
+    }
+  }
+}
+
 // per molecule, in item order: out[2 m] = E[T], out[2 m + 1] = E[ST]
 __global__ void k_triples_final(int nm, const long* __restrict__ prefix, const double* __restrict__ part,
                                 double* __restrict__ out) {
@@ -150,10 +308,21 @@ __global__ void k_triples_final(int nm, const long* __restrict__ prefix, const d
 }
 
 size_t triples_lds_bytes() { return sizeof(double) * (kTripThreads / 64 + 2 * kTripMaxV * kTripMaxV); }
+size_t triples_mfma_lds_bytes() {
+  return sizeof(double) * (kTripLdsW + 2 * 32 * 32 + kTripLdsE + 2 * (kTripMfmaThreads / 64));
+}
 
 hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int* tri,
-                          double* wscratch, long wstride, int grid, double* part, double* out, hipStream_t s) {
-  if (total > 0)
+                          double* wscratch, long wstride, int grid, int grid_mfma, bool any_global, double* part,
+                          double* out, hipStream_t s) {
+  if (total > 0 && grid_mfma > 0) {
+    const hipError_t e = hipFuncSetAttribute((const void*)k_triples_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)triples_mfma_lds_bytes());
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_triples_mfma, dim3(grid_mfma), dim3(kTripMfmaThreads), triples_mfma_lds_bytes(), s, mols, nm,
+                       prefix, total, tri, part);
+  }
+  if (total > 0 && any_global)
     hipLaunchKernelGGL(k_triples, dim3(grid), dim3(kTripThreads), triples_lds_bytes(), s, mols, nm, prefix, total, tri,
                        wscratch, wstride, part);
   hipLaunchKernelGGL(k_triples_final, dim3((nm + 63) / 64), dim3(64), 0, s, nm, prefix, part, out);

@@ -91,6 +91,12 @@ int hcc_set_mixed(hcc_ctx* ctx, int bf16, double kappa, int max_inner, int max_o
  * BF16 error of the correction.  No reference counterpart. */
 int hcc_set_mixed_history(hcc_ctx* ctx, int keep);
 
+/* Mixed solver: depth of the inner DIIS (0, default: the context's
+ * diis_steps; otherwise 2 .. diis_steps), for batches loaded afterwards.
+ * The inner history holds FP32 corrections; a shorter history reads fewer
+ * of them per inner iteration.  No reference counterpart. */
+int hcc_set_mixed_diis(hcc_ctx* ctx, int depth);
+
 /* Per molecule of the last mixed-precision batch solve: FP64 residual
  * evaluations (outer points) and inner iterations.  The first inner
  * iteration after each outer point takes its step from the FP64 residual

@@ -100,3 +100,24 @@ def test_rejects_more_than_32_occupied():
     with pytest.raises(capi.HccError, match="up to 32 occupied"):
         ctx.batch_triples()
     ctx.close()
+
+
+def test_batch_across_both_kernels():
+    """Molecules with v <= 24 take the matrix-core kernel (W in LDS), larger v
+    the global-W kernel; in one batch each skips the other's work items."""
+    mols = [M.octatetraene(), M.random_fragment(52, seed=5), M.batch_member(8)]
+    preps = [prepare(m) for m in mols]
+    assert preps[1]["n"] - preps[1]["nocc"] == 26
+    ctx = capi.Context(0, capi.HCC_FP64, 20)
+    ctx.batch_load(preps)
+    ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+    _, et = ctx.batch_triples()
+    ctx.close()
+    for k, p in enumerate(preps):
+        one = capi.Context(0, capi.HCC_FP64, 20)
+        one.batch_load([p])
+        one.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+        _, et1 = one.batch_triples()
+        one.close()
+        print(mols[k].name, et[k], et1[0])
+        assert np.abs(et1[0] - et[k]).max() < 1e-10 * abs(et[k, 0])
