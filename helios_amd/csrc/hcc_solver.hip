@@ -159,7 +159,8 @@ struct Batch {
   std::vector<GemmTable> tpre;
   TripMol* dtm = nullptr;
   long* dtprefix = nullptr;
-  int *dtri = nullptr, *dtsave = nullptr;
+  int2* dtri = nullptr;
+  int* dtsave = nullptr;
   double *dtw = nullptr, *dtpart = nullptr, *dtout = nullptr;
   long ttotal = 0, twstride = 0;
   int tgrid = 0, tgrid_mfma = 0;
@@ -1375,13 +1376,11 @@ static void setup_triples(Batch& b, int device) {
   const int nm = b.nm;
   b.tA.assign(nm, nullptr);
   b.tB.assign(nm, nullptr);
-  int omax = 0;
   std::vector<long> prefix(nm + 1, 0);
   for (int k = 0; k < nm; ++k) {
     const long o = b.o[k], v = b.v[k];
     b.tA[k] = b.dalloc<double>(o * v * v * v);
     b.tB[k] = b.dalloc<double>(o * o * o * v);
-    omax = std::max(omax, b.o[k]);
     prefix[k + 1] = prefix[k] + o * (o + 1) * (o + 2) / 6;
   }
   b.tpre.clear();
@@ -1399,10 +1398,13 @@ static void setup_triples(Batch& b, int device) {
     return prob(cmp(M.P, o, N, 1, 1, N2, 0), cmp(M.Q + o, 1, N2, 0, v, N, 1), rm(b.tB[k], o * v), o * o, o * v, N,
                 1.0, 0.0);
   }));
-  std::vector<int> tri;
-  for (int i = 0; i < omax; ++i)
-    for (int j = 0; j <= i; ++j)
-      for (int k = 0; k <= j; ++k) tri.push_back(i | (j << 8) | (k << 16));
+  // work items: molecule and occupied triple i >= j >= k (i ascending)
+  std::vector<int2> items;
+  items.reserve(prefix[nm]);
+  for (int m = 0; m < nm; ++m)
+    for (int i = 0; i < b.o[m]; ++i)
+      for (int j = 0; j <= i; ++j)
+        for (int k = 0; k <= j; ++k) items.push_back(int2{m, i | (j << 8) | (k << 16)});
   std::vector<TripMol> tm(nm);
   for (int k = 0; k < nm; ++k) {
     const Mol& M = b.hm[k];
@@ -1425,14 +1427,14 @@ static void setup_triples(Batch& b, int device) {
   b.tgrid = b.tglobal ? (int)std::max<long>(1, std::min<long>(b.ttotal, 1024)) : 1;
   b.dtm = b.dalloc<TripMol>(nm);
   b.dtprefix = b.dalloc<long>(nm + 1);
-  b.dtri = b.dalloc<int>(tri.size());
+  b.dtri = b.dalloc<int2>(items.size());
   b.dtw = b.dalloc<double>((size_t)b.tgrid * b.twstride);
   b.dtpart = b.dalloc<double>(2 * std::max<long>(1, b.ttotal));
   b.dtout = b.dalloc<double>(2 * (long)nm);
   b.dtsave = b.dalloc<int>(nm);
   HCHECK(hipMemcpy(b.dtm, tm.data(), nm * sizeof(TripMol), hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(b.dtprefix, prefix.data(), (nm + 1) * sizeof(long), hipMemcpyHostToDevice));
-  HCHECK(hipMemcpy(b.dtri, tri.data(), tri.size() * sizeof(int), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(b.dtri, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
 }
 
 int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {

@@ -21,13 +21,14 @@ constexpr int kTripLdsV = 24;    // v <= 24: matrix-core kernel with W in LDS (e
 
 size_t triples_lds_bytes();
 size_t triples_mfma_lds_bytes();
-// prefix: nm + 1 item offsets (items of molecule m: its occupied triples
-// i >= j >= k in the order of tri, packed i | j << 8 | k << 16); wscratch:
+// prefix: nm + 1 item offsets; items: per work item (molecule, occupied
+// triple i >= j >= k packed as i | j << 8 | k << 16), a molecule's items
+// consecutive from prefix[m]; wscratch:
 // grid blocks of wstride >= v^3 doubles (v > kTripLdsV); grid_mfma: workgroups
 // of the matrix-core kernel (0: no molecule with v <= kTripLdsV); any_global:
 // some molecule has v > kTripLdsV; part: 2 per item; out: 2 per molecule
 // (E[T], E[ST])
-hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int* tri,
+hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int2* items,
                           double* wscratch, long wstride, int grid, int grid_mfma, bool any_global, double* part,
                           double* out, hipStream_t s);
 

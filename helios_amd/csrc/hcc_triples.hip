@@ -42,8 +42,8 @@ __device__ double trip_block_sum(double v, double* red) {
 }
 
 __global__ __launch_bounds__(kTripThreads) void k_triples(const TripMol* __restrict__ mols, int nm,
-                                                          const long* __restrict__ prefix, long total,
-                                                          const int* __restrict__ tri, double* __restrict__ wscratch,
+                                                          long total, const int2* __restrict__ items,
+                                                          double* __restrict__ wscratch,
                                                           long wstride, double* __restrict__ part) {
   extern __shared__ double lds[];
   double* red = lds;                      // kTripThreads / 64
@@ -51,16 +51,11 @@ __global__ __launch_bounds__(kTripThreads) void k_triples(const TripMol* __restr
   double* Qs = Ts + kTripMaxV * kTripMaxV;   // [l][z], o x v
   double* W = wscratch + (long)blockIdx.x * wstride;
   for (long item = blockIdx.x; item < total; item += gridDim.x) {
-    // molecule of the item (prefix: nm + 1 entries)
-    int lo = 0, hi = nm - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const TripMol& M = mols[lo];
+    const int2 info = items[item];   // molecule, packed triple
+    const TripMol& M = mols[info.x];
     const int o = M.o, v = M.v, n = M.n;
     if (v <= kTripLdsV) continue;   // the matrix-core kernel takes it
-    const int packed = tri[item - prefix[lo]];
+    const int packed = info.y;
     const int occ[3] = {packed & 0xff, (packed >> 8) & 0xff, (packed >> 16) & 0xff};   // i >= j >= k
     for (int s = 0; s < 6; ++s) {
       // (p, q, r) = permuted (i, j, k); the output index (x, y, z) is the same
@@ -149,126 +144,208 @@ typedef double trip_d4 __attribute__((ext_vector_type(4)));
 constexpr int kTripMfmaThreads = 512;
 constexpr int kTripKD = (kTripLdsV + 3) / 4;   // K steps over d
 constexpr int kTripKL = kTripMaxOV / 4;        // K steps over l
-// LDS (doubles): W[v^3], B operands 2 x 32 x 32 (reused for the L slices of
+// LDS (doubles): W[v^3], B operands 4 x 32 x 32 (reused for the L slices of
 // the energy pass), orbital energies and t1 rows, reduction
-constexpr int kTripLdsW = kTripLdsV * kTripLdsV * kTripLdsV;
-constexpr int kTripLdsE = 2 * kTripMaxOV + 3 * kTripLdsV;
+// W[a][b][c] at a S2 + b S1 + c with S1 = v + 1 and S2 = v S1 + 1 (both odd):
+// accesses along any one index are free of LDS bank conflicts
+constexpr int kTripLdsW = kTripLdsV * (kTripLdsV * (kTripLdsV + 1) + 1);
+constexpr int kTripLdsE = 4 * kTripLdsV;
 
 __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol* __restrict__ mols, int nm,
-                                                                   const long* __restrict__ prefix, long total,
-                                                                   const int* __restrict__ tri,
+                                                                   long total, const int2* __restrict__ items,
                                                                    double* __restrict__ part) {
   extern __shared__ double lds[];
   double* W = lds;
-  double* Tb = W + kTripLdsW;                 // [d][z], 32 x 32
+  double* Tb = W + kTripLdsW;                 // [d][z], 32 x 32, permutation 2h
   double* Qb = Tb + 32 * 32;                  // [l][z], 32 x 32
-  double* E = Qb + 32 * 32;                   // e_occ (o), e_vir (v), t1 rows of i, j, k
+  double* Tb2 = Qb + 32 * 32;                 // the same for permutation 2h + 1
+  double* Qb2 = Tb2 + 32 * 32;
+  double* E = Qb2 + 32 * 32;                  // e_vir (v), t1 rows of i, j, k
   double* red = E + kTripLdsE;                // kTripMfmaThreads / 64
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
   constexpr int nwave = kTripMfmaThreads / 64;
+  int2 next = blockIdx.x < total ? items[blockIdx.x] : int2{0, 0};
   for (long item = blockIdx.x; item < total; item += gridDim.x) {
-    int lo = 0, hi = nm - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const TripMol& M = mols[lo];
+    const int2 info = next;   // molecule, packed triple; the next item's is loaded now
+    if (item + gridDim.x < total) next = items[item + gridDim.x];
+    const TripMol& M = mols[info.x];
     const int o = M.o, v = M.v, n = M.n;
     if (v > kTripLdsV) continue;   // uniform over the workgroup: the global-W kernel takes it
-    const int packed = tri[item - prefix[lo]];
+    const int packed = info.y;
     const int occ[3] = {packed & 0xff, (packed >> 8) & 0xff, (packed >> 16) & 0xff};
     const int vv = v * v, rt_n = (vv + 15) >> 4, ct_n = (v + 15) >> 4;
-    for (int s = 0; s < 6; ++s) {
-      const int p = occ[kPerm[s][0]], q = occ[kPerm[s][1]], r = occ[kPerm[s][2]];
-      const int stride3[3] = {vv, v, 1};
+    const int S1 = v + 1, S2 = v * S1 + 1;
+    for (int e = threadIdx.x; e < v * S2; e += kTripMfmaThreads) W[e] = 0.0;
+    const int i = occ[0], j = occ[1], k = occ[2];
+    // operands of the next phase, loaded into registers while the current
+    // phase runs: the B operands of pair h (2 elements of each 32 x 32 array
+    // per thread), then the L slices, virtual orbital energies and t1 rows
+    constexpr int kStg = 32 * 32 / kTripMfmaThreads;   // elements per 32 x 32 array and thread
+    static_assert(kStg * kTripMfmaThreads == 32 * 32, "staging covers each array exactly");
+    static_assert(3 * kStg * kTripMfmaThreads >= 3 * kTripLdsV * kTripLdsV, "staging covers the L slices");
+    static_assert(kTripMfmaThreads >= 4 * kTripLdsV, "one energy-pass vector element per thread");
+    double stg[4 * kStg];
+    auto fetch_b = [&](int h) {
+      const int q = occ[kPerm[2 * h][1]], r = occ[kPerm[2 * h][2]];
+#pragma unroll
+      for (int u = 0; u < kStg; ++u) {
+        const int e = threadIdx.x + u * kTripMfmaThreads, row = e >> 5, z = e & 31;
+        const bool dz = row < v && z < v, lz = row < o && z < v;
+        stg[u] = dz ? M.t2[(((long)r * o + q) * v + z) * v + row] : 0.0;                // t_rq^{z d}, d = row
+        stg[kStg + u] = dz ? M.t2[(((long)q * o + r) * v + z) * v + row] : 0.0;         // t_qr^{z d}
+        stg[2 * kStg + u] = lz ? M.B[(((long)q * o + row) * o + r) * v + z] : 0.0;      // (ql|rz), l = row
+        stg[3 * kStg + u] = lz ? M.B[(((long)r * o + row) * o + q) * v + z] : 0.0;      // (rl|qz)
+      }
+    };
+    // energy-pass data: element e of [Ljk | Lik | Lij] (3 v^2 <= 4 kStg x threads)
+    // and of [e_vir | t1_i | t1_j | t1_k] (4 v <= threads)
+    auto fetch_e = [&]() {
+#pragma unroll
+      for (int u = 0; u < 3 * kStg; ++u) {
+        const int e = threadIdx.x + u * kTripMfmaThreads, w = e / vv, f = e - w * vv;
+        const long pair = w == 0 ? (long)j * o + k : (w == 1 ? (long)i * o + k : (long)i * o + j);
+        stg[u] = w < 3 ? M.L[pair * vv + f] : 0.0;
+      }
+      const int w = threadIdx.x / v, f = threadIdx.x - w * v;
+      const int row = w == 1 ? i : (w == 2 ? j : k);
+      stg[3 * kStg] = w == 0 ? M.F[(long)(o + f) * n + o + f] : (w < 4 ? M.t1[(long)row * v + f] : 0.0);
+    };
+    fetch_b(0);
+    // permutations 2h and 2h + 1 share p (hence the A fragments) and swap q, r;
+    // the second one's output for (x, y, z) lands where the first one's lands
+    // for (x, z, y): each tile adds the first, then (after a barrier) the second
+    for (int h = 0; h < 3; ++h) {
+      const int s = 2 * h;
+      const int p = occ[kPerm[s][0]];
+      const int stride3[3] = {S2, S1, 1};
       const int sx = stride3[kPerm[s][0]], sy = stride3[kPerm[s][1]], sz = stride3[kPerm[s][2]];
       __syncthreads();
-      for (int e = threadIdx.x; e < 32 * 32; e += kTripMfmaThreads) {
-        const int row = e >> 5, z = e & 31;
-        Tb[e] = row < v && z < v ? M.t2[(((long)r * o + q) * v + z) * v + row] : 0.0;     // t_rq^{z d}, d = row
-        Qb[e] = row < o && z < v ? M.B[(((long)q * o + row) * o + r) * v + z] : 0.0;      // (ql|rz), l = row
+#pragma unroll
+      for (int u = 0; u < kStg; ++u) {
+        const int e = threadIdx.x + u * kTripMfmaThreads;
+        Tb[e] = stg[u];
+        Tb2[e] = stg[kStg + u];
+        Qb[e] = stg[2 * kStg + u];
+        Qb2[e] = stg[3 * kStg + u];
       }
       __syncthreads();
+      if (h < 2) fetch_b(h + 1);
+      else fetch_e();
       const double* Ap = M.A + (long)p * vv * v;     // [(x,y)][d]
       const double* Up = M.t2 + (long)p * o * vv;    // [l][(x,y)]
-      for (int rt = wave; rt < rt_n; rt += nwave) {
+      // A fragments of the next tile are prefetched; the t2 rows (contiguous
+      // along (x, y)) are loaded at the tile
+      double fa[kTripKD], fu[kTripKL], na[kTripKD];
+      auto fetch = [&](int rt) {
         const int row = rt * 16 + l16;
-        const bool rin = row < vv;
-        double fa[kTripKD], fu[kTripKL];
+        const bool rin = rt < rt_n && row < vv;
 #pragma unroll
         for (int k = 0; k < kTripKD; ++k) {
           const int d = 4 * k + kq;
-          fa[k] = rin && d < v ? Ap[(long)row * v + d] : 0.0;
+          na[k] = rin && d < v ? Ap[(long)row * v + d] : 0.0;
         }
-#pragma unroll
-        for (int k = 0; k < kTripKL; ++k) {
-          const int l = 4 * k + kq;
-          fu[k] = rin && l < o ? -Up[(long)l * vv + row] : 0.0;
-        }
-        trip_d4 acc[2];
-        acc[0] = trip_d4{0.0, 0.0, 0.0, 0.0};
-        acc[1] = trip_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k = 0; k < kTripKD; ++k) {
-          const int d = 4 * k + kq;
-          acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + l16], acc[0], 0, 0, 0);
-          if (ct_n > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + 16 + l16], acc[1], 0, 0, 0);
-        }
-#pragma unroll
-        for (int k = 0; k < kTripKL; ++k) {
-          if (4 * k >= o) break;
-          const int l = 4 * k + kq;
-          acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + l16], acc[0], 0, 0, 0);
-          if (ct_n > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + 16 + l16], acc[1], 0, 0, 0);
-        }
-        // the tile's rows (x, y) and columns z, added at the permuted position
+      };
+      fetch(wave);
+      // every phase writes one permutation's tiles (disjoint positions): the
+      // block steps through the tiles together, so the barriers line up
+      const int tiles_per_wave = (rt_n + nwave - 1) / nwave;
+      for (int t = 0; t < tiles_per_wave; ++t) {
+        const int rt = wave + t * nwave;
+        const bool has = rt < rt_n;
+        trip_d4 acc[2], acc2[2];
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
-          const int z = 16 * ct + l16;
-          if (ct >= ct_n || z >= v) continue;
+          acc[ct] = trip_d4{0.0, 0.0, 0.0, 0.0};
+          acc2[ct] = trip_d4{0.0, 0.0, 0.0, 0.0};
+        }
+        if (has) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int rr = rt * 16 + kq + 4 * e;
-            if (rr >= vv) continue;
-            const int idx = (rr / v) * sx + (rr % v) * sy + z * sz;
-            W[idx] = (s == 0 ? 0.0 : W[idx]) + acc[ct][e];
+          for (int k = 0; k < kTripKD; ++k) fa[k] = na[k];
+#pragma unroll
+          for (int k = 0; k < kTripKL; ++k) {
+            const int l = 4 * k + kq, row = rt * 16 + l16;
+            fu[k] = row < vv && l < o ? -Up[(long)l * vv + row] : 0.0;
+          }
+          if (rt + nwave < rt_n) fetch(rt + nwave);
+#pragma unroll
+          for (int k = 0; k < kTripKD; ++k) {
+            if (4 * k >= v) break;
+            const int d = 4 * k + kq;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              if (ct >= ct_n) break;
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + 16 * ct + l16], acc[ct], 0, 0, 0);
+              acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb2[d * 32 + 16 * ct + l16], acc2[ct], 0, 0, 0);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < kTripKL; ++k) {
+            if (4 * k >= o) break;
+            const int l = 4 * k + kq;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              if (ct >= ct_n) break;
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + 16 * ct + l16], acc[ct], 0, 0, 0);
+              acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb2[l * 32 + 16 * ct + l16], acc2[ct], 0, 0, 0);
+            }
+          }
+          // permutation 2h: rows (x, y), columns z at x sx + y sy + z sz
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            const int z = 16 * ct + l16;
+            if (ct >= ct_n || z >= v) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int rr = rt * 16 + kq + 4 * e;
+              if (rr < vv) W[(rr / v) * sx + (rr % v) * sy + z * sz] += acc[ct][e];
+            }
           }
         }
+        __syncthreads();
+        // permutation 2h + 1: the same tiles at x sx + y sz + z sy
+        if (has) {
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            const int z = 16 * ct + l16;
+            if (ct >= ct_n || z >= v) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int rr = rt * 16 + kq + 4 * e;
+              if (rr < vv) W[(rr / v) * sx + (rr % v) * sz + z * sy] += acc2[ct][e];
+            }
+          }
+        }
+        __syncthreads();
       }
     }
-    // energy of the item: L slices (jb|kc), (ia|kc), (ia|jb) and the t1 rows in LDS
-    const int i = occ[0], j = occ[1], k = occ[2];
+    // energy of the item: L slices (jb|kc), (ia|kc), (ia|jb) (in the B-operand
+    // region), virtual orbital energies and t1 rows of i, j, k in LDS
     __syncthreads();
     double* Ljk = Tb;
     double* Lik = Tb + vv;
-    double* Lij = Tb + 2 * vv;   // 3 v^2 <= 2 x 32 x 32 for v <= 26
-    for (int e = threadIdx.x; e < vv; e += kTripMfmaThreads) {
-      Ljk[e] = M.L[((long)j * o + k) * vv + e];
-      Lik[e] = M.L[((long)i * o + k) * vv + e];
-      Lij[e] = M.L[((long)i * o + j) * vv + e];
-    }
+    double* Lij = Tb + 2 * vv;
     double* ev = E;
     double* t1i = E + v;
     double* t1j = t1i + v;
     double* t1k = t1j + v;
-    for (int e = threadIdx.x; e < v; e += kTripMfmaThreads) {
-      ev[e] = M.F[(long)(o + e) * n + o + e];
-      t1i[e] = M.t1[(long)i * v + e];
-      t1j[e] = M.t1[(long)j * v + e];
-      t1k[e] = M.t1[(long)k * v + e];
+#pragma unroll
+    for (int u = 0; u < 3 * kStg; ++u) {
+      const int e = threadIdx.x + u * kTripMfmaThreads;
+      if (e < 3 * vv) Tb[e] = stg[u];
     }
+    if ((int)threadIdx.x < 4 * v) E[threadIdx.x] = stg[3 * kStg];
     __syncthreads();
     const double eijk = M.F[(long)i * n + i] + M.F[(long)j * n + j] + M.F[(long)k * n + k];
     double eT = 0.0, eST = 0.0;
     for (int e = threadIdx.x; e < vv * v; e += kTripMfmaThreads) {
       const int a = e / vv, b = (e / v) % v, c = e % v;
-      const double Wabc = W[e];
-      const double r3 = 4.0 * Wabc + W[(b * v + c) * v + a] + W[(c * v + a) * v + b] -
-                        2.0 * (W[(a * v + c) * v + b] + W[(b * v + a) * v + c] + W[(c * v + b) * v + a]);
-      const double D = eijk - ev[a] - ev[b] - ev[c];
+      const double Wabc = W[a * S2 + b * S1 + c];
+      const double r3 = 4.0 * Wabc + W[b * S2 + c * S1 + a] + W[c * S2 + a * S1 + b] -
+                        2.0 * (W[a * S2 + c * S1 + b] + W[b * S2 + a * S1 + c] + W[c * S2 + b * S1 + a]);
+      const double g = r3 / (3.0 * (eijk - ev[a] - ev[b] - ev[c]));
       const double V = t1i[a] * Ljk[b * v + c] + t1j[b] * Lik[a * v + c] + t1k[c] * Lij[a * v + b];
-      eT += Wabc * r3 / (3.0 * D);
-      eST += V * r3 / (3.0 * D);
+      eT += Wabc * g;
+      eST += V * g;
     }
     // block sums (fixed order)
     for (int off = 32; off > 0; off >>= 1) {
@@ -309,10 +386,10 @@ __global__ void k_triples_final(int nm, const long* __restrict__ prefix, const d
 
 size_t triples_lds_bytes() { return sizeof(double) * (kTripThreads / 64 + 2 * kTripMaxV * kTripMaxV); }
 size_t triples_mfma_lds_bytes() {
-  return sizeof(double) * (kTripLdsW + 2 * 32 * 32 + kTripLdsE + 2 * (kTripMfmaThreads / 64));
+  return sizeof(double) * (kTripLdsW + 4 * 32 * 32 + kTripLdsE + 2 * (kTripMfmaThreads / 64));
 }
 
-hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int* tri,
+hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int2* items,
                           double* wscratch, long wstride, int grid, int grid_mfma, bool any_global, double* part,
                           double* out, hipStream_t s) {
   if (total > 0 && grid_mfma > 0) {
@@ -320,10 +397,10 @@ hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long 
                                              (int)triples_mfma_lds_bytes());
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_triples_mfma, dim3(grid_mfma), dim3(kTripMfmaThreads), triples_mfma_lds_bytes(), s, mols, nm,
-                       prefix, total, tri, part);
+                       total, items, part);
   }
   if (total > 0 && any_global)
-    hipLaunchKernelGGL(k_triples, dim3(grid), dim3(kTripThreads), triples_lds_bytes(), s, mols, nm, prefix, total, tri,
+    hipLaunchKernelGGL(k_triples, dim3(grid), dim3(kTripThreads), triples_lds_bytes(), s, mols, nm, total, items,
                        wscratch, wstride, part);
   hipLaunchKernelGGL(k_triples_final, dim3((nm + 63) / 64), dim3(64), 0, s, nm, prefix, part, out);
   return hipGetLastError();
