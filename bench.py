@@ -32,11 +32,11 @@ PEAK_BF16 = 2.5e15
 PEAK_HBM = 8.0e12        # HBM3E bytes/s
 # HBM bytes per dispatch of the dominant kernels on the default 4096-molecule batch
 # with the default solver settings: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
-# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --diis 8 --steps 1 --warmup 0
-# --no-cpu --skip-reference" summarised in profiles/hbm_traffic_batch4096.md.  The GEMM
-# kernels never touch the DIIS arrays, so the figures hold at any DIIS depth.  A counter
-# pass cannot run inside this process, so other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 2.577e9        # gemm_f64_kernel, FP64 solve
+# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --workers 0 --no-cpu
+# --skip-reference --steps 1 --warmup 0 --no-profile" summarised in
+# profiles/hbm_traffic_batch4096.md.  A counter pass cannot run inside this process, so
+# other workloads report null.
+GEMM_TRAFFIC_BATCH4096 = 3.012e9        # gemm_f64_kernel, FP64 solve
 DUAL_GEMM_TRAFFIC_BATCH4096 = 1.328e9   # gemm_dual_bf16, mixed solve
 
 
@@ -289,6 +289,7 @@ def main():
         # the context's stream, graphs off): per-kernel-class device time
         gemm_ms = launches = prof_ms = None
         ops_ms = np.zeros(96)
+        ops_n = np.zeros(96, dtype=np.int64)
         if not args.no_profile:
             ctx.profile(True)
             t1 = time.perf_counter()
@@ -296,6 +297,7 @@ def main():
             prof_ms = (time.perf_counter() - t1) * 1e3
             gemm_ms, launches = ctx.profile_read()
             ops_ms = ctx.profile_ops()
+            ops_n = ctx.profile_launches()
             ctx.profile(False)
             ep, itp, stp = ctx.batch_results()
             assert np.array_equal(itp, iters) and np.array_equal(stp, st), "profiled solve differs"
@@ -305,7 +307,7 @@ def main():
         ok = (de < 1e-6) & (st == 1) & (st_ref == 1)
         step_s = float(np.mean(t_steps))
         return dict(e=e, iters=iters, st=st, de=de, ok=ok, step_s=step_s, gemm_ms=gemm_ms, launches=launches,
-                    ops_ms=ops_ms, fl_it=fl_it, fl_pre=fl_pre, outer=outer, inner=inner, fl_corr=fl_corr,
+                    ops_ms=ops_ms, ops_n=ops_n, fl_it=fl_it, fl_pre=fl_pre, outer=outer, inner=inner, fl_corr=fl_corr,
                     solves=ok.sum() / step_s, profiled_solve_ms=prof_ms)
 
     r64 = timed(capi.HCC_FP64, capi.GUESS_MP2)
@@ -343,7 +345,8 @@ def main():
                  "executed_tflops": gemm_exec / (gemm_ms * 1e-3) / 1e12,
                  "traffic": GEMM_TRAFFIC_BATCH4096 if default_workload else None,
                  "kernel": "gemm_f64_kernel (FP64 solver, all contractions)", "launches": int(r64["launches"]),
-                 "kernel_ms": gemm_ms, "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per molecule-iteration"}
+                 "kernel_ms": gemm_ms, "avg_launch_ms": gemm_ms / max(1, int(r64["launches"])),
+                 "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per molecule-iteration"}
         rooflines["roofline_fp64_solver"] = rfp64
         # integral build: the dressed integral blocks are rebuilt every iteration
         # (GEMM tables g1, g3, g5, g8, g9, g11, g14, g17, g18 and the set-up (kc|ld))
@@ -381,7 +384,8 @@ def main():
                         "executed_tflops": fexec / (corr_ms * 1e-3) / 1e12,
                         "traffic": DUAL_GEMM_TRAFFIC_BATCH4096 if default_workload else None,
                         "kernel": "gemm_dual_bf16 (mixed solver, correction contractions of the inner iterations)",
-                        "kernel_ms": corr_ms,
+                        "kernel_ms": corr_ms, "launches": int(rmx["ops_n"][32:57].sum()),
+                        "avg_launch_ms": corr_ms / max(1, int(rmx["ops_n"][32:57].sum())),
                         "work": "F_iter = 2o^2v^4 + 4o^4v^2 + 12o^3v^3 per correction evaluation"}
             groups = ["update", "diis", "update_inner", "diis_inner", "e2", "e2_correction", "residual_ew",
                       "residual_ew_correction", "outer_norm", "commit", "guess", "energy", "setup_ew", "control"]

@@ -55,6 +55,7 @@ SIGNATURES = {
     "hcc_gemm_flops": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_gemm_flops_correction": (_i, [_vp, _P(_d)]),
     "hcc_profile_ops": (_i, [_vp, _P(_d)]),
+    "hcc_profile_launches": (_i, [_vp, _P(_l)]),
     "hcc_destroy": (None, [_vp]),
     "hcc_solve_noalloc": (_i, [_i, _i, _i, _P(_d), _P(_d), _P(_d), _d, _i, _i, _d, _i, _P(_d), _P(_i), _P(_i),
                                _P(_d), _P(_d), ctypes.c_char_p, _i]),
@@ -292,6 +293,13 @@ class Context:
         self._check(self.lib.hcc_profile_ops(self.h, _dp(ms)), "hcc_profile_ops")
         return ms
 
+    def profile_launches(self):
+        """GEMM kernel dispatches per profile slot (96) since profile(True)."""
+        n = np.zeros(96, dtype=np.int64)
+        self._check(self.lib.hcc_profile_launches(self.h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_long))),
+                    "hcc_profile_launches")
+        return n
+
     def gemm_flops(self, nmol):
         it = np.zeros(nmol)
         pre = np.zeros(nmol)
@@ -302,8 +310,8 @@ class Context:
         """Mixed solver: keep the inner DIIS history across outer points."""
         self._check(self.lib.hcc_set_mixed_history(self.h, int(keep)), "hcc_set_mixed_history")
 
-    def set_mixed_diis(self, depth=0):
-        """Mixed solver: inner DIIS depth (0: diis_steps) for batches loaded afterwards."""
+    def set_mixed_diis(self, depth=8):
+        """Mixed solver: inner DIIS depth (library default 8; 0: diis_steps) for batches loaded afterwards."""
         self._check(self.lib.hcc_set_mixed_diis(self.h, int(depth)), "hcc_set_mixed_diis")
 
     def set_graphs(self, enable=True):

@@ -794,6 +794,7 @@ using namespace hcc;
 // Profile slots (hcc_profile_ops): 0..24 iteration GEMM tables, 32..56 their
 // correction forms, 60..63 set-up GEMMs, then the elementwise groups below.
 constexpr int kProfSlots = 96;
+static_assert(kProfSlots == 96, "profile slot arrays are sized 96");
 enum ProfGroup {
   kProfUpdate = 64,        // k_update1 + k_update2 (FP64 step, DIIS push fused)
   kProfDiis = 65,          // k_diis_dot + k_diis_solve + k_diis_extrap (FP64)
@@ -829,7 +830,7 @@ struct hcc_ctx {
   // mixed-precision solver options
   int mixed_bf16 = 1;
   int keep_diis = 1;   // mixed solver: inner DIIS history kept across outer steps
-  int mixed_diis = 0;  // mixed solver: inner DIIS depth (0: diis_steps)
+  int mixed_diis = 8;  // mixed solver: inner DIIS depth (0: diis_steps; at most diis_steps)
   double kappa = 0.05;
   int max_inner = 5;
   int max_outer = 30;
@@ -840,6 +841,7 @@ struct hcc_ctx {
   size_t prof_used = 0;
   double prof_ms = 0.0;
   long prof_launches = 0;
+  long prof_op_launches[96] = {0};   // GEMM kernel dispatches per profile slot
   double prof_op_ms[kProfSlots] = {0};
   // learned-guess network (FP32 parameters on the device), null until set
   float* guess_net = nullptr;
@@ -903,7 +905,10 @@ static void prof_close(hcc_ctx* c) {
 
 static void gemm(hcc_ctx* c, const GemmTable& t, int id = 63) {
   if (t.nprob == 0) return;
-  if (c->prof) c->prof_launches += table_parts(c, t);
+  if (c->prof) {
+    c->prof_launches += table_parts(c, t);
+    c->prof_op_launches[id] += table_parts(c, t);
+  }
   prof_open(c, id);
   launch_table(c, t);
   prof_close(c);
@@ -1001,7 +1006,10 @@ static void run_iteration(hcc_ctx* c, const Params& prm, const int* stop) {
 
 static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
   if (t.nprob == 0) return;
-  if (c->prof) c->prof_launches += table_parts(c, t);
+  if (c->prof) {
+    c->prof_launches += table_parts(c, t);
+    c->prof_op_launches[id] += table_parts(c, t);
+  }
   prof_open(c, id);
   for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr) {
     HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16,
@@ -1980,12 +1988,19 @@ int hcc_profile(hcc_ctx* ctx, int enable) {
   ctx->prof_ms = 0.0;
   ctx->prof_launches = 0;
   for (double& x : ctx->prof_op_ms) x = 0.0;
+  for (long& x : ctx->prof_op_launches) x = 0;
   return HCC_OK;
 }
 
 int hcc_profile_ops(hcc_ctx* ctx, double* ms96) {
   if (ctx == nullptr || ms96 == nullptr) return HCC_E_ARG;
   for (int k = 0; k < kProfSlots; ++k) ms96[k] = ctx->prof_op_ms[k];
+  return HCC_OK;
+}
+
+int hcc_profile_launches(hcc_ctx* ctx, long* n96) {
+  if (ctx == nullptr || n96 == nullptr) return HCC_E_ARG;
+  for (int k = 0; k < kProfSlots; ++k) n96[k] = ctx->prof_op_launches[k];
   return HCC_OK;
 }
 
