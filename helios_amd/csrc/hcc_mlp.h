@@ -18,7 +18,15 @@ constexpr int kMlpParams = 5313;
 void mlp_overlaps(const Mol* mols, Ragged rgNN, int nm, hipStream_t s);
 // t2 = t_mp2 + y / Dp for every amplitude, one evaluation per amplitude pair
 // (rgPairL: 1024 pairs (i a) <= (j b) per workgroup)
-void mlp_guess(const Mol* mols, Ragged rgPairL, int nm, const float* params, hipStream_t s);
+// occupied block pairs (i <= j) per workgroup of the learned guess: about 1024
+// amplitudes per workgroup
+__host__ __device__ inline int guess_pairs_per_workgroup(int v) { return v * v >= 1024 ? 1 : 1024 / (v * v); }
+
+// learned guess: rgPairBlk has ceil(o(o+1)/2 / guess_pairs_per_workgroup(v))
+// workgroups per molecule (molecules with v <= 32); rgPairL covers the
+// amplitude pairs of the larger ones (any_large: some molecule has v > 32)
+void mlp_guess(const Mol* mols, Ragged rgPairBlk, Ragged rgPairL, int nm, bool any_large, const float* params,
+               hipStream_t s);
 
 // device view of a loaded batch, for the trainer (hcc_solver.hip)
 struct BatchView {

@@ -48,6 +48,35 @@ __device__ __forceinline__ float gelu_prime(float z) {
 // ---------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------
+// The 16 features of an amplitude from the quantities of its pair
+// representative: orbital energies e_i, e_j, e_a, e_b, the midgap mu, the
+// integrals K = (ia|jb) and X = (ib|ja), the site-localisation overlaps
+// O_ij, O_ab, O_ia, O_jb, O_ib, O_ja and the site count.  Returns the MP2
+// amplitude t and Dp.  (One formula for the guess, the training rows and
+// every kernel that evaluates the network, so their features agree bitwise.)
+__device__ __forceinline__ void features_of(double ei, double ej, double ea, double eb, double mu, double K, double X,
+                                            double oij, double oab, double oia, double ojb, double oib, double oja,
+                                            int N, float* f, double& t, double& Dp) {
+  Dp = (ea + eb) - (ei + ej);
+  t = -K / Dp;
+  f[0] = (float)(10.0 * t);
+  f[1] = (float)(K / 2.0);
+  f[2] = (float)(X / 2.0);
+  f[3] = (float)(Dp / 10.0);
+  f[4] = (float)(2.0 / Dp);
+  f[5] = (float)(((ei + ej) - 2.0 * mu) / 10.0);
+  f[6] = (float)(((ea + eb) - 2.0 * mu) / 10.0);
+  f[7] = (float)(fabs(ei - ej) / 10.0);
+  f[8] = (float)(fabs(ea - eb) / 10.0);
+  f[9] = (float)oij;
+  f[10] = (float)oab;
+  f[11] = (float)((oia + ojb) / 2.0);
+  f[12] = (float)(fabs(oia - ojb) / 2.0);
+  f[13] = (float)((oib + oja) / 2.0);
+  f[14] = (float)(fabs(oib - oja) / 2.0);
+  f[15] = (float)(10.0 / N);
+}
+
 // Features of amplitude (i, j, a, b) (virtual indices from 0), computed from
 // the pair's representative (i a) <= (j b) so that t2 stays exactly
 // pair-symmetric.  Returns the MP2 amplitude and Dp.
@@ -63,27 +92,9 @@ __device__ void amp_features(const Mol& M, int i, int j, int a, int b, float* f,
   const double mu = 0.5 * (F[(long)(o - 1) * N + o - 1] + F[(long)o * N + o]);
   const double K = M.Gklcd[(((long)i * o + j) * v + a) * v + b];   // (ia|jb)
   const double X = M.Gklcd[(((long)i * o + j) * v + b) * v + a];   // (ib|ja)
-  Dp = (ea + eb) - (ei + ej);
-  t = -K / Dp;
   const double* O = M.Ov;
-  const double oia = O[(long)i * N + o + a], ojb = O[(long)j * N + o + b];
-  const double oib = O[(long)i * N + o + b], oja = O[(long)j * N + o + a];
-  f[0] = (float)(10.0 * t);
-  f[1] = (float)(K / 2.0);
-  f[2] = (float)(X / 2.0);
-  f[3] = (float)(Dp / 10.0);
-  f[4] = (float)(2.0 / Dp);
-  f[5] = (float)(((ei + ej) - 2.0 * mu) / 10.0);
-  f[6] = (float)(((ea + eb) - 2.0 * mu) / 10.0);
-  f[7] = (float)(fabs(ei - ej) / 10.0);
-  f[8] = (float)(fabs(ea - eb) / 10.0);
-  f[9] = (float)O[(long)i * N + j];
-  f[10] = (float)O[(long)(o + a) * N + o + b];
-  f[11] = (float)((oia + ojb) / 2.0);
-  f[12] = (float)(fabs(oia - ojb) / 2.0);
-  f[13] = (float)((oib + oja) / 2.0);
-  f[14] = (float)(fabs(oib - oja) / 2.0);
-  f[15] = (float)(10.0 / N);
+  features_of(ei, ej, ea, eb, mu, K, X, O[(long)i * N + j], O[(long)(o + a) * N + o + b], O[(long)i * N + o + a],
+              O[(long)j * N + o + b], O[(long)i * N + o + b], O[(long)j * N + o + a], N, f, t, Dp);
 }
 
 __device__ __forceinline__ void decode(long e, int o, int v, int& i, int& j, int& a, int& b) {
@@ -200,15 +211,104 @@ __global__ __launch_bounds__(256) void k_mlp_overlap(const Mol* __restrict__ mol
   M.Ov[e] = N * s;
 }
 
-// every amplitude of the molecule: 4 waves x 16 groups x 16 amplitudes per workgroup
-// One network evaluation per amplitude pair: work item p enumerates the pairs
-// (mm, nn) = ((i a), (j b)) with mm <= nn (p = nn (nn + 1) / 2 + mm), and the
-// value is written to t2[i][j][a][b] and its mirror t2[j][i][b][a].
-__global__ __launch_bounds__(256) void k_mlp_guess(const Mol* __restrict__ mols, Ragged rg, int nm,
-                                                   const float* __restrict__ w) {
+// The learned guess over the occupied block pairs i <= j of a molecule
+// (o(o+1)/2 pairs; each workgroup takes guess_pairs_per_workgroup(v)
+// consecutive ones, about 1024 amplitudes, so the network is loaded once per
+// workgroup).
+// Every (a, b) of block (i, j) with i < j is a pair representative
+// ((i a) <= (j b)); for i = j those with a <= b are.  The block's integrals
+// (kc|ld) at [i][j][:][:], the virtual-virtual overlaps and the overlap rows
+// of i and j are read once, coalesced, into LDS; the network runs once per
+// representative; t2[i][j][a][b] is written directly (consecutive b per wave)
+// and the mirror block t2[j][i][b][a] through an LDS transpose, so t2 stays
+// exactly pair-symmetric and every global access is coalesced.
+constexpr int kGuessV = 32;   // v <= 32 (larger molecules: k_mlp_guess_pairs)
+__global__ __launch_bounds__(256) void k_mlp_guess_blocks(const Mol* __restrict__ mols, Ragged rg, int nm,
+                                                          const float* __restrict__ w) {
+  __shared__ double sK[kGuessV * kGuessV], sOvv[kGuessV * kGuessV], sT[kGuessV * (kGuessV + 1)];
+  __shared__ double sOi[kGuessV], sOj[kGuessV], sEv[kGuessV];
+  const int mol = rg.bmol[blockIdx.x];
+  const Mol& M = mols[mol];
+  const int N = M.n, o = M.o, v = M.v, vv = v * v;
+  if (v > kGuessV) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
+  Net net;
+  net_load(net, w, lane);
+  Acts unused;
+  const double* F = M.F;
+  const double* O = M.Ov;
+  const double mu = 0.5 * (F[(long)(o - 1) * N + o - 1] + F[(long)o * N + o]);
+  for (int e = threadIdx.x; e < vv; e += 256) sOvv[e] = O[(long)(o + e / v) * N + o + e % v];
+  for (int e = threadIdx.x; e < v; e += 256) sEv[e] = F[(long)(o + e) * N + o + e];
+  const int npb = o * (o + 1) / 2, bpw = guess_pairs_per_workgroup(v);
+  const int pb0 = (blockIdx.x - rg.prefix[mol]) * bpw, pb1 = min(npb, pb0 + bpw);
+  for (int pb = pb0; pb < pb1; ++pb) {
+    int j = (int)((sqrt(8.0 * pb + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > pb) --j;
+    while ((j + 1) * (j + 2) / 2 <= pb) ++j;
+    const int i = pb - j * (j + 1) / 2;   // i <= j < o
+    const long blk = ((long)i * o + j) * vv;
+    __syncthreads();   // the previous pair's LDS reads and mirror writes are done
+    for (int e = threadIdx.x; e < vv; e += 256) sK[e] = M.Gklcd[blk + e];
+    for (int e = threadIdx.x; e < v; e += 256) {
+      sOi[e] = O[(long)i * N + o + e];
+      sOj[e] = O[(long)j * N + o + e];
+    }
+    __syncthreads();
+    const double ei = F[(long)i * N + i], ej = F[(long)j * N + j];
+    const double oij = O[(long)i * N + j];
+    // representatives in order: (a, b) row-major over the block (i < j), or
+    // the upper triangle a <= b row by row (i = j)
+    const int nrep = i < j ? vv : v * (v + 1) / 2;
+    for (int base = wave * 16; base < nrep; base += 64) {
+      const int q = base + n;
+      const bool ok = q < nrep;
+      int a = 0, b = 0;
+      if (ok) {
+        if (i < j) {
+          a = q / v;
+          b = q - a * v;
+        } else {
+          // row a of the upper triangle starts at a v - a (a - 1) / 2
+          a = (int)(((2.0 * v + 1.0) - sqrt((2.0 * v + 1.0) * (2.0 * v + 1.0) - 8.0 * q)) * 0.5);
+          while (a > 0 && a * v - a * (a - 1) / 2 > q) --a;
+          while ((a + 1) * v - (a + 1) * a / 2 <= q) ++a;
+          b = a + (q - (a * v - a * (a - 1) / 2));
+        }
+      }
+      float f[kMlpF];
+#pragma unroll
+      for (int r = 0; r < kMlpF; ++r) f[r] = 0.f;
+      double t = 0.0, Dp = 1.0;
+      if (ok && g < 2)
+        features_of(ei, ej, sEv[a], sEv[b], mu, sK[a * v + b], sK[b * v + a], oij, sOvv[a * v + b], sOi[a], sOj[b],
+                    sOi[b], sOj[a], N, f, t, Dp);
+      const float y = net_forward<false>(net, pack_x(f, g), unused);
+      if (ok && g == 0) {
+        const double val = t + (double)y / Dp;
+        M.t2[blk + a * v + b] = val;
+        if (i < j) sT[b * (kGuessV + 1) + a] = val;   // t2[j][i][b][a]
+        else M.t2[blk + b * v + a] = val;
+      }
+    }
+    if (i < j) {
+      __syncthreads();
+      const long mblk = ((long)j * o + i) * vv;
+      for (int e = threadIdx.x; e < vv; e += 256) M.t2[mblk + e] = sT[(e / v) * (kGuessV + 1) + e % v];
+    }
+  }
+}
+
+// Molecules with v > 32: one network evaluation per amplitude pair; work item
+// p enumerates the pairs (mm, nn) = ((i a), (j b)) with mm <= nn
+// (p = nn (nn + 1) / 2 + mm), and the value is written to t2[i][j][a][b] and
+// its mirror t2[j][i][b][a].  4 waves x 16 groups x 16 amplitudes per workgroup.
+__global__ __launch_bounds__(256) void k_mlp_guess_pairs(const Mol* __restrict__ mols, Ragged rg, int nm,
+                                                         const float* __restrict__ w) {
   const int mol = rg.bmol[blockIdx.x];
   const Mol& M = mols[mol];
   const int o = M.o, v = M.v;
+  if (v <= kGuessV) return;
   const long ne = (long)o * v, npair = ne * (ne + 1) / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
   const long base = (long)(blockIdx.x - rg.prefix[mol]) * 1024 + wave * 256;
@@ -245,9 +345,12 @@ __global__ __launch_bounds__(256) void k_mlp_guess(const Mol* __restrict__ mols,
 void mlp_overlaps(const Mol* mols, Ragged rgNN, int nm, hipStream_t s) {
   if (rgNN.nblocks > 0) hipLaunchKernelGGL(k_mlp_overlap, dim3(rgNN.nblocks), dim3(256), 0, s, mols, rgNN, nm);
 }
-void mlp_guess(const Mol* mols, Ragged rgPairL, int nm, const float* params, hipStream_t s) {
-  if (rgPairL.nblocks > 0)
-    hipLaunchKernelGGL(k_mlp_guess, dim3(rgPairL.nblocks), dim3(256), 0, s, mols, rgPairL, nm, params);
+void mlp_guess(const Mol* mols, Ragged rgPairBlk, Ragged rgPairL, int nm, bool any_large, const float* params,
+               hipStream_t s) {
+  if (rgPairBlk.nblocks > 0)
+    hipLaunchKernelGGL(k_mlp_guess_blocks, dim3(rgPairBlk.nblocks), dim3(256), 0, s, mols, rgPairBlk, nm, params);
+  if (any_large && rgPairL.nblocks > 0)
+    hipLaunchKernelGGL(k_mlp_guess_pairs, dim3(rgPairL.nblocks), dim3(256), 0, s, mols, rgPairL, nm, params);
 }
 
 // ---------------------------------------------------------------------------

@@ -130,7 +130,8 @@ struct Batch {
   State* dst = nullptr;
   int* dact = nullptr;
   std::vector<void*> allocs;
-  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL, rgOcc;
+  Ragged rgNN, rgN3, rgT2, rgT2S, rgOV, rgXX, rgND, rgN, rgP2, rgPairL, rgPairBlk, rgOcc;
+  bool guess_large = false;   // some molecule has v > 32 (pairwise learned-guess kernel)
   Ragged rgT2x, rgT2Sx;   // rgT2, rgT2S with kEwItems elements per thread
   double *part1 = nullptr, *part2 = nullptr, *partE = nullptr, *partD = nullptr;
   std::vector<GemmTable> pre, it;
@@ -560,6 +561,13 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.rgOcc = make_ragged(b, [](int, int o, int) { return (long)o * kE2Split * kEwThreads; });   // kE2Split blocks per i
   // learned guess: 1024 amplitude pairs (i a) <= (j b) per workgroup
   b.rgPairL = make_ragged(b, [](int, int o, int v) { const long ne = (long)o * v; return ne * (ne + 1) / 2; }, 4);
+  // consecutive occupied block pairs i <= j per workgroup
+  b.rgPairBlk = make_ragged(b, [](int, int o, int v) {
+    const int bpw = guess_pairs_per_workgroup(v);
+    return (long)((o * (o + 1) / 2 + bpw - 1) / bpw) * kEwThreads;
+  });
+  b.guess_large = false;
+  for (int k = 0; k < b.nm; ++k) b.guess_large = b.guess_large || b.v[k] > 32;
   b.rgT2S = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; });
   b.rgT2x = make_ragged(b, sz_t2, kEwItems);
   b.rgT2Sx = make_ragged(b, [](int, int o, int v) { return (long)o * o * v * v + (long)o * v; }, kEwItems);
@@ -1077,7 +1085,7 @@ static void apply_guess(hcc_ctx* c, int kind) {
   guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind == HCC_GUESS_LEARNED ? HCC_GUESS_MP2 : kind, c->stream);
   if (kind == HCC_GUESS_LEARNED) {
     mlp_overlaps(b.dm, b.rgNN, b.nm, c->stream);
-    mlp_guess(b.dm, b.rgPairL, b.nm, c->guess_net, c->stream);
+    mlp_guess(b.dm, b.rgPairBlk, b.rgPairL, b.nm, b.guess_large, c->guess_net, c->stream);
     HCHECK(hipGetLastError());
   }
   prof_close(c);
