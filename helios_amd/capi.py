@@ -47,6 +47,7 @@ SIGNATURES = {
     "hcc_batch_solve": (_i, [_vp, _i, _i, _d]),
     "hcc_batch_get_orbitals": (_i, [_vp, _i, _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_triples": (_i, [_vp, _P(_d), _P(_d)]),
+    "hcc_batch_lr_sigma": (_i, [_vp, _d, _P(_d), _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
@@ -258,6 +259,17 @@ class Context:
         et = np.zeros((self.nmol, 2))
         self._check(self.lib.hcc_batch_triples(self.h, _dp(e), _dp(et)), "hcc_batch_triples")
         return e, et
+
+    def batch_lr_sigma(self, r1, r2, h=1e-4):
+        """sigma = J r for one trial vector per loaded molecule: r1, r2 are the
+        molecules' [i][a] and [i][j][a][b] blocks concatenated (float64)."""
+        r1 = np.ascontiguousarray(r1, dtype=np.float64)
+        r2 = np.ascontiguousarray(r2, dtype=np.float64)
+        s1 = np.zeros_like(r1)
+        s2 = np.zeros_like(r2)
+        self._check(self.lib.hcc_batch_lr_sigma(self.h, float(h), _dp(r1), _dp(r2), _dp(s1), _dp(s2)),
+                    "hcc_batch_lr_sigma")
+        return s1, s2
 
     def batch_orbitals(self, mol, n):
         """(C, F_mo, e_ref) of loaded molecule `mol` with n sites."""

@@ -163,6 +163,11 @@ struct Batch {
   int2* dtri = nullptr;
   int* dtsave = nullptr;
   double *dtw = nullptr, *dtpart = nullptr, *dtout = nullptr;
+  // linear response: concatenated trial/sigma vectors, amplitude backups
+  long* dlr_off = nullptr;   // off1 (nm), off2 (nm)
+  double *dlr_r1 = nullptr, *dlr_r2 = nullptr, *dlr_s1 = nullptr, *dlr_s2 = nullptr;
+  double *dlr_t1b = nullptr, *dlr_t2b = nullptr;
+  long lr_n1 = 0, lr_n2 = 0, lr_span1 = 0, lr_span2 = 0;
   long ttotal = 0, twstride = 0;
   int tgrid = 0, tgrid_mfma = 0;
   bool tglobal = false;
@@ -1517,6 +1522,74 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
       energies[5 * k + 4] = c1;
     }
   }
+  API_END
+}
+
+// Buffers of the linear-response sigma products (once per batch).
+static void setup_lr(Batch& b) {
+  std::vector<long> off(2 * (size_t)b.nm);
+  long n1 = 0, n2 = 0;
+  for (int k = 0; k < b.nm; ++k) {
+    off[k] = n1;
+    off[b.nm + k] = n2;
+    n1 += (long)b.o[k] * b.v[k];
+    n2 += (long)b.o[k] * b.o[k] * b.v[k] * b.v[k];
+  }
+  const Mol& first = b.hm[0];
+  const Mol& last = b.hm[b.nm - 1];
+  b.lr_n1 = n1;
+  b.lr_n2 = n2;
+  b.lr_span1 = (long)(last.t1 - first.t1) + (long)last.o * last.v;
+  b.lr_span2 = (long)(last.t2 - first.t2) + (long)last.o * last.o * last.v * last.v;
+  b.dlr_off = b.dalloc<long>(off.size());
+  b.dlr_r1 = b.dalloc<double>(n1);
+  b.dlr_r2 = b.dalloc<double>(n2);
+  b.dlr_s1 = b.dalloc<double>(n1);
+  b.dlr_s2 = b.dalloc<double>(n2);
+  b.dlr_t1b = b.dalloc<double>(b.lr_span1);
+  b.dlr_t2b = b.dalloc<double>(b.lr_span2);
+  HCHECK(hipMemcpy(b.dlr_off, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice));
+}
+
+int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r2, double* s1, double* s2) {
+  if (ctx == nullptr || r1 == nullptr || r2 == nullptr || s1 == nullptr || s2 == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (!(h > 0.0)) {
+    ctx->err = "the difference step must be positive";
+    return HCC_E_ARG;
+  }
+  if (!ctx->have_guess) {
+    ctx->err = "linear response needs amplitudes: solve first";
+    return HCC_E_STATE;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  Batch& b = *ctx->batch;
+  hipStream_t s = ctx->stream;
+  if (b.dlr_off == nullptr) setup_lr(b);
+  if (b.dtsave == nullptr) b.dtsave = b.dalloc<int>(b.nm);
+  const Mol& first = b.hm[0];
+  const long* off1 = b.dlr_off;
+  const long* off2 = b.dlr_off + b.nm;
+  HCHECK(hipMemcpyAsync(b.dlr_r1, r1, b.lr_n1 * sizeof(double), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(b.dlr_r2, r2, b.lr_n2 * sizeof(double), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(b.dlr_t1b, first.t1, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HCHECK(hipMemcpyAsync(b.dlr_t2b, first.t2, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  arm_all(b.dst, b.dact, b.dtsave, b.nm, false, s);
+  // sigma = (Omega(t0 + h r) - Omega(t0 - h r)) / 2h
+  lr_perturb(b.dm, b.rgT2, b.nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, off1, off2, b.dlr_r1, b.dlr_r2, h, s);
+  residual(ctx);
+  lr_omega(b.dm, b.rgT2, b.nm, off1, off2, b.dlr_s1, b.dlr_s2, false, 1.0, s);
+  lr_perturb(b.dm, b.rgT2, b.nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, off1, off2, b.dlr_r1, b.dlr_r2, -h, s);
+  residual(ctx);
+  lr_omega(b.dm, b.rgT2, b.nm, off1, off2, b.dlr_s1, b.dlr_s2, true, 0.5 / h, s);
+  HCHECK(hipMemcpyAsync(first.t1, b.dlr_t1b, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HCHECK(hipMemcpyAsync(first.t2, b.dlr_t2b, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  arm_all(b.dst, b.dact, b.dtsave, b.nm, true, s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipMemcpyAsync(s1, b.dlr_s1, b.lr_n1 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HCHECK(hipMemcpyAsync(s2, b.dlr_s2, b.lr_n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HCHECK(hipStreamSynchronize(s));
   API_END
 }
 

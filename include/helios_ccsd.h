@@ -187,6 +187,19 @@ int hcc_batch_load_scf(hcc_ctx* ctx, int nmol, const int* n, const int* n_occ, c
  * virtual orbitals. */
 int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t);
 
+/* Linear response: sigma = J r for one trial vector per loaded molecule,
+ * J = dOmega/dt the Jacobian of the closed-shell (biorthogonal) CCSD residual
+ * at the converged amplitudes, whose eigenvalues are the singlet LR-CCSD
+ * (EOM-CCSD) excitation energies -- the quantity the reference's LR-CCSD
+ * iteration converges to for spin = 0 (src/methods/coupledCluster/lrccsd/
+ * mod_lrccsd.f90:176-326, projections proj-lrccsd-spin-hf-1/2.f90).
+ * r1, s1: per molecule [i][a] (o v values), r2, s2: [i][j][a][b] (o^2 v^2,
+ * pair-symmetric: r2[i][j][a][b] = r2[j][i][b][a]), molecules back to back
+ * in load order.  sigma is the central difference
+ * (Omega(t0 + h r) - Omega(t0 - h r)) / 2h of the FP64 residual (exact to
+ * O(h^2): Omega is a polynomial in t); the amplitudes are restored. */
+int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r2, double* s1, double* s2);
+
 /* MO coefficients C (n x n column-major, = hV), MO Fock matrix and reference
  * energy of loaded molecule `mol` (either array may be null): the values the
  * solve starts from (getSCFResult(vectors=hV), src/methods/mod_scf.f90:204-277,

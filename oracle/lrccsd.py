@@ -1,0 +1,94 @@
+"""Singlet LR-CCSD (EOM-CCSD) excitation energies, restated for the tests
+(test infrastructure only).
+
+The reference's LR-CCSD (src/methods/coupledCluster/lrccsd/mod_lrccsd.f90:
+176-326) iterates a trial vector r with the projections of
+proj-lrccsd-spin-hf-1/2.f90 (the CCSD Jacobian acting on r in spin
+orbitals), the Rayleigh quotient omega = <d, r>/<r, r> and the step
+r -= step (d - omega r), and reports omega at convergence: an eigenvalue of
+the CCSD Jacobian, i.e. an LR-CCSD excitation energy; spin = 0 keeps the
+singlet combination r1(alpha) = r1(beta) (:298-314).
+
+Here the singlet Jacobian is formed in the closed-shell amplitude space --
+singles t1[i,a] and the pair-symmetric doubles t2[i,j,a,b] = t2[j,i,b,a],
+one coordinate per pair (i a) <= (j b) -- as the central difference of the
+biorthogonal closed-shell residual ``oracle.dressed.omega`` along every
+coordinate, and diagonalised densely.  Pinned in the tests by the exact
+two-electron case: there EOM-CCSD is exact and its n(n+1)/2 - 1 singlet
+excitation energies are those of the two-electron singlet full CI
+(``singlets_two_electrons``).
+"""
+import numpy as np
+
+from oracle import dressed
+
+
+def pair_list(o, v):
+    ne = o * v
+    return [(m, n) for n in range(ne) for m in range(n + 1)]
+
+
+def jacobian(C, F, G, o, t1, t2, h=1e-5):
+    """Dense d Omega / d t over (singles, pair-symmetric doubles)."""
+    v = C.shape[0] - o
+    P = pair_list(o, v)
+    n1 = o * v
+    dim = n1 + len(P)
+    ii = np.array([m // v for m, n in P]); aa = np.array([m % v for m, n in P])
+    jj = np.array([n // v for m, n in P]); bb = np.array([n % v for m, n in P])
+
+    def vec(w1, w2):
+        return np.concatenate([w1.ravel(), w2[ii, jj, aa, bb]])
+
+    J = np.empty((dim, dim))
+    for c in range(dim):
+        d1 = np.zeros((o, v))
+        d2 = np.zeros((o, o, v, v))
+        if c < n1:
+            d1.ravel()[c] = 1.0
+        else:
+            k = c - n1
+            d2[ii[k], jj[k], aa[k], bb[k]] = 1.0
+            d2[jj[k], ii[k], bb[k], aa[k]] = 1.0
+        wp = dressed.omega(C, F, G, o, t1 + h * d1, t2 + h * d2)
+        wm = dressed.omega(C, F, G, o, t1 - h * d1, t2 - h * d2)
+        J[:, c] = (vec(*wp) - vec(*wm)) / (2 * h)
+    return J
+
+
+def excitation_energies(C, F, G, o, t1, t2, h=1e-5):
+    """All eigenvalues of the singlet Jacobian, real ones sorted, and the
+    count of complex ones."""
+    lam = np.linalg.eigvals(jacobian(C, F, G, o, t1, t2, h))
+    real = np.abs(lam.imag) <= 1e-8 * np.abs(lam.real)
+    return np.sort(lam.real[real]), int((~real).sum())
+
+
+def sigma(C, F, G, o, t1, t2, r1, r2, h=1e-4):
+    """J r by the same central difference, for full (pair-symmetric) r2."""
+    wp = dressed.omega(C, F, G, o, t1 + h * r1, t2 + h * r2)
+    wm = dressed.omega(C, F, G, o, t1 - h * r1, t2 - h * r2)
+    return (wp[0] - wm[0]) / (2 * h), (wp[1] - wm[1]) / (2 * h)
+
+
+def singlets_two_electrons(C, core, G):
+    """Excitation energies of the two-electron singlet full CI in the MO basis
+    (spatially symmetric products (phi_p phi_q + phi_q phi_p) / norm, p <= q):
+    <pq|H|rs> = h_pr d_qs + d_pr h_qs + (pr|qs) for the product phi_p(1) phi_q(2)."""
+    n = C.shape[0]
+    h = C.T @ core @ C
+    X = np.einsum("mp,mq->mpq", C, C)
+    g = np.einsum("mpq,mn,nrs->pqrs", X, G, X, optimize=True)
+    basis = [(p, q) for p in range(n) for q in range(p, n)]
+
+    def prod(p, q, r, s):
+        return h[p, r] * (q == s) + (p == r) * h[q, s] + g[p, r, q, s]
+
+    H = np.zeros((len(basis), len(basis)))
+    for x, (p, q) in enumerate(basis):
+        for y, (r, s) in enumerate(basis):
+            nx = 1.0 / np.sqrt(2.0 * (1 + (p == q)))
+            ny = 1.0 / np.sqrt(2.0 * (1 + (r == s)))
+            H[x, y] = nx * ny * (prod(p, q, r, s) + prod(p, q, s, r) + prod(q, p, r, s) + prod(q, p, s, r))
+    w = np.linalg.eigvalsh(H)
+    return w[1:] - w[0]
