@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from helios_amd import autotune, capi, learned_guess, molecules, ppp  # noqa: E402
+from helios_amd import autotune, capi, learned_guess, lrccsd, molecules, ppp  # noqa: E402
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (matrix) FLOP/s
 PEAK_BF16 = 2.5e15
@@ -113,6 +113,40 @@ def triples_leg(ctx, o, v):
            "work": "2 o^3 v^3 (o + v) flops per molecule (connected triples over ordered i, j, k)"}
     print(f"# (T): kernel {kms:.1f} ms ({out['achieved']:.1f} TFLOP/s, {out['frac']:.3f} of FP64 peak), "
           f"wall {wall * 1e3:.1f} ms for {len(o)} molecules", file=sys.stderr, flush=True)
+    return out
+
+
+def lr_leg(mols, nroots=3):
+    """Singlet LR-CCSD (three lowest roots) of a sample of the batch: device
+    sigma products (two FP64 residual passes each, with the vector transfers)
+    and the host Davidson, timed apart."""
+    ctx = capi.Context(0, capi.HCC_FP64, 20)
+    ctx.batch_load(mols)
+    ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+    calls = [0, 0.0]
+    sigma = ctx.batch_lr_sigma
+
+    def timed_sigma(r1, r2, h):
+        t1 = time.perf_counter()
+        out = sigma(r1, r2, h)
+        calls[0] += 1
+        calls[1] += time.perf_counter() - t1
+        return out
+
+    ctx.batch_lr_sigma = timed_sigma
+    t1 = time.perf_counter()
+    om, res, its, conv = lrccsd.excitation_energies(
+        ctx, [(m["nocc"], m["n"] - m["nocc"]) for m in mols], nroots=nroots,
+        log=lambda it, active: print(f"# LR-CCSD iteration {it}: {active} molecules active", file=sys.stderr,
+                                     flush=True))
+    wall = time.perf_counter() - t1
+    ctx.close()
+    out = {"molecules": len(mols), "roots": nroots, "wall_s": wall, "sigma_calls": calls[0],
+           "sigma_s": calls[1], "sigma_ms_per_call": 1e3 * calls[1] / max(1, calls[0]),
+           "host_davidson_s": wall - calls[1], "mean_iterations": float(its.mean()),
+           "converged": int(conv.sum()), "mean_omega_eV": [float(x) for x in np.nanmean(om, axis=0)]}
+    print(f"# LR-CCSD: {len(mols)} molecules, {nroots} roots, {wall:.1f} s ({calls[0]} sigma calls, "
+          f"{1e3 * calls[1] / max(1, calls[0]):.1f} ms each), converged {int(conv.sum())}", file=sys.stderr, flush=True)
     return out
 
 
@@ -226,7 +260,7 @@ def main():
     v = np.array([m["n"] - m["nocc"] for m in mols])
 
     # FP64 reference energies (untimed, tight threshold)
-    triples = None
+    triples = lr = None
     if args.skip_reference:
         e_ref = np.zeros((len(mols), 5))
         st_ref = np.ones(len(mols), dtype=np.int32)
@@ -237,6 +271,7 @@ def main():
         e_ref, it_ref, st_ref = ctx.batch_results()
         triples = triples_leg(ctx, o, v)
         ctx.close()
+        lr = lr_leg(mols[:64])
         print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
               f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
@@ -441,6 +476,8 @@ def main():
         line["config"]["device_scf"] = device_scf
     if triples is not None:
         line["roofline_triples"] = triples
+    if lr is not None:
+        line["config"]["lr_ccsd_sample"] = lr
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(mols, r64["iters"], threads)

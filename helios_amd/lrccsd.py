@@ -29,11 +29,12 @@ from helios_amd import capi
 
 
 class _Davidson:
-    """Subspace of one molecule: orthonormal basis V, sigma vectors S."""
+    """Subspace of one molecule: orthonormal basis rows Vm[:k], sigma rows Sm[:k]."""
 
     def __init__(self, o, v, e, nroots, max_space):
         self.o, self.v = o, v
         self.n1, self.n2 = o * v, o * o * v * v
+        N = self.n1 + self.n2
         eo, ev = e[:o], e[o:]
         d1 = ev[None, :] - eo[:, None]
         d2 = (ev[None, None, :, None] + ev[None, None, None, :] - eo[:, None, None, None]
@@ -47,14 +48,18 @@ class _Davidson:
         # the space's true dimension: singles plus one coordinate per pair (i a) <= (j b)
         self.dim = self.n1 + self.n1 * (self.n1 + 1) // 2
         self.nwork = min(self.nwork, self.dim)
-        self.max_space = max(max_space, 6 * self.nwork)
-        self.V, self.S = [], []
+        self.max_space = min(max(max_space, 6 * self.nwork), self.dim)
+        cap = self.max_space + 2 * self.nwork + 8
+        self.Vm = np.zeros((cap, N))
+        self.Sm = np.zeros((cap, N))
+        self.k = 0
+        self.G = np.zeros((0, 0))   # G[k, l] = <v_k, J v_l>, grown with the subspace
         self.new = []
         # unit singles on the lowest orbital-energy differences, more than the
         # roots: a Davidson subspace keeps the point-group symmetry of its
         # start, so every symmetry among the low states needs a start vector
         for idx in np.argsort(d1.ravel(), kind="stable")[:min(self.n1, nroots + 6)]:
-            x = np.zeros(self.n1 + self.n2)
+            x = np.zeros(N)
             x[idx] = 1.0
             self.new.append(x)
         # further start directions, in order of the diagonal: unit singles, then
@@ -62,9 +67,30 @@ class _Davidson:
         # ordered on first use
         self._d1, self._d2 = d1, d2
         self.seeds = None
-        self.omega = np.full(self.nwork, np.nan)
-        self.res = np.full(self.nwork, np.inf)
+        # the lowest pair-symmetric doubles start too: states of mainly double
+        # excitation character can lie far below their diagonal, and reached
+        # only through the singles' corrections they surface late
+        for e1, e2 in self._double_seeds(2):
+            x = np.zeros(N)
+            x[e1] = x[e2] = 1.0
+            self.new.append(x / np.linalg.norm(x))
+        self.prev = []     # last step's Ritz vectors (x, J x), kept through a collapse
+        self.omega = np.full(nroots, np.nan)
+        self.res = np.full(nroots, np.inf)
         self.done = False
+
+    def _double_seeds(self, count):
+        """The ``count`` pair-symmetric unit doubles (i a)(j b) of lowest diagonal."""
+        o, v, n1 = self.o, self.v, self.n1
+        if self.n1 < 2 or count <= 0:
+            return []
+        m, n = np.triu_indices(n1)
+        d = self._d2[m // v, n // v, m % v, n % v]
+        out = []
+        for q in np.argsort(d, kind="stable")[:count]:
+            i, a, j, b = m[q] // v, m[q] % v, n[q] // v, n[q] % v
+            out.append((n1 + ((i * o + j) * v + a) * v + b, n1 + ((j * o + i) * v + b) * v + a))
+        return out
 
     def _seed_order(self):
         o, v, n1 = self.o, self.v, self.n1
@@ -96,73 +122,108 @@ class _Davidson:
         return None
 
     def _orth(self, x):
+        V = self.Vm[:self.k]
         for _ in range(2):
-            for b in self.V:
-                x = x - (b @ x) * b
+            x = x - (V @ x) @ V
             for b in self.new:
                 x = x - (b @ x) * b
         return x
 
     def absorb(self, sig):
-        """Add the sigma vectors of the pending trial vectors."""
-        self.V.extend(self.new)
-        self.S.extend(sig)
+        """Add the pending trial vectors and their sigma vectors."""
+        k0, m = self.k, len(self.new)
+        for q in range(m):
+            self.Vm[k0 + q] = self.new[q]
+            self.Sm[k0 + q] = sig[q]
+        self.k = k0 + m
         self.new = []
+        V, S = self.Vm[:self.k], self.Sm[:self.k]
+        G = np.zeros((self.k, self.k))
+        G[:k0, :k0] = self.G
+        G[:, k0:] = V @ S[k0:].T
+        G[k0:, :k0] = V[k0:] @ S[:k0].T
+        self.G = G
 
     def step(self, eps):
-        V = np.array(self.V)
-        S = np.array(self.S)
-        G = V @ S.T          # G[k, l] = <v_k, J v_l>
-        lam, Y = np.linalg.eig(G)     # right eigenvectors of V^T J V
-        # real Ritz values only: a complex pair (J is not symmetric) is not an
-        # excitation energy and cannot be followed with real vectors
-        real = np.abs(lam.imag) <= 1e-8 * np.maximum(np.abs(lam.real), 1.0)
-        lam, Y = lam[real], Y[:, real]
+        V, S = self.Vm[:self.k], self.Sm[:self.k]
+        lam, Y = np.linalg.eig(self.G)   # right eigenvectors of V^T J V
+        # One entry per real Ritz value and per complex pair (J is not
+        # symmetric), lowest real parts first.  Complex pairs are not
+        # excitation energies; they stay in the working set (expanded along
+        # the real and imaginary parts of their corrections) so that a pair
+        # close to the real axis is resolved rather than read as a root.
+        keep = lam.imag >= -1e-12 * np.maximum(np.abs(lam.real), 1.0)
+        lam, Y = lam[keep], Y[:, keep]
         order = np.argsort(lam.real, kind="stable")
         lam, Y = lam[order], Y[:, order]
+        isreal = np.abs(lam.imag) <= 1e-10 * np.maximum(np.abs(lam.real), 1.0)
         k = min(self.nwork, len(lam))
         corrections = []
-        ritz = []
+        reals = []          # (value, residual, x, J x) of the real Ritz values in the working set
+        pairs = []          # real and imaginary parts of the complex Ritz pairs' (x, J x)
+        worst_pair = 0.0    # largest residual of a complex pair below the highest reported root
         for r in range(k):
-            y = Y[:, r].real
-            y = y / np.linalg.norm(y)
-            x = y @ V
-            q = y @ S - lam[r].real * x
-            self.omega[r] = lam[r].real
-            self.res[r] = np.abs(q).max()
-            ritz.append((x, y @ S))
-            if self.res[r] >= eps:
-                den = lam[r].real - self.D
-                den = np.where(np.abs(den) < 1e-3, np.copysign(1e-3, den), den)
-                corrections.append(q / den)
-        if k == self.nwork and np.all(self.res[:k] < eps):
+            y = Y[:, r] / np.linalg.norm(Y[:, r])
+            if isreal[r]:
+                y = y.real / np.linalg.norm(y.real)
+                x = y @ V
+                q = y @ S - lam[r].real * x
+                res = np.abs(q).max()
+                reals.append((lam[r].real, res, x, y @ S))
+                parts = [q] if res >= eps else []
+            else:
+                x = y @ V
+                sx = y @ S
+                q = sx - lam[r] * x
+                res = np.abs(q).max()
+                if len(reals) < self.nroots:
+                    worst_pair = max(worst_pair, res)
+                pairs.extend([(x.real, sx.real), (x.imag, sx.imag)])
+                parts = [q.real, q.imag] if res >= eps else []
+            den = lam[r].real - self.D
+            den = np.where(np.abs(den) < 1e-2, np.copysign(1e-2, den), den)
+            corrections.extend(c / den for c in parts)
+        for r, (val, res, _, _) in enumerate(reals[:self.nroots]):
+            self.omega[r] = val
+            self.res[r] = res
+        ritz = [(x, sx) for _, _, x, sx in reals] + pairs
+        nr = min(self.nroots, self.dim)
+        if len(reals) >= nr and all(res < eps for _, res, _, _ in reals[:nr]) and worst_pair < 1e-3:
             self.done = True
             return
-        if len(V) >= self.dim:   # the subspace spans the space: Ritz values are exact
+        if self.k >= self.dim:   # the subspace spans the space: Ritz values are exact
             self.done = True
             return
-        if len(V) + len(corrections) > self.max_space:
-            # collapse onto the orthonormalised Ritz vectors (their sigmas follow linearly)
-            self.V, self.S = [], []
-            for x, sx in ritz:
-                for b, sb in zip(self.V, self.S):
-                    c = b @ x
-                    x = x - c * b
-                    sx = sx - c * sb
+        if self.k + len(corrections) > self.max_space:
+            # collapse onto the orthonormalised Ritz vectors of this step and
+            # of the last one (their sigmas follow linearly): the last step's
+            # vectors keep the direction of approach, which a restart onto
+            # the current ones alone would lose
+            basis = ritz + self.prev
+            self.k = 0
+            for x, sx in basis:
+                n0 = np.linalg.norm(x)
+                for _ in range(2):
+                    c = self.Vm[:self.k] @ x
+                    x = x - c @ self.Vm[:self.k]
+                    sx = sx - c @ self.Sm[:self.k]
                 nrm = np.linalg.norm(x)
-                if nrm > 1e-8:
-                    self.V.append(x / nrm)
-                    self.S.append(sx / nrm)
+                if nrm > 1e-6 * n0:
+                    self.Vm[self.k] = x / nrm
+                    self.Sm[self.k] = sx / nrm
+                    self.k += 1
+            self.G = self.Vm[:self.k] @ self.Sm[:self.k].T
+        self.prev = ritz
         for c in corrections:
             n0 = np.linalg.norm(c)
             c = self._orth(c)
             nrm = np.linalg.norm(c)
             # keep a correction unless the subspace already contains it
-            if nrm > 1e-3 * n0 and len(self.V) + len(self.new) < self.dim:
+            if nrm > 1e-3 * n0 and self.k + len(self.new) < self.dim:
                 self.new.append(c / nrm)
         # fewer real Ritz values than working roots: widen the subspace along
         # the next diagonal directions
-        while k + len(self.new) < self.nwork and len(self.V) + len(self.new) < self.dim:
+        while k + len(self.new) < self.nwork and self.k + len(self.new) < self.dim:
             x = self._seed()
             if x is None:
                 break
@@ -171,13 +232,14 @@ class _Davidson:
             self.done = True
 
 
-def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_space=48):
+def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_space=48, log=None):
     """Lowest ``nroots`` singlet LR-CCSD excitation energies (eV) of every
     loaded molecule, after a converged solve of ``ctx``'s batch.
 
     dims: (n_occ, n_virt) of the loaded molecules in load order.
     Returns (omega [nmol, nroots], residual max-norms [nmol, nroots],
-    iterations [nmol], converged [nmol])."""
+    iterations [nmol], converged [nmol]).  ``log(iteration, active)`` is called
+    after every Davidson iteration when given."""
     subs = []
     for k, (o, v) in enumerate(dims):
         _, Fmo, _ = ctx.batch_orbitals(k, o + v)
@@ -209,11 +271,9 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
             s.absorb(sig[k])
             s.step(eps)
             its[k] = it
-    omega = np.full((len(subs), nroots), np.nan)
-    res = np.full((len(subs), nroots), np.inf)
-    for k, s in enumerate(subs):
-        m = min(nroots, s.nwork)
-        omega[k, :m] = s.omega[:m]
-        res[k, :m] = s.res[:m]
-    conv = np.array([bool(np.all(s.res < eps)) or len(s.V) >= s.dim for s in subs])
+        if log is not None:
+            log(it, sum(not s.done for s in subs))
+    omega = np.array([s.omega for s in subs])
+    res = np.array([s.res for s in subs])
+    conv = np.array([bool(np.all(s.res[:min(nroots, s.dim)] < eps)) or s.k >= s.dim for s in subs])
     return omega, res, its, conv
