@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libhelios_ccsd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
 ARCH = "gfx950"
-HIP_SOURCES = ["hcc_gemm.hip", "hcc_ops.hip", "hcc_solver.hip", "hcc_mlp.hip", "hcc_scf.hip", "hcc_triples.hip"]
+HIP_SOURCES = ["hcc_gemm.hip", "hcc_ops.hip", "hcc_solver.hip", "hcc_mlp.hip", "hcc_scf.hip", "hcc_triples.hip",
+               "hcc_lr.hip"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include")]

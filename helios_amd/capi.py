@@ -48,6 +48,13 @@ SIGNATURES = {
     "hcc_batch_get_orbitals": (_i, [_vp, _i, _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_triples": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_batch_lr_sigma": (_i, [_vp, _d, _P(_d), _P(_d), _P(_d), _P(_d)]),
+    "hcc_lr_space": (_i, [_vp, _i]),
+    "hcc_lr_put": (_i, [_vp, _i, _i, _P(_d)]),
+    "hcc_lr_get": (_i, [_vp, _i, _i, _P(_d)]),
+    "hcc_lr_dots": (_i, [_vp, _i, _P(_i), _i, _P(_d)]),
+    "hcc_lr_combine": (_i, [_vp, _i, _P(_i), _P(_d), _P(_i), _P(_d)]),
+    "hcc_lr_precondition": (_i, [_vp, _i, _P(_i), _P(_d), _d]),
+    "hcc_lr_sigma_slots": (_i, [_vp, _d, _P(_i), _P(_i)]),
     "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
@@ -270,6 +277,51 @@ class Context:
         self._check(self.lib.hcc_batch_lr_sigma(self.h, float(h), _dp(r1), _dp(r2), _dp(s1), _dp(s2)),
                     "hcc_batch_lr_sigma")
         return s1, s2
+
+    # device-resident vector space of the LR-CCSD eigensolver
+    def lr_space(self, nslots):
+        self._check(self.lib.hcc_lr_space(self.h, int(nslots)), "hcc_lr_space")
+
+    def lr_put(self, mol, slot, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self._check(self.lib.hcc_lr_put(self.h, int(mol), int(slot), _dp(x)), "hcc_lr_put")
+
+    def lr_get(self, mol, slot, n):
+        x = np.zeros(n)
+        self._check(self.lib.hcc_lr_get(self.h, int(mol), int(slot), _dp(x)), "hcc_lr_get")
+        return x
+
+    def lr_dots(self, jobs, mode=0):
+        """jobs: (J, 3) ints (mol, slot a, slot b) -> J values."""
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros(len(jobs))
+        if len(jobs):
+            self._check(self.lib.hcc_lr_dots(self.h, len(jobs), _ip(jobs), int(mode), _dp(out)), "hcc_lr_dots")
+        return out
+
+    def lr_combine(self, jobs, beta, term_slot, term_coef):
+        """jobs: (J, 4) ints (mol, dst, first term, count); dst = beta dst + sum coef * slot."""
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 4)
+        if not len(jobs):
+            return
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        ts = np.ascontiguousarray(term_slot if len(term_slot) else [0], dtype=np.int32)
+        tc = np.ascontiguousarray(term_coef if len(term_coef) else [0.0], dtype=np.float64)
+        self._check(self.lib.hcc_lr_combine(self.h, len(jobs), _ip(jobs), _dp(beta), _ip(ts), _dp(tc)),
+                    "hcc_lr_combine")
+
+    def lr_precondition(self, jobs, shift, floor):
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 2)
+        if not len(jobs):
+            return
+        shift = np.ascontiguousarray(shift, dtype=np.float64)
+        self._check(self.lib.hcc_lr_precondition(self.h, len(jobs), _ip(jobs), _dp(shift), float(floor)),
+                    "hcc_lr_precondition")
+
+    def lr_sigma_slots(self, in_slot, out_slot, h):
+        a = np.ascontiguousarray(in_slot, dtype=np.int32)
+        b = np.ascontiguousarray(out_slot, dtype=np.int32)
+        self._check(self.lib.hcc_lr_sigma_slots(self.h, float(h), _ip(a), _ip(b)), "hcc_lr_sigma_slots")
 
     def batch_orbitals(self, mol, n):
         """(C, F_mo, e_ref) of loaded molecule `mol` with n sites."""

@@ -50,10 +50,9 @@ void activate(State* st, int* act, int nm, hipStream_t s);
 void stop_running(State* st, int* act, int nm, hipStream_t s);
 void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s);
 void lr_perturb(const Mol* mols, Ragged rgT2, int nm, const double* t1base, const double* t1b, const double* t2base,
-                const double* t2b, const long* off1, const long* off2, const double* r1, const double* r2, double h,
-                hipStream_t s);
-void lr_omega(const Mol* mols, Ragged rgT2, int nm, const long* off1, const long* off2, double* o1, double* o2,
-              bool minus, double scale, hipStream_t s);
+                const double* t2b, const double* const* r1p, const double* const* r2p, double h, hipStream_t s);
+void lr_omega(const Mol* mols, Ragged rgT2, int nm, double* const* o1p, double* const* o2p, bool minus, double scale,
+              hipStream_t s);
 
 // mixed-precision solver (correction form)
 void mixed_dress(const Mol* m0, const MolD* md, const State* st, Ragged rgNN, Ragged rgN, Ragged rgN3, int nm,

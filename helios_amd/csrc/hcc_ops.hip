@@ -927,52 +927,54 @@ void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s
   hipLaunchKernelGGL(k_arm_all, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, save, nm, restore ? 1 : 0);
 }
 // Linear response: t = t0 + h r (t0 saved in the backup arenas at the same
-// offsets; r given per molecule, concatenated: r1 [i][a] at off1[mol], r2
-// [i][j][a][b] at off2[mol])
+// offsets; r given per molecule as device pointers to its r1 [i][a] and r2
+// [i][j][a][b] blocks; a null r1 leaves the molecule's amplitudes as they are)
 __global__ __launch_bounds__(kEwThreads) void k_lr_perturb(const Mol* __restrict__ mols, Ragged rg, int nm,
                                                           const double* __restrict__ t1base, const double* __restrict__ t1b,
                                                           const double* __restrict__ t2base, const double* __restrict__ t2b,
-                                                          const long* __restrict__ off1, const long* __restrict__ off2,
-                                                          const double* __restrict__ r1, const double* __restrict__ r2,
-                                                          double h) {
+                                                          const double* const* __restrict__ r1p,
+                                                          const double* const* __restrict__ r2p, double h) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_) return;
+  const double* r1 = r1p[mol];
+  if (r1 == nullptr) return;
+  const double* r2 = r2p[mol];
   const long n1 = (long)M.o * M.v;
   const long b2 = M.t2 - t2base;
-  M.t2[e] = t2b[b2 + e] + h * r2[off2[mol] + e];
+  M.t2[e] = t2b[b2 + e] + h * r2[e];
   if (e < n1) {
     const long b1 = M.t1 - t1base;
-    M.t1[e] = t1b[b1 + e] + h * r1[off1[mol] + e];
+    M.t1[e] = t1b[b1 + e] + h * r1[e];
   }
 }
 // Omega of the current residual pass (Omega1 = w1 + F~_ai, Omega2 = w2) into
-// the concatenated outputs; with `minus`, out = (out - Omega) * scale instead
+// the per-molecule outputs (null: skipped); with `minus`, out = (out - Omega) * scale
 __global__ __launch_bounds__(kEwThreads) void k_lr_omega(const Mol* __restrict__ mols, Ragged rg, int nm,
-                                                        const long* __restrict__ off1, const long* __restrict__ off2,
-                                                        double* __restrict__ o1, double* __restrict__ o2, int minus,
-                                                        double scale) {
+                                                        double* const* __restrict__ o1p, double* const* __restrict__ o2p,
+                                                        int minus, double scale) {
   RAGGED_PROLOGUE((long)mols[mol].o * mols[mol].o * mols[mol].v * mols[mol].v);
   if (e >= size_) return;
+  double* o1 = o1p[mol];
+  if (o1 == nullptr) return;
   const long n1 = (long)M.o * M.v;
   const double w2 = M.w2[e];
-  double* q2 = o2 + off2[mol] + e;
+  double* q2 = o2p[mol] + e;
   *q2 = minus ? (*q2 - w2) * scale : w2;
   if (e < n1) {
     const int N = M.n, o = M.o, v = M.v;
     const int i = (int)(e / v), a = (int)(e % v);
     const double w1 = M.w1[e] + M.Ft[(long)(o + a) * N + i];
-    double* q1 = o1 + off1[mol] + e;
+    double* q1 = o1 + e;
     *q1 = minus ? (*q1 - w1) * scale : w1;
   }
 }
 void lr_perturb(const Mol* mols, Ragged rgT2, int nm, const double* t1base, const double* t1b, const double* t2base,
-                const double* t2b, const long* off1, const long* off2, const double* r1, const double* r2, double h,
-                hipStream_t s) {
-  RAGGED_LAUNCH(k_lr_perturb, rgT2, mols, rgT2, nm, t1base, t1b, t2base, t2b, off1, off2, r1, r2, h);
+                const double* t2b, const double* const* r1p, const double* const* r2p, double h, hipStream_t s) {
+  RAGGED_LAUNCH(k_lr_perturb, rgT2, mols, rgT2, nm, t1base, t1b, t2base, t2b, r1p, r2p, h);
 }
-void lr_omega(const Mol* mols, Ragged rgT2, int nm, const long* off1, const long* off2, double* o1, double* o2,
-              bool minus, double scale, hipStream_t s) {
-  RAGGED_LAUNCH(k_lr_omega, rgT2, mols, rgT2, nm, off1, off2, o1, o2, minus ? 1 : 0, scale);
+void lr_omega(const Mol* mols, Ragged rgT2, int nm, double* const* o1p, double* const* o2p, bool minus, double scale,
+              hipStream_t s) {
+  RAGGED_LAUNCH(k_lr_omega, rgT2, mols, rgT2, nm, o1p, o2p, minus ? 1 : 0, scale);
 }
 void stop_running(State* st, int* act, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_stop_running, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, nm);

@@ -36,6 +36,7 @@
 #include "hcc_mlp.h"
 #include "hcc_scf.h"
 #include "hcc_triples.h"
+#include "hcc_lr.h"
 
 namespace hcc {
 
@@ -163,11 +164,22 @@ struct Batch {
   int2* dtri = nullptr;
   int* dtsave = nullptr;
   double *dtw = nullptr, *dtpart = nullptr, *dtout = nullptr;
-  // linear response: concatenated trial/sigma vectors, amplitude backups
-  long* dlr_off = nullptr;   // off1 (nm), off2 (nm)
+  // linear response: concatenated trial/sigma vectors, amplitude backups, and
+  // the per-molecule pointers (r1, r2, s1, s2; nm each) the sigma kernels read
   double *dlr_r1 = nullptr, *dlr_r2 = nullptr, *dlr_s1 = nullptr, *dlr_s2 = nullptr;
   double *dlr_t1b = nullptr, *dlr_t2b = nullptr;
+  double** dlr_pc = nullptr;   // pointers into the concatenated buffers
+  double** dlr_ps = nullptr;   // pointers into the vector space (rewritten per call)
   long lr_n1 = 0, lr_n2 = 0, lr_span1 = 0, lr_span2 = 0;
+  // device-resident vector space of the eigensolver (hcc_lr.h): lr_slots
+  // vectors of lr_len[k] doubles per molecule from element lr_base[k]; own
+  // allocations (not in `allocs`), replaced when the slot count changes
+  int lr_slots = 0;
+  double* dlr_space = nullptr;
+  std::vector<long> lr_len, lr_base;
+  long *dlr_len = nullptr, *dlr_base = nullptr;
+  char* dlr_stage = nullptr;   // job lists and results of the space calls
+  size_t lr_stage_cap = 0;
   long ttotal = 0, twstride = 0;
   int tgrid = 0, tgrid_mfma = 0;
   bool tglobal = false;
@@ -183,6 +195,12 @@ struct Batch {
     if (arena == nullptr)
       for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
+    if (dlr_space) (void)hipFree(dlr_space);
+    if (dlr_stage) (void)hipFree(dlr_stage);
+    dlr_space = nullptr;
+    dlr_stage = nullptr;
+    lr_slots = 0;
+    lr_stage_cap = 0;
     arena_used = 0;
     dm = nullptr;
     dst = nullptr;
@@ -1527,69 +1545,322 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t) {
 
 // Buffers of the linear-response sigma products (once per batch).
 static void setup_lr(Batch& b) {
-  std::vector<long> off(2 * (size_t)b.nm);
+  const int nm = b.nm;
+  std::vector<double*> pc(4 * (size_t)nm);
   long n1 = 0, n2 = 0;
-  for (int k = 0; k < b.nm; ++k) {
-    off[k] = n1;
-    off[b.nm + k] = n2;
+  for (int k = 0; k < nm; ++k) {
     n1 += (long)b.o[k] * b.v[k];
     n2 += (long)b.o[k] * b.o[k] * b.v[k] * b.v[k];
   }
   const Mol& first = b.hm[0];
-  const Mol& last = b.hm[b.nm - 1];
+  const Mol& last = b.hm[nm - 1];
   b.lr_n1 = n1;
   b.lr_n2 = n2;
   b.lr_span1 = (long)(last.t1 - first.t1) + (long)last.o * last.v;
   b.lr_span2 = (long)(last.t2 - first.t2) + (long)last.o * last.o * last.v * last.v;
-  b.dlr_off = b.dalloc<long>(off.size());
   b.dlr_r1 = b.dalloc<double>(n1);
   b.dlr_r2 = b.dalloc<double>(n2);
   b.dlr_s1 = b.dalloc<double>(n1);
   b.dlr_s2 = b.dalloc<double>(n2);
   b.dlr_t1b = b.dalloc<double>(b.lr_span1);
   b.dlr_t2b = b.dalloc<double>(b.lr_span2);
-  HCHECK(hipMemcpy(b.dlr_off, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice));
+  long a1 = 0, a2 = 0;
+  for (int k = 0; k < nm; ++k) {
+    pc[k] = b.dlr_r1 + a1;
+    pc[nm + k] = b.dlr_r2 + a2;
+    pc[2 * nm + k] = b.dlr_s1 + a1;
+    pc[3 * nm + k] = b.dlr_s2 + a2;
+    a1 += (long)b.o[k] * b.v[k];
+    a2 += (long)b.o[k] * b.o[k] * b.v[k] * b.v[k];
+  }
+  b.dlr_pc = b.dalloc<double*>(pc.size());
+  b.dlr_ps = b.dalloc<double*>(pc.size());
+  HCHECK(hipMemcpy(b.dlr_pc, pc.data(), pc.size() * sizeof(double*), hipMemcpyHostToDevice));
+  if (b.dtsave == nullptr) b.dtsave = b.dalloc<int>(nm);
+}
+
+static int lr_ready(hcc_ctx* ctx) {
+  if (int r = need_batch(ctx)) return r;
+  if (!ctx->have_guess) {
+    ctx->err = "linear response needs amplitudes: solve first";
+    return HCC_E_STATE;
+  }
+  return HCC_OK;
+}
+
+// sigma = (Omega(t0 + h r) - Omega(t0 - h r)) / 2h for every molecule whose r1
+// pointer is set; ptrs (device) = r1 (nm), r2 (nm), sigma1 (nm), sigma2 (nm).
+// The amplitudes and the active flags are restored.
+static void lr_sigma_run(hcc_ctx* ctx, double h, double* const* ptrs) {
+  Batch& b = *ctx->batch;
+  hipStream_t s = ctx->stream;
+  const int nm = b.nm;
+  const Mol& first = b.hm[0];
+  const double* const* r1p = ptrs;
+  const double* const* r2p = ptrs + nm;
+  HCHECK(hipMemcpyAsync(b.dlr_t1b, first.t1, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HCHECK(hipMemcpyAsync(b.dlr_t2b, first.t2, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  arm_all(b.dst, b.dact, b.dtsave, nm, false, s);
+  lr_perturb(b.dm, b.rgT2, nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, r1p, r2p, h, s);
+  residual(ctx);
+  lr_omega(b.dm, b.rgT2, nm, ptrs + 2 * nm, ptrs + 3 * nm, false, 1.0, s);
+  lr_perturb(b.dm, b.rgT2, nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, r1p, r2p, -h, s);
+  residual(ctx);
+  lr_omega(b.dm, b.rgT2, nm, ptrs + 2 * nm, ptrs + 3 * nm, true, 0.5 / h, s);
+  HCHECK(hipMemcpyAsync(first.t1, b.dlr_t1b, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HCHECK(hipMemcpyAsync(first.t2, b.dlr_t2b, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  arm_all(b.dst, b.dact, b.dtsave, nm, true, s);
+  HCHECK(hipGetLastError());
 }
 
 int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r2, double* s1, double* s2) {
   if (ctx == nullptr || r1 == nullptr || r2 == nullptr || s1 == nullptr || s2 == nullptr) return HCC_E_ARG;
-  if (int r = need_batch(ctx)) return r;
+  if (int r = lr_ready(ctx)) return r;
   if (!(h > 0.0)) {
     ctx->err = "the difference step must be positive";
     return HCC_E_ARG;
-  }
-  if (!ctx->have_guess) {
-    ctx->err = "linear response needs amplitudes: solve first";
-    return HCC_E_STATE;
   }
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
-  if (b.dlr_off == nullptr) setup_lr(b);
-  if (b.dtsave == nullptr) b.dtsave = b.dalloc<int>(b.nm);
-  const Mol& first = b.hm[0];
-  const long* off1 = b.dlr_off;
-  const long* off2 = b.dlr_off + b.nm;
+  if (b.dlr_pc == nullptr) setup_lr(b);
   HCHECK(hipMemcpyAsync(b.dlr_r1, r1, b.lr_n1 * sizeof(double), hipMemcpyHostToDevice, s));
   HCHECK(hipMemcpyAsync(b.dlr_r2, r2, b.lr_n2 * sizeof(double), hipMemcpyHostToDevice, s));
-  HCHECK(hipMemcpyAsync(b.dlr_t1b, first.t1, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HCHECK(hipMemcpyAsync(b.dlr_t2b, first.t2, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  arm_all(b.dst, b.dact, b.dtsave, b.nm, false, s);
-  // sigma = (Omega(t0 + h r) - Omega(t0 - h r)) / 2h
-  lr_perturb(b.dm, b.rgT2, b.nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, off1, off2, b.dlr_r1, b.dlr_r2, h, s);
-  residual(ctx);
-  lr_omega(b.dm, b.rgT2, b.nm, off1, off2, b.dlr_s1, b.dlr_s2, false, 1.0, s);
-  lr_perturb(b.dm, b.rgT2, b.nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, off1, off2, b.dlr_r1, b.dlr_r2, -h, s);
-  residual(ctx);
-  lr_omega(b.dm, b.rgT2, b.nm, off1, off2, b.dlr_s1, b.dlr_s2, true, 0.5 / h, s);
-  HCHECK(hipMemcpyAsync(first.t1, b.dlr_t1b, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HCHECK(hipMemcpyAsync(first.t2, b.dlr_t2b, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  arm_all(b.dst, b.dact, b.dtsave, b.nm, true, s);
-  HCHECK(hipGetLastError());
+  lr_sigma_run(ctx, h, b.dlr_pc);
   HCHECK(hipMemcpyAsync(s1, b.dlr_s1, b.lr_n1 * sizeof(double), hipMemcpyDeviceToHost, s));
   HCHECK(hipMemcpyAsync(s2, b.dlr_s2, b.lr_n2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident vector space of the eigensolver (hcc_lr.h)
+// ---------------------------------------------------------------------------
+int hcc_lr_space(hcc_ctx* ctx, int nslots) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = lr_ready(ctx)) return r;
+  if (nslots < 1) {
+    ctx->err = "the vector space needs at least one slot";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  Batch& b = *ctx->batch;
+  if (b.dlr_pc == nullptr) setup_lr(b);
+  const int nm = b.nm;
+  b.lr_len.assign(nm, 0);
+  b.lr_base.assign(nm, 0);
+  long total = 0;
+  for (int k = 0; k < nm; ++k) {
+    b.lr_len[k] = (long)b.o[k] * b.v[k] + (long)b.o[k] * b.o[k] * b.v[k] * b.v[k];
+    b.lr_base[k] = total;
+    total += (b.lr_len[k] * nslots + 31) / 32 * 32;   // 256-byte aligned molecule regions
+  }
+  if (b.dlr_space) HCHECK(hipFree(b.dlr_space));
+  b.dlr_space = nullptr;
+  b.lr_slots = 0;
+  HCHECK(hipMalloc(&b.dlr_space, (size_t)total * sizeof(double)));
+  HCHECK(hipMemsetAsync(b.dlr_space, 0, (size_t)total * sizeof(double), ctx->stream));
+  if (b.dlr_len == nullptr) {
+    b.dlr_len = b.dalloc<long>(nm);
+    b.dlr_base = b.dalloc<long>(nm);
+  }
+  HCHECK(hipMemcpy(b.dlr_len, b.lr_len.data(), nm * sizeof(long), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(b.dlr_base, b.lr_base.data(), nm * sizeof(long), hipMemcpyHostToDevice));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  b.lr_slots = nslots;
+  API_END
+}
+
+static int lr_space_ready(hcc_ctx* ctx) {
+  if (int r = lr_ready(ctx)) return r;
+  if (ctx->batch->lr_slots == 0) {
+    ctx->err = "no vector space: call hcc_lr_space first";
+    return HCC_E_STATE;
+  }
+  return HCC_OK;
+}
+
+static bool lr_slot_ok(const Batch& b, int mol, int slot) {
+  return mol >= 0 && mol < b.nm && slot >= 0 && slot < b.lr_slots;
+}
+
+// staging region of at least `bytes` (grown by doubling; contents not kept)
+static char* lr_stage(Batch& b, size_t bytes) {
+  if (bytes > b.lr_stage_cap) {
+    if (b.dlr_stage) HCHECK(hipFree(b.dlr_stage));
+    b.dlr_stage = nullptr;
+    size_t cap = std::max<size_t>(bytes, 2 * b.lr_stage_cap);
+    HCHECK(hipMalloc(&b.dlr_stage, cap));
+    b.lr_stage_cap = cap;
+  }
+  return b.dlr_stage;
+}
+static size_t lr_align(size_t x) { return (x + 255) / 256 * 256; }
+
+int hcc_lr_put(hcc_ctx* ctx, int mol, int slot, const double* x) {
+  if (ctx == nullptr || x == nullptr) return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  if (!lr_slot_ok(b, mol, slot)) {
+    ctx->err = "molecule or slot out of range";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  HCHECK(hipMemcpyAsync(b.dlr_space + b.lr_base[mol] + (long)slot * b.lr_len[mol], x, b.lr_len[mol] * sizeof(double),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int hcc_lr_get(hcc_ctx* ctx, int mol, int slot, double* x) {
+  if (ctx == nullptr || x == nullptr) return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  if (!lr_slot_ok(b, mol, slot)) {
+    ctx->err = "molecule or slot out of range";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  HCHECK(hipMemcpyAsync(x, b.dlr_space + b.lr_base[mol] + (long)slot * b.lr_len[mol], b.lr_len[mol] * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int hcc_lr_dots(hcc_ctx* ctx, int njobs, const int* jobs, int mode, double* out) {
+  if (ctx == nullptr || njobs < 0 || (njobs > 0 && (jobs == nullptr || out == nullptr)) || (mode != 0 && mode != 1))
+    return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  for (int j = 0; j < njobs; ++j) {
+    const int mol = jobs[3 * j], sa = jobs[3 * j + 1], sb = jobs[3 * j + 2];
+    if (!lr_slot_ok(b, mol, sa) || !(lr_slot_ok(b, mol, sb) || (mode == 1 && sb == -1))) {
+      ctx->err = "dot job " + std::to_string(j) + ": molecule or slot out of range";
+      return HCC_E_ARG;
+    }
+  }
+  if (njobs == 0) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t jb = lr_align(3 * sizeof(int) * (size_t)njobs);
+  char* st = lr_stage(b, jb + sizeof(double) * (size_t)njobs);
+  HCHECK(hipMemcpyAsync(st, jobs, 3 * sizeof(int) * (size_t)njobs, hipMemcpyHostToDevice, s));
+  double* dout = reinterpret_cast<double*>(st + jb);
+  lr_dots(b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st), mode, dout, s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)njobs, hipMemcpyDeviceToHost, s));
+  HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+int hcc_lr_combine(hcc_ctx* ctx, int njobs, const int* jobs, const double* beta, const int* term_slot,
+                   const double* term_coef) {
+  if (ctx == nullptr || njobs < 0 || (njobs > 0 && (jobs == nullptr || beta == nullptr))) return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  long nterms = 0;
+  std::vector<int> prefix(njobs + 1, 0);
+  for (int j = 0; j < njobs; ++j) {
+    const int mol = jobs[4 * j], dst = jobs[4 * j + 1], first = jobs[4 * j + 2], cnt = jobs[4 * j + 3];
+    if (!lr_slot_ok(b, mol, dst) || first < 0 || cnt < 0 || (cnt > 0 && (term_slot == nullptr || term_coef == nullptr))) {
+      ctx->err = "combine job " + std::to_string(j) + ": invalid molecule, slot or term range";
+      return HCC_E_ARG;
+    }
+    for (int t = first; t < first + cnt; ++t)
+      if (!lr_slot_ok(b, mol, term_slot[t]) || term_slot[t] == dst) {
+        ctx->err = "combine job " + std::to_string(j) + ": term slot out of range or equal to the destination";
+        return HCC_E_ARG;
+      }
+    nterms = std::max<long>(nterms, (long)first + cnt);
+    prefix[j + 1] = prefix[j] + (int)((b.lr_len[mol] + kLrChunk - 1) / kLrChunk);
+  }
+  if (njobs == 0) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t o_pre = 0, o_job = lr_align(sizeof(int) * (njobs + 1)), o_beta = o_job + lr_align(sizeof(int4) * njobs),
+               o_ts = o_beta + lr_align(sizeof(double) * njobs), o_tc = o_ts + lr_align(sizeof(int) * (nterms + 1)),
+               total = o_tc + sizeof(double) * (nterms + 1);
+  char* st = lr_stage(b, total);
+  HCHECK(hipMemcpyAsync(st + o_pre, prefix.data(), sizeof(int) * (njobs + 1), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_job, jobs, sizeof(int4) * njobs, hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_beta, beta, sizeof(double) * njobs, hipMemcpyHostToDevice, s));
+  if (nterms > 0) {
+    HCHECK(hipMemcpyAsync(st + o_ts, term_slot, sizeof(int) * nterms, hipMemcpyHostToDevice, s));
+    HCHECK(hipMemcpyAsync(st + o_tc, term_coef, sizeof(double) * nterms, hipMemcpyHostToDevice, s));
+  }
+  lr_combine(b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st + o_pre),
+             reinterpret_cast<int4*>(st + o_job), reinterpret_cast<double*>(st + o_beta),
+             reinterpret_cast<int*>(st + o_ts), reinterpret_cast<double*>(st + o_tc), prefix[njobs], s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+int hcc_lr_precondition(hcc_ctx* ctx, int njobs, const int* jobs, const double* shift, double floor) {
+  if (ctx == nullptr || njobs < 0 || (njobs > 0 && (jobs == nullptr || shift == nullptr)) || !(floor > 0.0))
+    return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  std::vector<int> prefix(njobs + 1, 0);
+  for (int j = 0; j < njobs; ++j) {
+    if (!lr_slot_ok(b, jobs[2 * j], jobs[2 * j + 1])) {
+      ctx->err = "precondition job " + std::to_string(j) + ": molecule or slot out of range";
+      return HCC_E_ARG;
+    }
+    prefix[j + 1] = prefix[j] + (int)((b.lr_len[jobs[2 * j]] + kLrChunk - 1) / kLrChunk);
+  }
+  if (njobs == 0) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t o_job = lr_align(sizeof(int) * (njobs + 1)), o_sh = o_job + lr_align(sizeof(int2) * njobs),
+               total = o_sh + sizeof(double) * njobs;
+  char* st = lr_stage(b, total);
+  HCHECK(hipMemcpyAsync(st, prefix.data(), sizeof(int) * (njobs + 1), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_job, jobs, sizeof(int2) * njobs, hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_sh, shift, sizeof(double) * njobs, hipMemcpyHostToDevice, s));
+  lr_precondition(b.dm, b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st),
+                  reinterpret_cast<int2*>(st + o_job), reinterpret_cast<double*>(st + o_sh), floor, prefix[njobs], s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* out_slot) {
+  if (ctx == nullptr || in_slot == nullptr || out_slot == nullptr) return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  if (!(h > 0.0)) {
+    ctx->err = "the difference step must be positive";
+    return HCC_E_ARG;
+  }
+  Batch& b = *ctx->batch;
+  const int nm = b.nm;
+  std::vector<double*> ptrs(4 * (size_t)nm, nullptr);
+  for (int k = 0; k < nm; ++k) {
+    if (in_slot[k] < 0) continue;
+    if (!lr_slot_ok(b, k, in_slot[k]) || !lr_slot_ok(b, k, out_slot[k]) || in_slot[k] == out_slot[k]) {
+      ctx->err = "sigma slots of molecule " + std::to_string(k) + " out of range or equal";
+      return HCC_E_ARG;
+    }
+    const long n1 = (long)b.o[k] * b.v[k];
+    double* r = b.dlr_space + b.lr_base[k] + (long)in_slot[k] * b.lr_len[k];
+    double* q = b.dlr_space + b.lr_base[k] + (long)out_slot[k] * b.lr_len[k];
+    ptrs[k] = r;
+    ptrs[nm + k] = r + n1;
+    ptrs[2 * nm + k] = q;
+    ptrs[3 * nm + k] = q + n1;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  HCHECK(hipMemcpyAsync(b.dlr_ps, ptrs.data(), ptrs.size() * sizeof(double*), hipMemcpyHostToDevice, ctx->stream));
+  lr_sigma_run(ctx, h, b.dlr_ps);
+  HCHECK(hipStreamSynchronize(ctx->stream));
   API_END
 }
 

@@ -10,79 +10,122 @@ at convergence omega is an eigenvalue of the CCSD Jacobian -- the LR-CCSD
 excitation energy -- and ``energyLR`` (:330-389) reports it.
 
 Here the same eigenvalues come from the closed-shell (singlet) Jacobian
-J = dOmega/dt at the converged amplitudes, applied on the GPU for every
-molecule of the batch at once (``hcc_batch_lr_sigma``: central differences
-of the FP64 residual), with a block Davidson solve on the host for the
-``nroots`` lowest roots of each molecule: Rayleigh-Ritz on an orthonormal
-subspace, residual corrections preconditioned by the orbital-energy
-differences, subspace collapse onto the Ritz vectors when it grows too
-large.  Trial vectors start as unit singles on the lowest orbital-energy
-differences (the reference's ``unt`` guess).  The difference step h = 1e-5
-keeps sigma within about 1e-9 of J r (the truncation error is O(h^2) times
-the third derivative of a quartic polynomial; the FP64 rounding error is
-about 1e-16 |Omega| / h).  Triplets (spin = 1) need the
-spin-orbital Jacobian and are not provided.
+J = dOmega/dt at the converged amplitudes, with a block Davidson solve for
+the ``nroots`` lowest roots of every molecule of the batch at once:
+Rayleigh-Ritz on an orthonormal subspace, residual corrections
+preconditioned by the orbital-energy differences, complex Ritz pairs kept in
+the working set, restarts onto the current and previous Ritz vectors.  The
+subspace lives in device memory (``hcc_lr_space``): the Jacobian products
+(``hcc_lr_sigma_slots``: central differences of the FP64 residual), the
+dot products, linear combinations and the preconditioner run as batched
+kernels over all molecules, and the host solves only the small projected
+eigenproblems and decides what to combine next.  Trial vectors start as the
+lowest CIS singlets (the reference's default ``cis`` guess) or as unit
+singles (its ``unt``).  The difference step h = 1e-5 keeps sigma within
+about 1e-9 of J r (the truncation error is O(h^2) times the third
+derivative of a quartic polynomial; the FP64 rounding error is about
+1e-16 |Omega| / h).  Triplets (spin = 1) need the spin-orbital Jacobian and
+are not provided.
 """
 import numpy as np
 
-from helios_amd import capi
+
+def cis_singlets(C, gamma, e, o):
+    """Singlet CIS of the RHF reference: eigenvalues (ascending) and
+    eigenvectors (columns over (i, a), i slowest) of
+    A[(i a), (j b)] = (e_a - e_i) d_ij d_ab + 2 [ai|jb] - [ab|ji],
+    [pq|rs] = sum_mn C_mp C_mq gamma_mn C_nr C_ns -- the matrix the reference
+    diagonalises for its 'cis' guess (mod_lrccsd.f90:1253-1282 with
+    spat_hf_int, :1423-1445)."""
+    n = C.shape[0]
+    v = n - o
+    Co, Cv = C[:, :o], C[:, o:]
+    X = (Co[:, :, None] * Cv[:, None, :]).reshape(n, o * v)       # C_mi C_ma
+    coul = X.T @ gamma @ X                                          # [ai|jb]
+    Y = (Cv[:, :, None] * Cv[:, None, :]).reshape(n, v * v)       # C_ma C_mb
+    Z = (Co[:, :, None] * Co[:, None, :]).reshape(n, o * o)       # C_mi C_mj
+    exch = (Y.T @ gamma @ Z).reshape(v, v, o, o).transpose(2, 0, 3, 1).reshape(o * v, o * v)   # [ab|ij]
+    A = 2.0 * coul - exch
+    A[np.diag_indices(o * v)] += (e[o:][None, :] - e[:o][:, None]).ravel()
+    return np.linalg.eigh(A)
 
 
-class _Davidson:
-    """Subspace of one molecule: orthonormal basis rows Vm[:k], sigma rows Sm[:k]."""
+def average_excitation_level(r1, r2):
+    """The reference's average excitation level of a state,
+    (S1 + 2 S2) / (S1 + S2) with S1 = sum r1^2 and S2 = sum_{i<j, a<b} r2^2 over
+    spin orbitals (mod_lrccsd.f90:340-363), for a singlet given by its
+    closed-shell components r1[i, a], r2[i, j, a, b] (pair-symmetric).  The
+    spin-orbital amplitudes follow from the closed-shell ones as the
+    amplitudes do (convertSpatialToSpinCC): alpha and beta singles equal r1;
+    the four mixed-spin doubles blocks each hold r2 up to index order and sign,
+    the two same-spin blocks r2 - r2(j i a b); each (i<j, a<b) element occurs
+    four times in the antisymmetric tensor."""
+    s1 = 2.0 * float(np.sum(r1 * r1))
+    aa = r2 - r2.transpose(1, 0, 2, 3)
+    s2 = (4.0 * float(np.sum(r2 * r2)) + 2.0 * float(np.sum(aa * aa))) / 4.0
+    return (s1 + 2.0 * s2) / (s1 + s2)
 
-    def __init__(self, o, v, e, nroots, max_space):
+
+class _Layout:
+    """Slots of every molecule's device vectors: basis V and sigmas S (cap
+    each), two buffers of Ritz vectors with their sigmas (the current step's
+    and the last step's, R rows each), and the candidates for new basis
+    vectors (corrections, start vectors, seeds)."""
+
+    def __init__(self, nroots, max_space):
+        self.nwork = nroots + 4
+        self.R = 2 * self.nwork            # a real Ritz vector takes one row, a complex pair two
+        self.max_space = max(max_space, 6 * self.nwork)
+        self.cap = self.max_space + self.R + 8
+        self.V, self.S = 0, self.cap
+        self.XB = [2 * self.cap, 2 * self.cap + 2 * self.R]   # X rows at XB[b], SX rows at XB[b] + R
+        self.C = 2 * self.cap + 4 * self.R
+        self.nC = self.R + self.nwork + 6
+        self.nslots = self.C + self.nC
+
+
+class _Sub:
+    """Host state of one molecule's subspace."""
+
+    def __init__(self, o, v, e, nroots, L):
         self.o, self.v = o, v
         self.n1, self.n2 = o * v, o * o * v * v
-        N = self.n1 + self.n2
+        self.N = self.n1 + self.n2
         eo, ev = e[:o], e[o:]
-        d1 = ev[None, :] - eo[:, None]
-        d2 = (ev[None, None, :, None] + ev[None, None, None, :] - eo[:, None, None, None]
-              - eo[None, :, None, None])
-        self.D = np.concatenate([d1.ravel(), d2.ravel()])
+        self._d1 = ev[None, :] - eo[:, None]
+        self._d2 = (ev[None, None, :, None] + ev[None, None, None, :] - eo[:, None, None, None]
+                    - eo[None, :, None, None])
         self.nroots = nroots
+        # the space's true dimension: singles plus one coordinate per pair (i a) <= (j b)
+        self.dim = self.n1 + self.n1 * (self.n1 + 1) // 2
         # the subspace is expanded along nwork > nroots Ritz pairs: states whose
         # energy drops only once their doubles enter the subspace start high
         # among the Ritz values and would never be targeted otherwise
-        self.nwork = nroots + 4
-        # the space's true dimension: singles plus one coordinate per pair (i a) <= (j b)
-        self.dim = self.n1 + self.n1 * (self.n1 + 1) // 2
-        self.nwork = min(self.nwork, self.dim)
-        self.max_space = min(max(max_space, 6 * self.nwork), self.dim)
-        cap = self.max_space + 2 * self.nwork + 8
-        self.Vm = np.zeros((cap, N))
-        self.Sm = np.zeros((cap, N))
-        self.k = 0
-        self.G = np.zeros((0, 0))   # G[k, l] = <v_k, J v_l>, grown with the subspace
-        self.new = []
-        # unit singles on the lowest orbital-energy differences, more than the
-        # roots: a Davidson subspace keeps the point-group symmetry of its
-        # start, so every symmetry among the low states needs a start vector
-        for idx in np.argsort(d1.ravel(), kind="stable")[:min(self.n1, nroots + 6)]:
-            x = np.zeros(N)
-            x[idx] = 1.0
-            self.new.append(x)
-        # further start directions, in order of the diagonal: unit singles, then
-        # pair-symmetric unit doubles (for spaces the singles do not fill);
-        # ordered on first use
-        self._d1, self._d2 = d1, d2
+        self.nwork = min(L.nwork, self.dim)
+        self.max_space = min(L.max_space, self.dim)
+        self.k = 0            # basis vectors with sigmas: V[0:k], S[0:k]
+        self.pending = 0      # basis vectors V[k:k+pending] awaiting their sigmas
+        self.G = np.zeros((0, 0))   # G[a, b] = <V_a, S_b>
+        self.cur = 0          # Ritz buffer written by the next step
+        self.prev_rows = 0    # rows of the last step's Ritz buffer (1 - cur)
         self.seeds = None
-        # the lowest pair-symmetric doubles start too: states of mainly double
-        # excitation character can lie far below their diagonal, and reached
-        # only through the singles' corrections they surface late
-        for e1, e2 in self._double_seeds(2):
-            x = np.zeros(N)
-            x[e1] = x[e2] = 1.0
-            self.new.append(x / np.linalg.norm(x))
-        self.prev = []     # last step's Ritz vectors (x, J x), kept through a collapse
         self.omega = np.full(nroots, np.nan)
         self.res = np.full(nroots, np.inf)
+        self.root_slot = [None] * nroots
         self.done = False
+        self.working = 0      # Ritz values in the working set at the last step
 
-    def _double_seeds(self, count):
+    def unit(self, e1, e2=-1):
+        x = np.zeros(self.N)
+        x[e1] = 1.0
+        if e2 >= 0:
+            x[e2] = 1.0
+        return x / np.linalg.norm(x)
+
+    def double_seeds(self, count):
         """The ``count`` pair-symmetric unit doubles (i a)(j b) of lowest diagonal."""
         o, v, n1 = self.o, self.v, self.n1
-        if self.n1 < 2 or count <= 0:
+        if n1 < 2 or count <= 0:
             return []
         m, n = np.triu_indices(n1)
         d = self._d2[m // v, n // v, m % v, n % v]
@@ -92,188 +135,375 @@ class _Davidson:
             out.append((n1 + ((i * o + j) * v + a) * v + b, n1 + ((j * o + i) * v + b) * v + a))
         return out
 
-    def _seed_order(self):
-        o, v, n1 = self.o, self.v, self.n1
-        m, n = np.triu_indices(n1)                  # pairs (i a) = m <= n = (j b)
-        i, a, j, b = m // v, m % v, n // v, n % v
-        e1 = ((i * o + j) * v + a) * v + b
-        e2 = ((j * o + i) * v + b) * v + a
-        diag = np.concatenate([self._d1.ravel(), self._d2[i, j, a, b]])
-        first = np.concatenate([np.arange(n1), n1 + e1])
-        second = np.concatenate([np.full(n1, -1), n1 + e2])
-        order = np.argsort(diag, kind="stable")
-        return list(zip(first[order].tolist(), second[order].tolist()))
-
-    def _seed(self):
-        """Next start direction not yet in the subspace (None when exhausted)."""
+    def next_seeds(self, count):
+        """Further start directions in order of the diagonal: unit singles,
+        then pair-symmetric unit doubles (for spaces the singles do not fill)."""
         if self.seeds is None:
-            self.seeds = self._seed_order()
-        while self.seeds:
-            e1, e2 = self.seeds.pop(0)
-            x = np.zeros(self.n1 + self.n2)
-            x[e1] = 1.0
-            if e2 >= 0:
-                x[e2] = 1.0
-            n0 = np.linalg.norm(x)
-            x = self._orth(x)
-            nrm = np.linalg.norm(x)
-            if nrm > 1e-3 * n0:
-                return x / nrm
-        return None
+            o, v, n1 = self.o, self.v, self.n1
+            m, n = np.triu_indices(n1)
+            i, a, j, b = m // v, m % v, n // v, n % v
+            e1 = ((i * o + j) * v + a) * v + b
+            e2 = ((j * o + i) * v + b) * v + a
+            diag = np.concatenate([self._d1.ravel(), self._d2[i, j, a, b]])
+            first = np.concatenate([np.arange(n1), n1 + e1])
+            second = np.concatenate([np.full(n1, -1), n1 + e2])
+            order = np.argsort(diag, kind="stable")
+            self.seeds = list(zip(first[order].tolist(), second[order].tolist()))
+            self.seeds.reverse()   # pop from the end
+        out = []
+        while self.seeds and len(out) < count:
+            out.append(self.seeds.pop())
+        return out
 
-    def _orth(self, x):
-        V = self.Vm[:self.k]
+
+class _Ops:
+    """Batched calls on the device space, with job lists built per molecule."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def dots(self, jobs, mode=0):
+        return self.ctx.lr_dots(np.asarray(jobs, dtype=np.int32).reshape(-1, 3), mode)
+
+    def combine(self, jobs):
+        """jobs: list of (mol, dst, beta, [(slot, coef), ...])."""
+        if not jobs:
+            return
+        head, beta, ts, tc = [], [], [], []
+        for mol, dst, b, terms in jobs:
+            head.append((mol, dst, len(ts), len(terms)))
+            beta.append(b)
+            for sl, c in terms:
+                ts.append(sl)
+                tc.append(c)
+        self.ctx.lr_combine(np.array(head, dtype=np.int32), np.array(beta), np.array(ts, dtype=np.int32),
+                            np.array(tc))
+
+
+def _orthonormalize(ops, L, subs, cands, thresh):
+    """Append the candidates (per molecule: list of (slot, norm before the
+    projections)) to the basis as V[k + pending], each without its components
+    along the basis and the candidates accepted before it (Gram-Schmidt,
+    twice), normalised; a candidate keeps less than ``thresh`` of its norm
+    only if the basis already holds it, and is dropped."""
+    width = max((len(c) for c in cands.values()), default=0)
+    for r in range(width):
+        act = [k for k, c in cands.items()
+               if r < len(c) and subs[k].k + subs[k].pending < min(subs[k].dim, L.cap)]
+        if not act:
+            continue
         for _ in range(2):
-            x = x - (V @ x) @ V
-            for b in self.new:
-                x = x - (b @ x) * b
-        return x
-
-    def absorb(self, sig):
-        """Add the pending trial vectors and their sigma vectors."""
-        k0, m = self.k, len(self.new)
-        for q in range(m):
-            self.Vm[k0 + q] = self.new[q]
-            self.Sm[k0 + q] = sig[q]
-        self.k = k0 + m
-        self.new = []
-        V, S = self.Vm[:self.k], self.Sm[:self.k]
-        G = np.zeros((self.k, self.k))
-        G[:k0, :k0] = self.G
-        G[:, k0:] = V @ S[k0:].T
-        G[k0:, :k0] = V[k0:] @ S[:k0].T
-        self.G = G
-
-    def step(self, eps):
-        V, S = self.Vm[:self.k], self.Sm[:self.k]
-        lam, Y = np.linalg.eig(self.G)   # right eigenvectors of V^T J V
-        # One entry per real Ritz value and per complex pair (J is not
-        # symmetric), lowest real parts first.  Complex pairs are not
-        # excitation energies; they stay in the working set (expanded along
-        # the real and imaginary parts of their corrections) so that a pair
-        # close to the real axis is resolved rather than read as a root.
-        keep = lam.imag >= -1e-12 * np.maximum(np.abs(lam.real), 1.0)
-        lam, Y = lam[keep], Y[:, keep]
-        order = np.argsort(lam.real, kind="stable")
-        lam, Y = lam[order], Y[:, order]
-        isreal = np.abs(lam.imag) <= 1e-10 * np.maximum(np.abs(lam.real), 1.0)
-        k = min(self.nwork, len(lam))
-        corrections = []
-        reals = []          # (value, residual, x, J x) of the real Ritz values in the working set
-        pairs = []          # real and imaginary parts of the complex Ritz pairs' (x, J x)
-        worst_pair = 0.0    # largest residual of a complex pair below the highest reported root
-        for r in range(k):
-            y = Y[:, r] / np.linalg.norm(Y[:, r])
-            if isreal[r]:
-                y = y.real / np.linalg.norm(y.real)
-                x = y @ V
-                q = y @ S - lam[r].real * x
-                res = np.abs(q).max()
-                reals.append((lam[r].real, res, x, y @ S))
-                parts = [q] if res >= eps else []
-            else:
-                x = y @ V
-                sx = y @ S
-                q = sx - lam[r] * x
-                res = np.abs(q).max()
-                if len(reals) < self.nroots:
-                    worst_pair = max(worst_pair, res)
-                pairs.extend([(x.real, sx.real), (x.imag, sx.imag)])
-                parts = [q.real, q.imag] if res >= eps else []
-            den = lam[r].real - self.D
-            den = np.where(np.abs(den) < 1e-2, np.copysign(1e-2, den), den)
-            corrections.extend(c / den for c in parts)
-        for r, (val, res, _, _) in enumerate(reals[:self.nroots]):
-            self.omega[r] = val
-            self.res[r] = res
-        ritz = [(x, sx) for _, _, x, sx in reals] + pairs
-        nr = min(self.nroots, self.dim)
-        if len(reals) >= nr and all(res < eps for _, res, _, _ in reals[:nr]) and worst_pair < 1e-3:
-            self.done = True
-            return
-        if self.k >= self.dim:   # the subspace spans the space: Ritz values are exact
-            self.done = True
-            return
-        if self.k + len(corrections) > self.max_space:
-            # collapse onto the orthonormalised Ritz vectors of this step and
-            # of the last one (their sigmas follow linearly): the last step's
-            # vectors keep the direction of approach, which a restart onto
-            # the current ones alone would lose
-            basis = ritz + self.prev
-            self.k = 0
-            for x, sx in basis:
-                n0 = np.linalg.norm(x)
-                for _ in range(2):
-                    c = self.Vm[:self.k] @ x
-                    x = x - c @ self.Vm[:self.k]
-                    sx = sx - c @ self.Sm[:self.k]
-                nrm = np.linalg.norm(x)
-                if nrm > 1e-6 * n0:
-                    self.Vm[self.k] = x / nrm
-                    self.Sm[self.k] = sx / nrm
-                    self.k += 1
-            self.G = self.Vm[:self.k] @ self.Sm[:self.k].T
-        self.prev = ritz
-        for c in corrections:
-            n0 = np.linalg.norm(c)
-            c = self._orth(c)
-            nrm = np.linalg.norm(c)
-            # keep a correction unless the subspace already contains it
-            if nrm > 1e-3 * n0 and self.k + len(self.new) < self.dim:
-                self.new.append(c / nrm)
-        # fewer real Ritz values than working roots: widen the subspace along
-        # the next diagonal directions
-        while k + len(self.new) < self.nwork and self.k + len(self.new) < self.dim:
-            x = self._seed()
-            if x is None:
-                break
-            self.new.append(x)
-        if not self.new:
-            self.done = True
+            jobs, spans = [], []
+            for k in act:
+                s, sl = subs[k], cands[k][r][0]
+                nb = s.k + s.pending
+                spans.append((len(jobs), nb))
+                jobs.extend((k, sl, L.V + j) for j in range(nb))
+            P = ops.dots(jobs)
+            cj = []
+            for k, (at, nb) in zip(act, spans):
+                if nb:
+                    cj.append((k, cands[k][r][0], 1.0, [(L.V + j, -P[at + j]) for j in range(nb)]))
+            ops.combine(cj)
+        nrm = np.sqrt(np.maximum(ops.dots([(k, cands[k][r][0], cands[k][r][0]) for k in act]), 0.0))
+        cj = []
+        for k, nr in zip(act, nrm):
+            s = subs[k]
+            if nr > thresh * cands[k][r][1]:
+                cj.append((k, L.V + s.k + s.pending, 0.0, [(cands[k][r][0], 1.0 / nr)]))
+                s.pending += 1
+        ops.combine(cj)
 
 
-def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_space=48, log=None):
+def _collapse(ops, L, subs, which):
+    """Restart the subspaces of molecules ``which`` on their current and last
+    Ritz vectors (orthonormalised, sigmas following linearly); the last
+    step's vectors keep the direction of approach, which a restart onto the
+    current ones alone would lose."""
+    rows = {}
+    for k in which:
+        s = subs[k]
+        cur = [(L.XB[s.cur] + q, L.XB[s.cur] + L.R + q) for q in range(s.rows)]
+        old = [(L.XB[1 - s.cur] + q, L.XB[1 - s.cur] + L.R + q) for q in range(s.prev_rows)]
+        rows[k] = cur + old
+        s.k = 0
+    width = max((len(r) for r in rows.values()), default=0)
+    for r in range(width):
+        act = [k for k in which if r < len(rows[k])]
+        if not act:
+            continue
+        jobs, spans = [], []
+        for k in act:
+            x, _ = rows[k][r]
+            nb = subs[k].k
+            spans.append((len(jobs), nb))
+            jobs.append((k, x, x))
+            jobs.extend((k, x, L.V + j) for j in range(nb))
+        P = ops.dots(jobs)
+        n0 = {k: np.sqrt(max(P[at], 0.0)) for k, (at, _) in zip(act, spans)}
+        cj = []
+        for k, (at, nb) in zip(act, spans):
+            x, sx = rows[k][r]
+            a = subs[k].k
+            cj.append((k, L.V + a, 0.0, [(x, 1.0)] + [(L.V + j, -P[at + 1 + j]) for j in range(nb)]))
+            cj.append((k, L.S + a, 0.0, [(sx, 1.0)] + [(L.S + j, -P[at + 1 + j]) for j in range(nb)]))
+        ops.combine(cj)
+        jobs, spans = [], []
+        for k in act:
+            a = subs[k].k
+            spans.append((len(jobs), a))
+            jobs.extend((k, L.V + a, L.V + j) for j in range(a))
+        P = ops.dots(jobs)
+        cj = []
+        for k, (at, nb) in zip(act, spans):
+            a = subs[k].k
+            if nb:
+                cj.append((k, L.V + a, 1.0, [(L.V + j, -P[at + j]) for j in range(nb)]))
+                cj.append((k, L.S + a, 1.0, [(L.S + j, -P[at + j]) for j in range(nb)]))
+        ops.combine(cj)
+        nrm = np.sqrt(np.maximum(ops.dots([(k, L.V + subs[k].k, L.V + subs[k].k) for k in act]), 0.0))
+        cj = []
+        for k, nr in zip(act, nrm):
+            s = subs[k]
+            if nr > 1e-6 * n0[k]:
+                cj.append((k, L.V + s.k, 1.0 / nr, []))
+                cj.append((k, L.S + s.k, 1.0 / nr, []))
+                s.k += 1
+        ops.combine(cj)
+    jobs, spans = [], []
+    for k in which:
+        kk = subs[k].k
+        spans.append((len(jobs), kk))
+        jobs.extend((k, L.V + a, L.S + b) for a in range(kk) for b in range(kk))
+    g = ops.dots(jobs)
+    for k, (at, kk) in zip(which, spans):
+        subs[k].G = g[at:at + kk * kk].reshape(kk, kk)
+
+
+def _absorb(ops, L, subs, act):
+    """Extend G by the rows and columns of the vectors whose sigmas just arrived."""
+    jobs, spans = [], []
+    for k in act:
+        s = subs[k]
+        k0, k1 = s.k, s.k + s.pending
+        spans.append((len(jobs), k0, k1))
+        jobs.extend((k, L.V + a, L.S + b) for a in range(k1) for b in range(k0, k1))
+        jobs.extend((k, L.V + a, L.S + b) for a in range(k0, k1) for b in range(k0))
+    g = ops.dots(jobs)
+    for k, (at, k0, k1) in zip(act, spans):
+        s = subs[k]
+        G = np.zeros((k1, k1))
+        G[:k0, :k0] = s.G
+        p = k1 - k0
+        G[:, k0:] = g[at:at + k1 * p].reshape(k1, p)
+        G[k0:, :k0] = g[at + k1 * p:at + k1 * p + p * k0].reshape(p, k0)
+        s.G = G
+        s.k, s.pending = k1, 0
+
+
+def _ritz(s):
+    """Rayleigh-Ritz of one subspace: the working Ritz values, lowest real
+    parts first (one entry per real value and per complex pair), and the
+    subspace coefficients of their vectors (a real one: one row; a complex
+    pair: its real and imaginary parts)."""
+    lam, Y = np.linalg.eig(s.G)   # right eigenvectors of V^T J V
+    keep = lam.imag >= -1e-12 * np.maximum(np.abs(lam.real), 1.0)
+    lam, Y = lam[keep], Y[:, keep]
+    order = np.argsort(lam.real, kind="stable")
+    lam, Y = lam[order], Y[:, order]
+    isreal = np.abs(lam.imag) <= 1e-10 * np.maximum(np.abs(lam.real), 1.0)
+    rows, entries = [], []
+    for r in range(min(s.nwork, len(lam))):
+        y = Y[:, r] / np.linalg.norm(Y[:, r])
+        entries.append((lam[r], bool(isreal[r]), len(rows)))
+        if isreal[r]:
+            rows.append(y.real / np.linalg.norm(y.real))
+        else:
+            rows.extend([y.real, y.imag])
+    return entries, rows
+
+
+def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_space=48, log=None,
+                        gammas=None):
     """Lowest ``nroots`` singlet LR-CCSD excitation energies (eV) of every
     loaded molecule, after a converged solve of ``ctx``'s batch.
 
     dims: (n_occ, n_virt) of the loaded molecules in load order.
+    gammas: the molecules' site matrices gamma; when given, the Davidson starts
+    from the lowest CIS singlets (the reference's default 'cis' guess),
+    otherwise from unit singles (its 'unt' guess).
     Returns (omega [nmol, nroots], residual max-norms [nmol, nroots],
-    iterations [nmol], converged [nmol]).  ``log(iteration, active)`` is called
-    after every Davidson iteration when given."""
+    iterations [nmol], converged [nmol], average excitation levels
+    [nmol, nroots]).  ``log(iteration, active)`` is called after every
+    Davidson iteration when given."""
+    L = _Layout(nroots, max_space)
+    ctx.lr_space(L.nslots)
+    ops = _Ops(ctx)
     subs = []
+    cands = {}
     for k, (o, v) in enumerate(dims):
-        _, Fmo, _ = ctx.batch_orbitals(k, o + v)
-        subs.append(_Davidson(o, v, np.diag(Fmo).copy(), nroots, max_space))
-    n1 = [s.n1 for s in subs]
-    n2 = [s.n2 for s in subs]
-    o1 = np.concatenate([[0], np.cumsum(n1)])
-    o2 = np.concatenate([[0], np.cumsum(n2)])
+        C, Fmo, _ = ctx.batch_orbitals(k, o + v)
+        e = np.diag(Fmo).copy()
+        s = _Sub(o, v, e, nroots, L)
+        subs.append(s)
+        # more start vectors than roots: a Davidson subspace keeps the
+        # point-group symmetry of its start, so every symmetry among the low
+        # states needs one.  Either the lowest CIS states or unit singles on
+        # the lowest orbital-energy differences; then the lowest
+        # pair-symmetric doubles: states of mainly double excitation
+        # character can lie far below their diagonal, and reached only
+        # through the singles' corrections they surface late.
+        count = min(s.n1, nroots + 6)
+        starts = []
+        if gammas is not None:
+            U = cis_singlets(C, np.asarray(gammas[k], dtype=np.float64), e, o)[1]
+            for q in range(count):
+                x = np.zeros(s.N)
+                x[:s.n1] = U[:, q] / np.linalg.norm(U[:, q])
+                starts.append(x)
+        else:
+            for idx in np.argsort(s._d1.ravel(), kind="stable")[:count]:
+                starts.append(s.unit(int(idx)))
+        starts.extend(s.unit(e1, e2) for e1, e2 in s.double_seeds(2))
+        cands[k] = []
+        for q, x in enumerate(starts):
+            ctx.lr_put(k, L.C + q, x)
+            cands[k].append((L.C + q, 1.0))
+    _orthonormalize(ops, L, subs, cands, 1e-3)
     its = np.zeros(len(subs), dtype=int)
     for it in range(1, maxiter + 1):
-        width = max((len(s.new) for s in subs if not s.done), default=0)
-        if width == 0:
-            break
-        sig = [[] for _ in subs]
-        for c in range(width):
-            r1 = np.zeros(o1[-1])
-            r2 = np.zeros(o2[-1])
-            for k, s in enumerate(subs):
-                if not s.done and c < len(s.new):
-                    r1[o1[k]:o1[k + 1]] = s.new[c][:s.n1]
-                    r2[o2[k]:o2[k + 1]] = s.new[c][s.n1:]
-            s1, s2 = ctx.batch_lr_sigma(r1, r2, h)
-            for k, s in enumerate(subs):
-                if not s.done and c < len(s.new):
-                    sig[k].append(np.concatenate([s1[o1[k]:o1[k + 1]], s2[o2[k]:o2[k + 1]]]))
+        act = [k for k, s in enumerate(subs) if not s.done and s.pending > 0]
         for k, s in enumerate(subs):
-            if s.done:
-                continue
-            s.absorb(sig[k])
-            s.step(eps)
+            if not s.done and s.pending == 0:
+                s.done = True
+        if not act:
+            break
+        # sigmas of the pending vectors, one batched product per column
+        for c in range(max(subs[k].pending for k in act)):
+            ins = np.full(len(subs), -1, dtype=np.int32)
+            outs = np.zeros(len(subs), dtype=np.int32)
+            for k in act:
+                s = subs[k]
+                if c < s.pending:
+                    ins[k] = L.V + s.k + c
+                    outs[k] = L.S + s.k + c
+            ctx.lr_sigma_slots(ins, outs, h)
+        _absorb(ops, L, subs, act)
+        # Ritz vectors and their sigmas into the current buffer
+        ritz = {}
+        cj = []
+        for k in act:
+            s = subs[k]
+            entries, rows = _ritz(s)
+            ritz[k] = entries
+            s.rows = len(rows)
+            xb = L.XB[s.cur]
+            for q, y in enumerate(rows):
+                cj.append((k, xb + q, 0.0, [(L.V + j, y[j]) for j in range(s.k)]))
+                cj.append((k, xb + L.R + q, 0.0, [(L.S + j, y[j]) for j in range(s.k)]))
+        ops.combine(cj)
+        # residuals J x - lam x into the candidate slots, and their max-norms
+        cj, mj, where = [], [], {}
+        for k in act:
+            s, xb = subs[k], L.XB[subs[k].cur]
+            q = 0
+            where[k] = []
+            for lam, real, c in ritz[k]:
+                X, SX = xb + c, xb + L.R + c
+                if real:
+                    cj.append((k, L.C + q, 0.0, [(SX, 1.0), (X, -lam.real)]))
+                    mj.append((k, L.C + q, -1))
+                    where[k].append((q, 1))
+                    q += 1
+                else:
+                    a, b = lam.real, lam.imag
+                    cj.append((k, L.C + q, 0.0, [(SX, 1.0), (X, -a), (X + 1, b)]))
+                    cj.append((k, L.C + q + 1, 0.0, [(SX + 1, 1.0), (X + 1, -a), (X, -b)]))
+                    mj.append((k, L.C + q, L.C + q + 1))
+                    where[k].append((q, 2))
+                    q += 2
+        ops.combine(cj)
+        resn = ops.dots(mj, mode=1)
+        # convergence, reported roots, corrections to keep
+        at = 0
+        corr = {}
+        pj, shifts = [], []
+        collapse = []
+        for k in act:
+            s = subs[k]
             its[k] = it
+            reals, worst_pair, keep = [], 0.0, []
+            for (lam, real, c), (q, width) in zip(ritz[k], where[k]):
+                res = resn[at]
+                at += 1
+                if real:
+                    reals.append((lam.real, res, c))
+                elif len(reals) < s.nroots:
+                    worst_pair = max(worst_pair, res)
+                if res >= eps:
+                    for w in range(width):
+                        keep.append(L.C + q + w)
+                        pj.append((k, L.C + q + w))
+                        shifts.append(lam.real)
+            s.working = len(ritz[k])
+            for r, (val, res, c) in enumerate(reals[:s.nroots]):
+                s.omega[r] = val
+                s.res[r] = res
+                s.root_slot[r] = L.XB[s.cur] + c
+            nr = min(s.nroots, s.dim)
+            if len(reals) >= nr and all(res < eps for _, res, _ in reals[:nr]) and worst_pair < 1e-3:
+                s.done = True
+                continue
+            if s.k >= s.dim:   # the subspace spans the space: Ritz values are exact
+                s.done = True
+                continue
+            corr[k] = keep
+            if s.k + len(keep) > s.max_space:
+                collapse.append(k)
+        ops_pre = [(k, sl) for k, sl in pj if k in corr]
+        ctx.lr_precondition(np.array(ops_pre, dtype=np.int32).reshape(-1, 2),
+                            np.array([sh for (k, _), sh in zip(pj, shifts) if k in corr]), 1e-2)
+        live = list(corr)
+        n0 = ops.dots([(k, sl, sl) for k in live for sl in corr[k]])
+        cands, at = {}, 0
+        for k in live:
+            cands[k] = [(sl, np.sqrt(max(n0[at + q], 0.0))) for q, sl in enumerate(corr[k])]
+            at += len(corr[k])
+        if collapse:
+            _collapse(ops, L, subs, collapse)
+        for k in act:
+            s = subs[k]
+            s.prev_rows = s.rows
+            s.cur = 1 - s.cur
+        _orthonormalize(ops, L, subs, cands, 1e-3)
+        # fewer Ritz values than working roots: widen the subspace along the
+        # next diagonal directions
+        for _ in range(8):
+            seeds = {}
+            for k in live:
+                s = subs[k]
+                need = s.nwork - s.working - s.pending
+                room = min(s.dim, L.cap) - s.k - s.pending
+                if need > 0 and room > 0:
+                    got = s.next_seeds(min(need, room))
+                    if got:
+                        seeds[k] = []
+                        for q, (e1, e2) in enumerate(got):
+                            ctx.lr_put(k, L.C + q, s.unit(e1, e2))
+                            seeds[k].append((L.C + q, 1.0))
+            if not seeds:
+                break
+            _orthonormalize(ops, L, subs, seeds, 1e-3)
         if log is not None:
             log(it, sum(not s.done for s in subs))
     omega = np.array([s.omega for s in subs])
     res = np.array([s.res for s in subs])
     conv = np.array([bool(np.all(s.res[:min(nroots, s.dim)] < eps)) or s.k >= s.dim for s in subs])
-    return omega, res, its, conv
+    ael = np.full((len(subs), nroots), np.nan)
+    for k, s in enumerate(subs):
+        for r, sl in enumerate(s.root_slot):
+            if sl is not None:
+                x = ctx.lr_get(k, sl, s.N)
+                ael[k, r] = average_excitation_level(x[:s.n1].reshape(s.o, s.v),
+                                                     x[s.n1:].reshape(s.o, s.o, s.v, s.v))
+    return omega, res, its, conv, ael

@@ -200,6 +200,33 @@ int hcc_batch_triples(hcc_ctx* ctx, double* energies, double* e_t);
  * O(h^2): Omega is a polynomial in t); the amplitudes are restored. */
 int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r2, double* s1, double* s2);
 
+/* Device-resident vector space of the LR-CCSD eigensolver, so that the
+ * subspace iteration around the Jacobian products runs in device memory.
+ * The reference keeps its trial vectors r1/r2 and their DIIS history in host
+ * arrays (mod_lrccsd.f90:176-326, controlMemoryLR); here each loaded molecule
+ * gets `nslots` vectors of o v + o^2 v^2 doubles (singles [i][a], then the
+ * pair-symmetric doubles [i][j][a][b]), zeroed.  A second call replaces the
+ * space.  Needs solved amplitudes (as hcc_batch_lr_sigma). */
+int hcc_lr_space(hcc_ctx* ctx, int nslots);
+/* Copy one vector (o v + o^2 v^2 doubles) into / out of slot `slot` of molecule `mol`. */
+int hcc_lr_put(hcc_ctx* ctx, int mol, int slot, const double* x);
+int hcc_lr_get(hcc_ctx* ctx, int mol, int slot, double* x);
+/* njobs jobs of 3 ints {mol, slot a, slot b}: out[j] = <a, b> (mode 0), or
+ * max_e sqrt(a_e^2 + b_e^2) (mode 1; b = -1 gives max_e |a_e|). */
+int hcc_lr_dots(hcc_ctx* ctx, int njobs, const int* jobs, int mode, double* out);
+/* njobs jobs of 4 ints {mol, dst slot, first term, term count}:
+ * dst = beta[j] dst + sum_t term_coef[t] * slot term_slot[t] over the job's
+ * terms (same molecule; no term may be dst; beta = 0 ignores dst's contents). */
+int hcc_lr_combine(hcc_ctx* ctx, int njobs, const int* jobs, const double* beta, const int* term_slot,
+                   const double* term_coef);
+/* njobs jobs of 2 ints {mol, slot}: x_e /= (shift[j] - D_e) with the
+ * orbital-energy difference D_e (e_a - e_i for singles, e_a + e_b - e_i - e_j
+ * for doubles) and |shift - D_e| raised to `floor` where smaller. */
+int hcc_lr_precondition(hcc_ctx* ctx, int njobs, const int* jobs, const double* shift, double floor);
+/* sigma = J r as hcc_batch_lr_sigma, from slot in_slot[k] into slot
+ * out_slot[k] of every molecule k (in_slot[k] < 0: molecule skipped). */
+int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* out_slot);
+
 /* MO coefficients C (n x n column-major, = hV), MO Fock matrix and reference
  * energy of loaded molecule `mol` (either array may be null): the values the
  * solve starts from (getSCFResult(vectors=hV), src/methods/mod_scf.f90:204-277,

@@ -56,12 +56,29 @@ def jacobian(C, F, G, o, t1, t2, h=1e-5):
     return J
 
 
-def excitation_energies(C, F, G, o, t1, t2, h=1e-5):
+def excitation_energies(C, F, G, o, t1, t2, h=1e-5, vectors=False):
     """All eigenvalues of the singlet Jacobian, real ones sorted, and the
-    count of complex ones."""
-    lam = np.linalg.eigvals(jacobian(C, F, G, o, t1, t2, h))
+    count of complex ones; with ``vectors``, also the right eigenvectors of
+    the real ones as (r1[i, a], r2[i, j, a, b]) with r2 pair-symmetric (a
+    pair coordinate moves both t2[i,j,a,b] and t2[j,i,b,a])."""
+    J = jacobian(C, F, G, o, t1, t2, h)
+    if not vectors:
+        lam = np.linalg.eigvals(J)
+        real = np.abs(lam.imag) <= 1e-8 * np.abs(lam.real)
+        return np.sort(lam.real[real]), int((~real).sum())
+    lam, X = np.linalg.eig(J)
     real = np.abs(lam.imag) <= 1e-8 * np.abs(lam.real)
-    return np.sort(lam.real[real]), int((~real).sum())
+    order = np.argsort(lam.real[real], kind="stable")
+    v = C.shape[0] - o
+    n1 = o * v
+    P = pair_list(o, v)
+    out = []
+    for x in X[:, real][:, order].real.T:
+        r2 = np.zeros((o, o, v, v))
+        for k, (m, n) in enumerate(P):
+            r2[m // v, n // v, m % v, n % v] = r2[n // v, m // v, n % v, m % v] = x[n1 + k]
+        out.append((x[:n1].reshape(o, v), r2))
+    return lam.real[real][order], int((~real).sum()), out
 
 
 def sigma(C, F, G, o, t1, t2, r1, r2, h=1e-4):
@@ -92,3 +109,36 @@ def singlets_two_electrons(C, core, G):
             H[x, y] = nx * ny * (prod(p, q, r, s) + prod(p, q, s, r) + prod(q, p, r, s) + prod(q, p, s, r))
     w = np.linalg.eigvalsh(H)
     return w[1:] - w[0]
+
+
+def cis_spin_orbital(C, F, G, o):
+    """Spin-orbital CIS matrix A[(i a), (j b)] = (e_a - e_i) d_ij d_ab + <aj||ib>
+    over the 2o occupied and 2v virtual spin orbitals (alpha, beta alternating,
+    as the reference numbers them), with <aj||ib> = R(a, i, j, b) of the
+    reference's spin_hf_int (mod_lrccsd.f90:1449-1478).  Its eigenvalues are the
+    singlet and the (threefold) triplet CIS energies together."""
+    from oracle.triples import spin_integrals
+    n = C.shape[0]
+    R = spin_integrals(C, G)
+    e = np.repeat(np.diag(F), 2)
+    no, ns = 2 * o, 2 * n
+    occ, vir = np.arange(no), np.arange(no, ns)
+    A = R[np.ix_(vir, occ, occ, vir)].transpose(1, 0, 2, 3).reshape(no * (ns - no), no * (ns - no))
+    A[np.diag_indices(A.shape[0])] += (e[vir][None, :] - e[occ][:, None]).ravel()
+    return A
+
+
+def average_excitation_level(r1, r2):
+    """(S1 + 2 S2) / (S1 + S2) with S1 = sum r1^2, S2 = sum over i<j, a<b of r2^2
+    (mod_lrccsd.f90:340-363), on the spin-orbital vector that the closed-shell
+    (r1, r2) converts to (convertSpatialToSpinCC, mod_cc.f90:2601-2678)."""
+    from oracle.triples import spin_amplitudes
+    s1v, s2v = spin_amplitudes(r1, r2)
+    no, nv = s1v.shape
+    s1 = float(np.sum(s1v ** 2))
+    s2 = 0.0
+    for i in range(no - 1):
+        for j in range(i + 1, no):
+            for a in range(nv - 1):
+                s2 += float(np.sum(s2v[i, j, a, a + 1:] ** 2))
+    return (s1 + 2 * s2) / (s1 + s2)

@@ -6,9 +6,14 @@ against the oracle (oracle/lrccsd.py, pinned by the two-electron full CI):
   molecule of a batch, and the amplitudes are left as they were;
 * the three lowest singlet excitation energies of a batch (polyenes with
   their doubly excited lowest state, honeycomb fragments) equal the lowest
-  real eigenvalues of the oracle's dense Jacobian within 2e-6 eV;
+  real eigenvalues of the oracle's dense Jacobian within 2e-6 eV, from unit
+  singles and from the CIS start vectors, and their average excitation
+  levels equal those of the dense eigenvectors;
 * for two electrons they equal the singlet full-CI excitation energies;
-* a molecule's roots in the batch equal its roots alone.
+* a molecule's roots in the batch equal its roots alone;
+* the device vector space's operations (dot products, max-norms, linear
+  combinations, the orbital-energy preconditioner, sigma between slots)
+  equal their numpy counterparts, and invalid jobs are rejected.
 """
 import numpy as np
 import pytest
@@ -66,18 +71,24 @@ def test_sigma_matches_oracle():
     ctx.close()
 
 
-def test_lowest_roots_match_dense_jacobian():
+@pytest.mark.parametrize("guess", ["unt", "cis"])
+def test_lowest_roots_match_dense_jacobian(guess):
     mols = [M.octatetraene(), M.polyene(6), M.random_fragment(10, seed=4), M.random_fragment(12, seed=2)]
     preps = [prepare(m) for m in mols]
     ctx = _solved_ctx(preps)
-    om, res, its, conv = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=3)
+    gammas = [p["G"] for p in preps] if guess == "cis" else None
+    om, res, its, conv, ael = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=3,
+                                                         gammas=gammas)
     ctx.close()
-    for m, p, w, c, it in zip(mols, preps, om, conv, its):
+    for m, p, w, c, it, lv in zip(mols, preps, om, conv, its, ael):
         r = ccsd.solve(p["C"], p["F"], p["G"], m.nocc, p["eref"], eps=1e-11, diis_scale=True)
-        lam, _ = olr.excitation_energies(p["C"], p["F"], p["G"], m.nocc, r["t1"], r["t2"])
-        print(m.name, "gpu", w, "oracle", lam[:3], "iterations", it, "diff", np.abs(w - lam[:3]).max())
+        lam, _, vec = olr.excitation_energies(p["C"], p["F"], p["G"], m.nocc, r["t1"], r["t2"], vectors=True)
+        ref_ael = [olr.average_excitation_level(*x) for x in vec[:3]]
+        print(m.name, "gpu", w, "oracle", lam[:3], "iterations", it, "diff", np.abs(w - lam[:3]).max(),
+              "AEL", lv, ref_ael)
         assert c
         assert np.abs(w - lam[:3]).max() < 2e-6
+        assert np.abs(lv - ref_ael).max() < 1e-4
 
 
 def test_two_electrons_equal_fci():
@@ -90,7 +101,7 @@ def test_two_electrons_equal_fci():
         preps.append(dict(C=r["C"], F=F, G=G, nocc=1, eref=eref, n=m.n))
         fcis.append(olr.singlets_two_electrons(r["C"], core, G))
     ctx = _solved_ctx(preps)
-    om, res, its, conv = lrccsd.excitation_energies(ctx, [(1, m.n - 1) for m in mols], nroots=2)
+    om, res, its, conv, _ = lrccsd.excitation_energies(ctx, [(1, m.n - 1) for m in mols], nroots=2)
     ctx.close()
     for m, w, f, c in zip(mols, om, fcis, conv):
         print(m.name, "gpu", w, "fci", f[:2])
@@ -102,10 +113,85 @@ def test_batch_equals_alone():
     mols = [M.polyene(6), M.random_fragment(10, seed=4)]
     preps = [prepare(m) for m in mols]
     ctx = _solved_ctx(preps)
-    om, _, _, _ = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=2)
+    om, _, _, _, _ = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=2)
     ctx.close()
     for k, (m, p) in enumerate(zip(mols, preps)):
         one = _solved_ctx([p])
-        w1, _, _, _ = lrccsd.excitation_energies(one, [(m.nocc, m.n - m.nocc)], nroots=2)
+        w1, _, _, _, _ = lrccsd.excitation_energies(one, [(m.nocc, m.n - m.nocc)], nroots=2)
         one.close()
         assert np.abs(w1[0] - om[k]).max() < 1e-6
+
+
+def test_vector_space_operations():
+    mols = [M.octatetraene(), M.random_fragment(12, seed=2), M.batch_member(11)]
+    preps = [prepare(m) for m in mols]
+    ctx = _solved_ctx(preps)
+    ctx.lr_space(6)
+    rng = np.random.default_rng(7)
+    X = []
+    for k, m in enumerate(mols):
+        o, v = m.nocc, m.n - m.nocc
+        rows = []
+        for sl in range(6):
+            r1 = rng.standard_normal(o * v)
+            r2 = _pair_symmetric(rng, o, v).ravel()
+            x = np.concatenate([r1, r2])
+            ctx.lr_put(k, sl, x)
+            rows.append(x)
+        X.append(np.array(rows))
+    for k in range(len(mols)):
+        assert np.array_equal(ctx.lr_get(k, 3, X[k].shape[1]), X[k][3])
+    jobs = [(k, a, b) for k in range(len(mols)) for a in range(3) for b in range(2, 5)]
+    got = ctx.lr_dots(jobs)
+    ref = np.array([X[k][a] @ X[k][b] for k, a, b in jobs])
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    got = ctx.lr_dots([(0, 1, 2), (1, 4, -1), (2, 0, 5)], mode=1)
+    ref = [np.sqrt(X[0][1] ** 2 + X[0][2] ** 2).max(), np.abs(X[1][4]).max(), np.sqrt(X[2][0] ** 2 + X[2][5] ** 2).max()]
+    assert np.allclose(got, ref, rtol=1e-15, atol=0)
+    # dst = beta dst + sum coef slot
+    ctx.lr_combine(np.array([[0, 5, 0, 2], [1, 0, 2, 3], [2, 4, 5, 0]]), np.array([0.5, 0.0, -2.0]),
+                   np.array([1, 2, 1, 2, 3]), np.array([0.25, -1.5, 2.0, 3.0, -0.75]))
+    want = {(0, 5): 0.5 * X[0][5] + 0.25 * X[0][1] - 1.5 * X[0][2],
+            (1, 0): 2.0 * X[1][1] + 3.0 * X[1][2] - 0.75 * X[1][3],
+            (2, 4): -2.0 * X[2][4]}
+    for (k, sl), w in want.items():
+        g = ctx.lr_get(k, sl, X[k].shape[1])
+        assert np.abs(g - w).max() < 1e-13 * np.abs(w).max()
+        X[k][sl] = g
+    # orbital-energy preconditioner
+    ctx.lr_precondition(np.array([[0, 1], [2, 3]]), np.array([3.9, 4.4]), 1e-2)
+    for (k, sl), w in [((0, 1), 3.9), ((2, 3), 4.4)]:
+        m = mols[k]
+        o = m.nocc
+        _, F, _ = ctx.batch_orbitals(k, m.n)
+        e = np.diag(F)
+        D = np.concatenate([(e[o:][None, :] - e[:o][:, None]).ravel(),
+                            (e[o:][None, None, :, None] + e[o:][None, None, None, :] - e[:o][:, None, None, None]
+                             - e[:o][None, :, None, None]).ravel()])
+        den = w - D
+        den = np.where(np.abs(den) < 1e-2, np.copysign(1e-2, den), den)
+        g = ctx.lr_get(k, sl, X[k].shape[1])
+        assert np.abs(g - X[k][sl] / den).max() < 1e-14 * np.abs(g).max()
+        X[k][sl] = g
+    # sigma between slots = sigma of the host vectors, molecule 1 skipped
+    ctx.lr_sigma_slots(np.array([2, -1, 4]), np.array([0, 0, 1]), 1e-5)
+    r1 = np.concatenate([X[k][sl][:mols[k].nocc * (mols[k].n - mols[k].nocc)] for k, sl in enumerate([2, 2, 4])])
+    r2 = np.concatenate([X[k][sl][mols[k].nocc * (mols[k].n - mols[k].nocc):] for k, sl in enumerate([2, 2, 4])])
+    s1, s2 = ctx.batch_lr_sigma(r1, r2, 1e-5)
+    a1 = a2 = 0
+    for k, m in enumerate(mols):
+        o, v = m.nocc, m.n - m.nocc
+        g = ctx.lr_get(k, [0, 0, 1][k], X[k].shape[1])
+        if k == 1:
+            assert np.array_equal(g, X[k][0])        # skipped: slot untouched
+        else:
+            assert np.array_equal(g, np.concatenate([s1[a1:a1 + o * v], s2[a2:a2 + o * o * v * v]]))
+        a1, a2 = a1 + o * v, a2 + o * o * v * v
+    # invalid jobs are rejected before any work
+    with pytest.raises(capi.HccError):
+        ctx.lr_combine(np.array([[0, 1, 0, 1]]), np.array([0.0]), np.array([1]), np.array([1.0]))   # dst is a term
+    with pytest.raises(capi.HccError):
+        ctx.lr_dots([(0, 6, 0)])
+    with pytest.raises(capi.HccError):
+        ctx.lr_sigma_slots(np.array([0, -1, 0]), np.array([0, 0, 1]), 1e-5)   # in == out
+    ctx.close()
