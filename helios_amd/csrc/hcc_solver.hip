@@ -373,8 +373,11 @@ static GemmTable make_table(Batch& b, const std::function<GemmProb(int)>& build,
         ks = it->second[1];
       }
     }
-    if (dual) ts = force_tile ? force_tile : choose_tile_dual(p);
-    else if (ts != 32 && ts != 64 && ts != 128) ts = choose_tile(p);
+    if (dual) {
+      if (ts != 32 && ts != 64 && ts != 256) ts = choose_tile_dual(p);
+    } else if (ts != 32 && ts != 64 && ts != 128) {
+      ts = choose_tile(p);
+    }
     if (!allow_split) force_split(p, 1);
     else if (ks >= 1) force_split(p, ks);
     else choose_split(p, ts);
@@ -793,24 +796,32 @@ static void build_tables(Batch& b) {
   });
 }
 
-// correction-form twins of the iteration GEMMs (64x64 tiles)
+// Correction-form twin of iteration GEMM `op` for molecule k: the FP64
+// problem plus its correction operands and BF16 operand copies.
+static GemmProb delta_prob(Batch& b, int op, int k) {
+  GemmProb p = b.ops[op](b.hm[k]);
+  if (p.M <= 0 || p.N <= 0) return p;   // handled outside the GEMM tables
+  p.dA = delta_of(b, k, p.A.p);
+  p.dB = delta_of(b, k, p.B.p);
+  p.Cd = delta_of(b, k, p.C.p);
+  if (p.Cd == nullptr) throw std::string("correction table: constant output operand");
+  if (p.dA == nullptr && p.dB == nullptr) throw std::string("correction table: both operands constant");
+  p.hA = shadow_of(b, p.A.p);
+  p.hB = shadow_of(b, p.B.p);
+  if (p.hA == nullptr || p.hB == nullptr) throw std::string("correction table: operand without a BF16 copy");
+  return p;
+}
+
+// Tuning keys of the correction-form tables: kDualTuneOp + op.
+constexpr int kDualTuneOp = 100;
+
+// correction-form twins of the iteration GEMMs (BF16 tiles chosen per
+// problem, or FP64 64x64 tiles in exact mode)
 static void build_delta_tables(Batch& b) {
   b.dit.clear();
-  for (const auto& f : b.ops) {
-    b.dit.push_back(make_table(b, [&](int k) {
-      GemmProb p = f(b.hm[k]);
-      if (p.M <= 0 || p.N <= 0) return p;   // handled outside the GEMM tables
-      p.dA = delta_of(b, k, p.A.p);
-      p.dB = delta_of(b, k, p.B.p);
-      p.Cd = delta_of(b, k, p.C.p);
-      if (p.Cd == nullptr) throw std::string("correction table: constant output operand");
-      if (p.dA == nullptr && p.dB == nullptr) throw std::string("correction table: both operands constant");
-      p.hA = shadow_of(b, p.A.p);
-      p.hB = shadow_of(b, p.B.p);
-      if (p.hA == nullptr || p.hB == nullptr) throw std::string("correction table: operand without a BF16 copy");
-      return p;
-    }, true, b.mixed_bf16 ? 0 : 64, -1, 0, b.mixed_bf16));
-  }
+  for (int q = 0; q < (int)b.ops.size(); ++q)
+    b.dit.push_back(make_table(b, [&](int k) { return delta_prob(b, q, k); }, true, b.mixed_bf16 ? 0 : 64,
+                               b.mixed_bf16 ? kDualTuneOp + q : -1, 0, b.mixed_bf16));
 }
 
 enum ItOp {
@@ -2392,8 +2403,11 @@ int hcc_set_gemm_tuning(hcc_ctx* ctx, int count, const int* entries) {
   std::map<std::array<int, 3>, std::array<int, 2>> m;
   for (int q = 0; q < count; ++q) {
     const int* e = entries + 5 * q;
-    if (e[0] < 0 || e[1] < 1 || e[2] < 1 || (e[3] != 32 && e[3] != 64 && e[3] != 128) || e[4] < 1 || e[4] > 256) {
-      ctx->err = "invalid tuning entry (op, n_occ, n_virt, tile 32|64|128, ksplit 1..256)";
+    const bool dual = e[0] >= kDualTuneOp;
+    const bool tile_ok = dual ? (e[3] == 32 || e[3] == 64 || e[3] == 256) : (e[3] == 32 || e[3] == 64 || e[3] == 128);
+    if (e[0] < 0 || e[1] < 1 || e[2] < 1 || !tile_ok || e[4] < 1 || e[4] > 256) {
+      ctx->err = "invalid tuning entry (op, n_occ, n_virt, tile 32|64|128 -- 32|64|256 for correction ops "
+                 "100 + op --, ksplit 1..256)";
       return HCC_E_ARG;
     }
     m[{e[0], e[1], e[2]}] = {e[3], e[4]};
@@ -2421,23 +2435,43 @@ int hcc_tune_gemm(hcc_ctx* ctx, int op, int tile, int ksplit, int reps, double* 
   if (ctx == nullptr || ms == nullptr || reps < 1) return HCC_E_ARG;
   if (int r = need_batch(ctx)) return r;
   Batch& b = *ctx->batch;
-  if (op < 0 || op >= (int)b.ops.size() || (tile != 0 && tile != 32 && tile != 64 && tile != 128) || ksplit < 0) {
-    ctx->err = "invalid tuning request";
+  const bool dual = op >= kDualTuneOp;
+  const int q = dual ? op - kDualTuneOp : op;
+  const bool tile_ok = tile == 0 || tile == 32 || tile == 64 || (dual ? tile == 256 : tile == 128);
+  if (q < 0 || q >= (int)b.ops.size() || !tile_ok || ksplit < 0 || (dual && !(b.mixed && b.mixed_bf16))) {
+    ctx->err = "invalid tuning request (correction ops 100 + op need a mixed-precision BF16 context)";
     return HCC_E_ARG;
   }
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   const size_t nalloc = b.allocs.size(), nalt = b.alt.size();
-  GemmTable t = tile == 0 ? b.it[op]
-                          : make_table(b, [&](int k) { return b.ops[op](b.hm[k]); }, true, tile, -1,
-                                       ksplit > 0 ? ksplit : 1);
+  GemmTable t;
+  if (dual)
+    t = tile == 0 ? b.dit[q]
+                  : make_table(b, [&](int k) { return delta_prob(b, q, k); }, true, tile, -1, ksplit > 0 ? ksplit : 1,
+                               true);
+  else
+    t = tile == 0 ? b.it[q]
+                  : make_table(b, [&](int k) { return b.ops[q](b.hm[k]); }, true, tile, -1, ksplit > 0 ? ksplit : 1);
+  auto run = [&]() {
+    if (!dual) {
+      launch_table(ctx, t);
+      return;
+    }
+    for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &b.alt[p->alt] : nullptr) {
+      if (p->nprob == 0) continue;
+      HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, b.dact, ctx->stream, p->flags, true, p->tile));
+      if (p->red_blocks > 0)
+        HCHECK(launch_splitk_reduce(p->d, p->red_prefix, p->red_bprob, p->red_blocks, b.dact, ctx->stream));
+    }
+  };
   reset_state(b.dst, b.dact, b.nm, ctx->stream);   // every molecule active
   hipEvent_t e0, e1;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
-  launch_table(ctx, t);
+  run();
   HCHECK(hipEventRecord(e0, ctx->stream));
-  for (int q = 0; q < reps; ++q) launch_table(ctx, t);
+  for (int r = 0; r < reps; ++r) run();
   HCHECK(hipEventRecord(e1, ctx->stream));
   HCHECK(hipEventSynchronize(e1));
   float el = 0.f;

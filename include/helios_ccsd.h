@@ -275,15 +275,17 @@ int hcc_profile_launches(hcc_ctx* ctx, long* n96);
  * Default 1. */
 int hcc_set_graphs(hcc_ctx* ctx, int enable);
 
-/* GEMM autotuning (no reference counterpart).  A contraction `op` (0..23, the
+/* GEMM autotuning (no reference counterpart).  A contraction `op` (0..24, the
  * iteration's grouped GEMMs in launch order) of a molecule with (n_occ, n_virt)
  * runs with the tile kind (64: 64x64, 32: 32x32, 128: 128x32) and K split of a
- * matching entry; molecules
- * without one use the cost heuristic.  entries: count x 5 ints
+ * matching entry; `op` = 100 + c keys the BF16 correction form of contraction
+ * c in the mixed-precision solver (tiles 64, 32, 256: 128x128).  Molecules
+ * without an entry use the cost heuristic.  entries: count x 5 ints
  * (op, n_occ, n_virt, tile, ksplit); applies to molecules loaded afterwards.
- * hcc_tune_gemm times one contraction over the loaded batch (reps launches,
- * mean ms per launch) with a forced tile and ksplit (tile 0: the current
- * choice); it overwrites that contraction's output buffers. */
+ * hcc_tune_gemm times one contraction (or, for 100 + c on a mixed-precision
+ * context, its correction form) over the loaded batch (reps launches, mean ms
+ * per launch) with a forced tile and ksplit (tile 0: the current choice); it
+ * overwrites that contraction's output buffers. */
 int hcc_set_gemm_tuning(hcc_ctx* ctx, int count, const int* entries);
 int hcc_tune_gemm(hcc_ctx* ctx, int op, int tile, int ksplit, int reps, double* ms);
 /* M, N, K of contraction `op` for loaded molecule `mol`. */

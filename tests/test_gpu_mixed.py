@@ -87,3 +87,32 @@ def test_bf16_within_1e6_ev(keep):
     print("fp64 its", it64, "outer", outer, "inner", inner, "dE", e[:, 0] - ref)
     assert (st == 1).all()
     assert np.abs(e[:, 0] - ref).max() < 1e-6
+
+
+@pytest.mark.parametrize("tile,ksplit", [(32, 1), (64, 2), (256, 1)])
+def test_tuned_correction_tiles(tile, ksplit):
+    """Tuning entries for the correction forms (keys 100 + op) change the
+    tiles and K splits of the BF16 correction GEMMs; the solve still reaches
+    the FP64 energies within 1e-6 eV.  An FP64-only tile kind is refused for
+    a correction key, and correction timing needs a mixed context."""
+    mols = batch([3, 5, 12, 20])
+    ref, _ = fp64_energies(mols)
+    sizes = sorted({(m["nocc"], m["C"].shape[0] - m["nocc"]) for m in mols})
+    entries = [[100 + op, o, v, tile, ksplit] for op in range(25) for o, v in sizes]
+    ctx = capi.Context(0, capi.HCC_MIXED, 8)
+    ctx.set_gemm_tuning(entries)
+    ctx.batch_load(mols)
+    assert ctx.tune_gemm(100 + 3, tile, ksplit, 2) > 0.0
+    ctx.batch_solve(capi.GUESS_MP2, 2000, 2e-7)
+    e, it, st = ctx.batch_results()
+    with pytest.raises(capi.HccError):
+        ctx.set_gemm_tuning([[100, 5, 5, 128, 1]])
+    ctx.close()
+    print(tile, ksplit, "dE", e[:, 0] - ref)
+    assert (st == 1).all()
+    assert np.abs(e[:, 0] - ref).max() < 1e-6
+    f64 = capi.Context(0, capi.HCC_FP64, 8)
+    f64.batch_load(mols)
+    with pytest.raises(capi.HccError):
+        f64.tune_gemm(100 + 3, 64, 1, 1)
+    f64.close()
