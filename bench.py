@@ -36,8 +36,8 @@ PEAK_HBM = 8.0e12        # HBM3E bytes/s
 # --skip-reference --steps 1 --warmup 0 --no-profile" summarised in
 # profiles/hbm_traffic_batch4096.md.  A counter pass cannot run inside this process, so
 # other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 3.012e9        # gemm_f64_kernel, FP64 solve
-DUAL_GEMM_TRAFFIC_BATCH4096 = 1.328e9   # gemm_dual_bf16, mixed solve
+GEMM_TRAFFIC_BATCH4096 = 3.010e9        # gemm_f64_kernel, FP64 solve
+DUAL_GEMM_TRAFFIC_BATCH4096 = 1.360e9   # gemm_dual_bf16, mixed solve
 
 
 def _prep(k):
@@ -251,6 +251,7 @@ def main():
     ap.add_argument("--max-inner", type=int, default=5)
     ap.add_argument("--max-outer", type=int, default=30)
     ap.add_argument("--no-profile", action="store_true", help="skip the profiled solves (kernel attribution)")
+    ap.add_argument("--no-lr", action="store_true", help="skip the LR-CCSD leg (64-molecule sample)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (HBM counter calibration for profiler passes)")
     args = ap.parse_args()
@@ -280,7 +281,8 @@ def main():
         e_ref, it_ref, st_ref = ctx.batch_results()
         triples = triples_leg(ctx, o, v)
         ctx.close()
-        lr = lr_leg(mols[:64])
+        if not args.no_lr:
+            lr = lr_leg(mols[:64])
         print(f"# reference solve: statuses {np.unique(st_ref, return_counts=True)}, "
               f"mean iterations {it_ref.mean():.1f}", file=sys.stderr, flush=True)
 
