@@ -126,6 +126,7 @@ def lr_leg(mols, nroots=3):
     ctx = capi.Context(0, capi.HCC_FP64, 20)
     ctx.batch_load(mols)
     ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
+    f_it, _ = ctx.gemm_flops(len(mols))   # residual GEMM flops per molecule and pass
     timers = {"sigma": [0, 0.0], "space": [0, 0.0]}
 
     def timed(fn, key):
@@ -141,7 +142,7 @@ def lr_leg(mols, nroots=3):
     for name in ("lr_dots", "lr_combine", "lr_precondition", "lr_put", "lr_get", "lr_space"):
         setattr(ctx, name, timed(getattr(ctx, name), "space"))
     t1 = time.perf_counter()
-    om, res, its, conv, ael = lrccsd.excitation_energies(
+    om, res, its, conv, ael, r0 = lrccsd.excitation_energies(
         ctx, [(m["nocc"], m["n"] - m["nocc"]) for m in mols], nroots=nroots, gammas=[m["G"] for m in mols],
         log=lambda it, active: print(f"# LR-CCSD iteration {it}: {active} molecules active", file=sys.stderr,
                                      flush=True))
@@ -152,7 +153,15 @@ def lr_leg(mols, nroots=3):
            "sigma_ms_per_call": 1e3 * ts / max(1, ns), "space_calls": nv, "space_s": tv,
            "host_s": wall - ts - tv, "mean_iterations": float(its.mean()),
            "converged": int(conv.sum()), "mean_omega_eV": [float(x) for x in np.nanmean(om, axis=0)],
-           "mean_excitation_level": [float(x) for x in np.nanmean(ael, axis=0)], "start": "cis"}
+           "mean_excitation_level": [float(x) for x in np.nanmean(ael, axis=0)],
+           "mean_reference_weight_r0": [float(x) for x in np.nanmean(r0, axis=0)], "start": "cis"}
+    # every sigma call runs two residual passes over the whole sample
+    sig_rate = 2.0 * float(f_it.sum()) * ns / max(ts, 1e-9)
+    out["roofline_sigma"] = {"bound": "mfma", "achieved": sig_rate / 1e12, "peak": PEAK_FP64 / 1e12,
+                             "unit": "TFLOP/s", "frac": sig_rate / PEAK_FP64,
+                             "work": "2 x residual GEMM flops of the sample per sigma call, over the sigma "
+                                     "calls' wall time (transfers of the slot pointers and elementwise steps "
+                                     "included)"}
     print(f"# LR-CCSD: {len(mols)} molecules, {nroots} roots, {wall:.2f} s: {ns} sigma calls "
           f"({1e3 * ts / max(1, ns):.1f} ms each), {nv} vector-space calls ({tv:.2f} s), host {wall - ts - tv:.2f} s; "
           f"converged {int(conv.sum())}", file=sys.stderr, flush=True)

@@ -48,6 +48,7 @@ SIGNATURES = {
     "hcc_batch_get_orbitals": (_i, [_vp, _i, _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_triples": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_batch_lr_sigma": (_i, [_vp, _d, _P(_d), _P(_d), _P(_d), _P(_d)]),
+    "hcc_batch_get_amplitudes": (_i, [_vp, _i, _P(_d), _P(_d)]),
     "hcc_lr_space": (_i, [_vp, _i]),
     "hcc_lr_put": (_i, [_vp, _i, _i, _P(_d)]),
     "hcc_lr_get": (_i, [_vp, _i, _i, _P(_d)]),
@@ -277,6 +278,14 @@ class Context:
         self._check(self.lib.hcc_batch_lr_sigma(self.h, float(h), _dp(r1), _dp(r2), _dp(s1), _dp(s2)),
                     "hcc_batch_lr_sigma")
         return s1, s2
+
+    def batch_amplitudes(self, mol, o, v, doubles=True):
+        """(t1[i,a], t2[i,j,a,b] or None) of loaded molecule `mol`."""
+        t1 = np.zeros((o, v), order="F")
+        t2 = np.zeros((o, o, v, v), order="F") if doubles else None
+        self._check(self.lib.hcc_batch_get_amplitudes(self.h, int(mol), _dp(t1), _dp(t2) if doubles else None),
+                    "hcc_batch_get_amplitudes")
+        return np.ascontiguousarray(t1), (np.ascontiguousarray(t2) if doubles else None)
 
     # device-resident vector space of the LR-CCSD eigensolver
     def lr_space(self, nslots):

@@ -1979,12 +1979,12 @@ int hcc_energy(hcc_ctx* ctx, double energy_out[5]) {
 }
 
 // t1, t2 of molecule 0 into the reference's column-major layouts (either may be null)
-static void copy_amplitudes_out(hcc_ctx* ctx, double* t1, double* t2) {
+static void copy_amplitudes_out(hcc_ctx* ctx, double* t1, double* t2, int mol = 0) {
   Batch& b = *ctx->batch;
-  const int o = b.o[0], v = b.v[0];
+  const int o = b.o[mol], v = b.v[mol];
   std::vector<double> h1((size_t)o * v), h2((size_t)o * o * v * v);
-  HCHECK(hipMemcpyAsync(h1.data(), b.hm[0].t1, h1.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HCHECK(hipMemcpyAsync(h2.data(), b.hm[0].t2, h2.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HCHECK(hipMemcpyAsync(h1.data(), b.hm[mol].t1, h1.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (t2) HCHECK(hipMemcpyAsync(h2.data(), b.hm[mol].t2, h2.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
   HCHECK(hipStreamSynchronize(ctx->stream));
   if (t1)
     for (int i = 0; i < o; ++i)
@@ -2003,6 +2003,19 @@ int hcc_get_amplitudes(hcc_ctx* ctx, double* t1, double* t2) {
   if (int r = need_batch(ctx)) return r;
   API_BEGIN
   copy_amplitudes_out(ctx, t1, t2);
+  API_END
+}
+
+int hcc_batch_get_amplitudes(hcc_ctx* ctx, int mol, double* t1, double* t2) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (mol < 0 || mol >= ctx->batch->nm) {
+    ctx->err = "molecule index out of range";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  copy_amplitudes_out(ctx, t1, t2, mol);
   API_END
 }
 

@@ -66,6 +66,37 @@ def average_excitation_level(r1, r2):
     return (s1 + 2.0 * s2) / (s1 + s2)
 
 
+def ovov_integrals(C, gamma, o):
+    """(ia|jb) at [i, a, j, b] (ZDO: sum_mn C_mi C_ma gamma_mn C_nj C_nb)."""
+    n = C.shape[0]
+    v = n - o
+    X = (C[:, :o, None] * C[:, None, o:]).reshape(n, o * v)
+    return (X.T @ gamma @ X).reshape(o, v, o, v)
+
+
+def reference_weight(g, f_ov, t1, omega, r1, r2):
+    """The reference's r0 of a state (energyLR, mod_lrccsd.f90:365-379):
+    r0 = (sum_ia Fbar_ia r_ia + 1/4 sum_ijab <ij||ab> r_ijab) / omega over
+    spin orbitals, with Fbar_ia = F_ia + sum_kd t_kd <ik||ad>
+    (proj-lrccsd-spin-hf-intermediates.f90:369-382), for the vector normalised
+    as the reference's ``normalize`` does (sum r1^2 + sum_{i<j, a<b} r2^2 = 1);
+    its magnitude is returned (an eigenvector's sign is arbitrary: the
+    reference's follows from its iteration history).  For a
+    singlet in closed-shell components (r1, r2 pair-symmetric) the spin sums
+    reduce to 2 sum_ia Fbar_ia r1_ia with Fbar_ia = F_ia + sum_kd t1_kd
+    (2 (ia|kd) - (id|ka)), and sum_ijab (2 (ia|jb) - (ib|ja)) r2_ijab (the
+    closed-shell form of 1/4 sum <ij||ab> r_ijab, as in the CCSD energy).
+    g: (ia|jb) at [i, a, j, b]; f_ov: the MO Fock block F[i, o + a]."""
+    aa = r2 - r2.transpose(1, 0, 2, 3)
+    s1 = 2.0 * float(np.sum(r1 * r1))
+    s2 = (4.0 * float(np.sum(r2 * r2)) + 2.0 * float(np.sum(aa * aa))) / 4.0
+    scale = 1.0 / np.sqrt(s1 + s2)
+    fbar = f_ov + 2.0 * np.einsum("kd,iakd->ia", t1, g) - np.einsum("kd,idka->ia", t1, g)
+    num = 2.0 * float(np.sum(fbar * r1)) + float(np.sum((2.0 * g.transpose(0, 2, 1, 3)
+                                                         - g.transpose(0, 2, 3, 1)) * r2))
+    return abs(scale * num / omega)
+
+
 class _Layout:
     """Slots of every molecule's device vectors: basis V and sigmas S (cap
     each), two buffers of Ritz vectors with their sigmas (the current step's
@@ -334,8 +365,9 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
     otherwise from unit singles (its 'unt' guess).
     Returns (omega [nmol, nroots], residual max-norms [nmol, nroots],
     iterations [nmol], converged [nmol], average excitation levels
-    [nmol, nroots]).  ``log(iteration, active)`` is called after every
-    Davidson iteration when given."""
+    [nmol, nroots], reference weights r0 [nmol, nroots] -- NaN without
+    ``gammas``).  ``log(iteration, active)`` is called after every Davidson
+    iteration when given."""
     L = _Layout(nroots, max_space)
     ctx.lr_space(L.nslots)
     ops = _Ops(ctx)
@@ -500,10 +532,19 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
     res = np.array([s.res for s in subs])
     conv = np.array([bool(np.all(s.res[:min(nroots, s.dim)] < eps)) or s.k >= s.dim for s in subs])
     ael = np.full((len(subs), nroots), np.nan)
+    r0 = np.full((len(subs), nroots), np.nan)
     for k, s in enumerate(subs):
+        g = f_ov = t1 = None
+        if gammas is not None:
+            C, Fmo, _ = ctx.batch_orbitals(k, s.o + s.v)
+            g = ovov_integrals(C, np.asarray(gammas[k], dtype=np.float64), s.o)
+            f_ov = Fmo[:s.o, s.o:]
+            t1, _ = ctx.batch_amplitudes(k, s.o, s.v, doubles=False)
         for r, sl in enumerate(s.root_slot):
             if sl is not None:
                 x = ctx.lr_get(k, sl, s.N)
-                ael[k, r] = average_excitation_level(x[:s.n1].reshape(s.o, s.v),
-                                                     x[s.n1:].reshape(s.o, s.o, s.v, s.v))
-    return omega, res, its, conv, ael
+                x1, x2 = x[:s.n1].reshape(s.o, s.v), x[s.n1:].reshape(s.o, s.o, s.v, s.v)
+                ael[k, r] = average_excitation_level(x1, x2)
+                if g is not None:
+                    r0[k, r] = reference_weight(g, f_ov, t1, s.omega[r], x1, x2)
+    return omega, res, its, conv, ael, r0

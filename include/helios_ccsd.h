@@ -131,6 +131,8 @@ int hcc_energy(hcc_ctx* ctx, double energy[5]);
 /* Copy amplitudes out / in (the public t1, t2 of module coupledCluster,
  * mod_cc.f90:69-74). */
 int hcc_get_amplitudes(hcc_ctx* ctx, double* t1, double* t2);
+/* The same for loaded molecule `mol` of a batch (either array may be null). */
+int hcc_batch_get_amplitudes(hcc_ctx* ctx, int mol, double* t1, double* t2);
 int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2);
 
 /* Reference-form residuals d1(i,a), d2(i,j,a,b) at the current amplitudes

@@ -142,3 +142,35 @@ def average_excitation_level(r1, r2):
             for a in range(nv - 1):
                 s2 += float(np.sum(s2v[i, j, a, a + 1:] ** 2))
     return (s1 + 2 * s2) / (s1 + s2)
+
+
+def reference_weight(C, F, G, o, t1, omega, r1, r2):
+    """r0 of energyLR (mod_lrccsd.f90:365-379) over spin orbitals: the vector
+    converted as the amplitudes are (convertSpatialToSpinCC), normalised as
+    ``normalize`` does (mod_lrccsd.f90:393-440: sum r1^2 + sum_{i<j,a<b} r2^2
+    = 1); Fia(i,a) = F(i,a) + sum_kd t1(k,d) R(i,a,k,d)
+    (proj-lrccsd-spin-hf-intermediates.f90:369-382); r0 = (sum_ia Fia r1 +
+    sum_ijab R(j,b,i,a) r2(i,j,a,b) / 4) / omega with R = spin_hf_int.  Its
+    magnitude (the eigenvector's sign is arbitrary)."""
+    from oracle.triples import spin_amplitudes, spin_integrals
+    n = C.shape[0]
+    no, ns = 2 * o, 2 * n
+    R = spin_integrals(C, G)
+    Fs = np.zeros((ns, ns))
+    Fs[0::2, 0::2] = F
+    Fs[1::2, 1::2] = F
+    st1, _ = spin_amplitudes(t1, np.zeros((o, o) + (n - o,) * 2))
+    s1, s2 = spin_amplitudes(r1, r2)
+    norm = np.sum(s1 ** 2)
+    for i in range(no - 1):
+        for j in range(i + 1, no):
+            for a in range(ns - no - 1):
+                norm += np.sum(s2[i, j, a, a + 1:] ** 2)
+    scale = 1.0 / np.sqrt(norm)
+    s1, s2 = s1 * scale, s2 * scale
+    occ, vir = np.arange(no), np.arange(no, ns)
+    Rovov = R[np.ix_(occ, vir, occ, vir)]
+    Fia = Fs[np.ix_(occ, vir)] + np.einsum("kd,iakd->ia", st1, Rovov)
+    sum1 = np.sum(Fia * s1)
+    sum2 = np.einsum("jbia,ijab->", Rovov, s2)
+    return abs((sum1 + sum2 / 4.0) / omega)

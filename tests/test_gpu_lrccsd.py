@@ -8,7 +8,7 @@ against the oracle (oracle/lrccsd.py, pinned by the two-electron full CI):
   their doubly excited lowest state, honeycomb fragments) equal the lowest
   real eigenvalues of the oracle's dense Jacobian within 2e-6 eV, from unit
   singles and from the CIS start vectors, and their average excitation
-  levels equal those of the dense eigenvectors;
+  levels and reference weights r0 equal those of the dense eigenvectors;
 * for two electrons they equal the singlet full-CI excitation energies;
 * a molecule's roots in the batch equal its roots alone;
 * the device vector space's operations (dot products, max-norms, linear
@@ -77,18 +77,21 @@ def test_lowest_roots_match_dense_jacobian(guess):
     preps = [prepare(m) for m in mols]
     ctx = _solved_ctx(preps)
     gammas = [p["G"] for p in preps] if guess == "cis" else None
-    om, res, its, conv, ael = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=3,
-                                                         gammas=gammas)
+    om, res, its, conv, ael, r0 = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols],
+                                                             nroots=3, gammas=gammas)
     ctx.close()
-    for m, p, w, c, it, lv in zip(mols, preps, om, conv, its, ael):
+    for m, p, w, c, it, lv, q in zip(mols, preps, om, conv, its, ael, r0):
         r = ccsd.solve(p["C"], p["F"], p["G"], m.nocc, p["eref"], eps=1e-11, diis_scale=True)
         lam, _, vec = olr.excitation_energies(p["C"], p["F"], p["G"], m.nocc, r["t1"], r["t2"], vectors=True)
         ref_ael = [olr.average_excitation_level(*x) for x in vec[:3]]
         print(m.name, "gpu", w, "oracle", lam[:3], "iterations", it, "diff", np.abs(w - lam[:3]).max(),
-              "AEL", lv, ref_ael)
+              "AEL", lv, ref_ael, "r0", q)
         assert c
         assert np.abs(w - lam[:3]).max() < 2e-6
         assert np.abs(lv - ref_ael).max() < 1e-4
+        if guess == "cis":
+            want = [olr.reference_weight(p["C"], p["F"], p["G"], m.nocc, r["t1"], lam[k], *vec[k]) for k in range(3)]
+            assert np.abs(q - want).max() < 1e-5
 
 
 def test_two_electrons_equal_fci():
@@ -101,7 +104,7 @@ def test_two_electrons_equal_fci():
         preps.append(dict(C=r["C"], F=F, G=G, nocc=1, eref=eref, n=m.n))
         fcis.append(olr.singlets_two_electrons(r["C"], core, G))
     ctx = _solved_ctx(preps)
-    om, res, its, conv, _ = lrccsd.excitation_energies(ctx, [(1, m.n - 1) for m in mols], nroots=2)
+    om, res, its, conv, _, _ = lrccsd.excitation_energies(ctx, [(1, m.n - 1) for m in mols], nroots=2)
     ctx.close()
     for m, w, f, c in zip(mols, om, fcis, conv):
         print(m.name, "gpu", w, "fci", f[:2])
@@ -113,11 +116,11 @@ def test_batch_equals_alone():
     mols = [M.polyene(6), M.random_fragment(10, seed=4)]
     preps = [prepare(m) for m in mols]
     ctx = _solved_ctx(preps)
-    om, _, _, _, _ = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=2)
+    om, _, _, _, _, _ = lrccsd.excitation_energies(ctx, [(m.nocc, m.n - m.nocc) for m in mols], nroots=2)
     ctx.close()
     for k, (m, p) in enumerate(zip(mols, preps)):
         one = _solved_ctx([p])
-        w1, _, _, _, _ = lrccsd.excitation_energies(one, [(m.nocc, m.n - m.nocc)], nroots=2)
+        w1, _, _, _, _, _ = lrccsd.excitation_energies(one, [(m.nocc, m.n - m.nocc)], nroots=2)
         one.close()
         assert np.abs(w1[0] - om[k]).max() < 1e-6
 
