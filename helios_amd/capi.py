@@ -16,6 +16,7 @@ HCC_MIXED = 1
 GUESS_REFERENCE = 0
 GUESS_MP2 = 1
 GUESS_LEARNED = 2
+GUESS_CURRENT = 3   # the amplitudes already in the context (warm start)
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
 MLP_ROW = 18
@@ -49,6 +50,7 @@ SIGNATURES = {
     "hcc_batch_triples": (_i, [_vp, _P(_d), _P(_d)]),
     "hcc_batch_lr_sigma": (_i, [_vp, _d, _P(_d), _P(_d), _P(_d), _P(_d)]),
     "hcc_batch_get_amplitudes": (_i, [_vp, _i, _P(_d), _P(_d)]),
+    "hcc_batch_set_amplitudes": (_i, [_vp, _i, _P(_d), _P(_d)]),
     "hcc_lr_space": (_i, [_vp, _i]),
     "hcc_lr_put": (_i, [_vp, _i, _i, _P(_d)]),
     "hcc_lr_get": (_i, [_vp, _i, _i, _P(_d)]),
@@ -286,6 +288,11 @@ class Context:
         self._check(self.lib.hcc_batch_get_amplitudes(self.h, int(mol), _dp(t1), _dp(t2) if doubles else None),
                     "hcc_batch_get_amplitudes")
         return np.ascontiguousarray(t1), (np.ascontiguousarray(t2) if doubles else None)
+
+    def batch_set_amplitudes(self, mol, t1, t2):
+        """Amplitudes t1[i,a], t2[i,j,a,b] of loaded molecule `mol` (for GUESS_CURRENT)."""
+        a, b = fortran(t1), fortran(t2)
+        self._check(self.lib.hcc_batch_set_amplitudes(self.h, int(mol), _dp(a), _dp(b)), "hcc_batch_set_amplitudes")
 
     # device-resident vector space of the LR-CCSD eigensolver
     def lr_space(self, nslots):

@@ -174,6 +174,9 @@ struct Batch {
   // device-resident vector space of the eigensolver (hcc_lr.h): lr_slots
   // vectors of lr_len[k] doubles per molecule from element lr_base[k]; own
   // allocations (not in `allocs`), replaced when the slot count changes
+  // molecules whose amplitude arenas hold meaningful amplitudes (a guess, a
+  // solve or hcc_batch_set_amplitudes): HCC_GUESS_CURRENT needs all of them
+  std::vector<char> amp_valid;
   int lr_slots = 0;
   double* dlr_space = nullptr;
   std::vector<long> lr_len, lr_base;
@@ -1112,9 +1115,16 @@ static void precompute(hcc_ctx* c) {
   HCHECK(hipGetLastError());
 }
 
-// initial amplitudes of every molecule; the learned guess corrects the MP2 one
+// initial amplitudes of every molecule; the learned guess corrects the MP2
+// one; HCC_GUESS_CURRENT keeps the amplitudes in place and only resets the
+// iteration state
 static void apply_guess(hcc_ctx* c, int kind) {
   Batch& b = *c->batch;
+  b.amp_valid.assign(b.nm, 1);
+  if (kind == HCC_GUESS_CURRENT) {
+    reset_state(b.dst, b.dact, b.nm, c->stream);
+    return;
+  }
   prof_open(c, kProfGuess);
   guess(b.dm, b.dst, b.dact, b.rgT2S, b.nm, kind == HCC_GUESS_LEARNED ? HCC_GUESS_MP2 : kind, c->stream);
   if (kind == HCC_GUESS_LEARNED) {
@@ -1907,9 +1917,20 @@ static int need_batch(hcc_ctx* ctx) {
 }
 
 static int check_guess(hcc_ctx* ctx, int kind) {
-  if (kind != HCC_GUESS_REFERENCE && kind != HCC_GUESS_MP2 && kind != HCC_GUESS_LEARNED) {
+  if (kind != HCC_GUESS_REFERENCE && kind != HCC_GUESS_MP2 && kind != HCC_GUESS_LEARNED &&
+      kind != HCC_GUESS_CURRENT) {
     ctx->err = "unsupported guess kind";
     return HCC_E_ARG;
+  }
+  if (kind == HCC_GUESS_CURRENT) {
+    const Batch& b = *ctx->batch;
+    bool all = (int)b.amp_valid.size() == b.nm;
+    for (size_t k = 0; all && k < b.amp_valid.size(); ++k) all = b.amp_valid[k] != 0;
+    if (!all) {
+      ctx->err = "HCC_GUESS_CURRENT needs amplitudes for every molecule (a solve, a guess or "
+                 "hcc_batch_set_amplitudes)";
+      return HCC_E_STATE;
+    }
   }
   if (kind == HCC_GUESS_LEARNED && ctx->guess_net == nullptr) {
     ctx->err = "no guess network set (hcc_set_guess_network)";
@@ -2019,12 +2040,9 @@ int hcc_batch_get_amplitudes(hcc_ctx* ctx, int mol, double* t1, double* t2) {
   API_END
 }
 
-int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2) {
-  if (ctx == nullptr || t1 == nullptr || t2 == nullptr) return HCC_E_ARG;
-  if (int r = need_batch(ctx)) return r;
-  API_BEGIN
+static void copy_amplitudes_in(hcc_ctx* ctx, int mol, const double* t1, const double* t2) {
   Batch& b = *ctx->batch;
-  const int o = b.o[0], v = b.v[0];
+  const int o = b.o[mol], v = b.v[mol];
   std::vector<double> h1((size_t)o * v), h2((size_t)o * o * v * v);
   for (int i = 0; i < o; ++i)
     for (int a = 0; a < v; ++a) h1[(size_t)i * v + a] = t1[i + (size_t)a * o];
@@ -2033,10 +2051,36 @@ int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2) {
       for (int a = 0; a < v; ++a)
         for (int bb = 0; bb < v; ++bb)
           h2[(((size_t)i * o + j) * v + a) * v + bb] = t2[i + (size_t)o * (j + (size_t)o * (a + (size_t)v * bb))];
-  HCHECK(hipMemcpyAsync(b.hm[0].t1, h1.data(), h1.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  HCHECK(hipMemcpyAsync(b.hm[0].t2, h2.data(), h2.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  if (!ctx->have_guess) reset_state(b.dst, b.dact, b.nm, ctx->stream);
+  HCHECK(hipMemcpyAsync(b.hm[mol].t1, h1.data(), h1.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  HCHECK(hipMemcpyAsync(b.hm[mol].t2, h2.data(), h2.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   HCHECK(hipStreamSynchronize(ctx->stream));
+  if ((int)b.amp_valid.size() != b.nm) b.amp_valid.assign(b.nm, 0);
+  b.amp_valid[mol] = 1;
+}
+
+int hcc_batch_set_amplitudes(hcc_ctx* ctx, int mol, const double* t1, const double* t2) {
+  if (ctx == nullptr || t1 == nullptr || t2 == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  if (mol < 0 || mol >= ctx->batch->nm) {
+    ctx->err = "molecule index out of range";
+    return HCC_E_ARG;
+  }
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  copy_amplitudes_in(ctx, mol, t1, t2);
+  API_END
+}
+
+int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2) {
+  if (ctx == nullptr || t1 == nullptr || t2 == nullptr) return HCC_E_ARG;
+  if (int r = need_batch(ctx)) return r;
+  API_BEGIN
+  Batch& b = *ctx->batch;
+  copy_amplitudes_in(ctx, 0, t1, t2);
+  if (!ctx->have_guess) {
+    reset_state(b.dst, b.dact, b.nm, ctx->stream);
+    HCHECK(hipStreamSynchronize(ctx->stream));
+  }
   ctx->have_guess = true;
   API_END
 }

@@ -144,16 +144,49 @@ def prepare_points(mol, points, step, kind="ohno-klopman"):
     return out
 
 
+def rotate_amplitudes(t1, t2, C_from, C_to, o):
+    """Amplitudes of one field point carried into the orbitals of another,
+    with U = C_to^T C_from (the site basis is orthonormal): t2'_ijab =
+    Uo_ik Uo_jl Uv_ac Uv_bd t2_klcd and t1' = Uo t1 Uv^T + X^T, so that a sign
+    flip or a rotation among near-degenerate orbitals between the points
+    carries over.  X = U_vo Uo^-1 is the occupied-virtual mixing between the
+    two references written as a singles excitation (Thouless: the old
+    reference is e^X times the new one, up to normalisation); its other
+    first-order effects are left out (a guess only)."""
+    U = C_to.T @ C_from
+    Uo, Uv = U[:o, :o], U[o:, o:]
+    X = U[o:, :o] @ np.linalg.inv(Uo)
+    return (Uo @ t1 @ Uv.T + X.T,
+            np.einsum("ik,jl,klcd,ac,bd->ijab", Uo, Uo, t2, Uv, Uv, optimize=True))
+
+
 def compute(mol, scales="x", npoints=5, step=0.02, precision=capi.HCC_FP64, eps=1e-10, diis_steps=20,
-            guess=capi.GUESS_MP2, maxiter=500):
+            guess=capi.GUESS_MP2, maxiter=500, warm=False):
     """Energies at every field point (one batched GPU solve) and their
     derivatives.  Returns dict(points, energy, status, iterations, dipole,
-    alpha, beta, gamma, gamma_mixed) with the reference's units."""
+    alpha, beta, gamma, gamma_mixed) with the reference's units.
+
+    warm: solve the zero-field point first and start every point from its
+    amplitudes (rotated into the point's orbitals, HCC_GUESS_CURRENT) instead
+    of from ``guess``; the reference starts every point from guessCC
+    (mod_property.f90:467-486 calls getEnergy per point)."""
     points = field_points(scales, npoints)
     inputs = prepare_points(mol, points, step)
     ctx = capi.Context(0, precision, diis_steps)
     ctx.batch_load(inputs)
-    ctx.batch_solve(guess, maxiter, eps)
+    if warm:
+        o, v = mol.nocc, mol.n - mol.nocc
+        z = points.index((0, 0, 0))
+        zero = capi.Context(0, precision, diis_steps)
+        zero.batch_load([inputs[z]])
+        zero.batch_solve(guess, maxiter, eps)
+        t1, t2 = zero.batch_amplitudes(0, o, v)
+        zero.close()
+        for k, inp in enumerate(inputs):
+            ctx.batch_set_amplitudes(k, *rotate_amplitudes(t1, t2, inputs[z]["C"], inp["C"], o))
+        ctx.batch_solve(capi.GUESS_CURRENT, maxiter, eps)
+    else:
+        ctx.batch_solve(guess, maxiter, eps)
     e, it, st = ctx.batch_results()
     ctx.close()
     grid = {p: e[k, 0] for k, p in enumerate(points)}
@@ -180,6 +213,7 @@ def main():
     ap.add_argument("--step", type=float, default=0.02)
     ap.add_argument("--mixed", action="store_true", help="mixed-precision solver")
     ap.add_argument("--eps", type=float, default=1e-10)
+    ap.add_argument("--warm", action="store_true", help="start every point from the zero-field amplitudes")
     a = ap.parse_args()
     if a.molecule == "octatetraene":
         mol = molecules.octatetraene()
@@ -190,7 +224,8 @@ def main():
         mol = molecules.random_fragment(int(n), seed=int(seed))
     else:
         sys.exit(f"unknown molecule {a.molecule}")
-    r = compute(mol, a.components, a.points, a.step, capi.HCC_MIXED if a.mixed else capi.HCC_FP64, a.eps)
+    r = compute(mol, a.components, a.points, a.step, capi.HCC_MIXED if a.mixed else capi.HCC_FP64, a.eps,
+                warm=a.warm)
     out = {k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in r.items()}
     print(json.dumps(out))
 

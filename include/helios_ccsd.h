@@ -40,6 +40,9 @@ typedef struct hcc_ctx hcc_ctx;
 #define HCC_GUESS_REFERENCE 0 /* t2 = (2(ia|jb)-(ib|ja))/D, mod_cc.f90:634-650 */
 #define HCC_GUESS_MP2 1       /* t2 = (ia|jb)/D, mod_mbpt.f90:131-175 amplitudes */
 #define HCC_GUESS_LEARNED 2   /* MP2 amplitudes + learned MLP correction (hcc_set_guess_network) */
+#define HCC_GUESS_CURRENT 3   /* the amplitudes already in the context: a previous solve, hcc_set_amplitudes
+                                 or hcc_batch_set_amplitudes for every molecule (warm start; no reference
+                                 counterpart -- guessCC always starts from its formula) */
 
 /* Create a context on HIP device `device`.  precision: HCC_FP64 or HCC_MIXED.
  * diis_steps: DIIS depth (ccbd%diisSteps, src/input/definebd.f90:152), 2..32.
@@ -131,8 +134,10 @@ int hcc_energy(hcc_ctx* ctx, double energy[5]);
 /* Copy amplitudes out / in (the public t1, t2 of module coupledCluster,
  * mod_cc.f90:69-74). */
 int hcc_get_amplitudes(hcc_ctx* ctx, double* t1, double* t2);
-/* The same for loaded molecule `mol` of a batch (either array may be null). */
+/* The same for loaded molecule `mol` of a batch (either array may be null
+ * when reading). */
 int hcc_batch_get_amplitudes(hcc_ctx* ctx, int mol, double* t1, double* t2);
+int hcc_batch_set_amplitudes(hcc_ctx* ctx, int mol, const double* t1, const double* t2);
 int hcc_set_amplitudes(hcc_ctx* ctx, const double* t1, const double* t2);
 
 /* Reference-form residuals d1(i,a), d2(i,j,a,b) at the current amplitudes

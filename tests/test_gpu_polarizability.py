@@ -52,3 +52,24 @@ def test_two_electrons_alpha_equals_fci():
     print("ethylene alpha GPU", r["alpha"]["xx"], "FCI", a_fci)
     assert np.abs(r["energy"] - e_fci).max() < 1e-9
     assert abs(r["alpha"]["xx"] - a_fci) < 1e-6 * abs(a_fci)
+
+
+def test_warm_start_from_the_zero_field_amplitudes():
+    """Starting every field point from the zero-field amplitudes (rotated into
+    its orbitals, HCC_GUESS_CURRENT) reaches the same energies in fewer
+    iterations (a few per cent: the start is ~40x closer, but convergence to
+    1e-10 is set by the asymptotic rate); HCC_GUESS_CURRENT without
+    amplitudes for every molecule is refused."""
+    from helios_amd import capi
+    mol = M.random_fragment(10, seed=7)
+    cold = pol.compute(mol, "xy", 5, 0.02, eps=1e-10)
+    warm = pol.compute(mol, "xy", 5, 0.02, eps=1e-10, warm=True)
+    print("iterations cold", cold["iterations"].mean(), "warm", warm["iterations"].mean())
+    assert (warm["status"] == 1).all()
+    assert np.abs(warm["energy"] - cold["energy"]).max() < 1e-9
+    assert warm["iterations"].mean() < cold["iterations"].mean()
+    ctx = capi.Context(0, capi.HCC_FP64, 8)
+    ctx.batch_load(pol.prepare_points(mol, pol.field_points("x", 3), 0.02))
+    with pytest.raises(capi.HccError):
+        ctx.batch_solve(capi.GUESS_CURRENT, 100, 1e-8)
+    ctx.close()

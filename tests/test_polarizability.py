@@ -86,3 +86,25 @@ def test_product_field_inputs_match_oracle_scf():
     for k, p in enumerate(ins):
         r = ccsd.solve(p["C"], p["F"], p["G"], 4, p["eref"], eps=1e-10, diis_scale=True)
         assert abs(r["energy"][0] - e_or[k]) < 1e-8
+
+
+def test_rotated_amplitudes_solve_the_rotated_equations():
+    """CCSD is invariant under rotations among the occupied and among the
+    virtual orbitals: the amplitudes solved in rotated orbitals C' = C Q
+    (Q block-orthogonal) equal the ones solved in C and carried over by
+    rotate_amplitudes.  A transposed block or a missing factor fails."""
+    from tests._molprep import prepare
+    mol = M.octatetraene()
+    p = prepare(mol)
+    o, n = mol.nocc, p["n"]
+    r0 = ccsd.solve(p["C"], p["F"], p["G"], o, p["eref"], eps=1e-11, diis_scale=True)
+    rng = np.random.default_rng(4)
+    Q = np.zeros((n, n))
+    Q[:o, :o] = np.linalg.qr(rng.standard_normal((o, o)))[0]
+    Q[o:, o:] = np.linalg.qr(rng.standard_normal((n - o, n - o)))[0]
+    C1 = p["C"] @ Q
+    F1 = Q.T @ p["F"] @ Q
+    r1 = ccsd.solve(C1, F1, p["G"], o, p["eref"], eps=1e-11, diis_scale=True)
+    t1, t2 = pol.rotate_amplitudes(r0["t1"], r0["t2"], p["C"], C1, o)
+    assert abs(r1["energy"][0] - r0["energy"][0]) < 1e-9
+    assert np.abs(t1 - r1["t1"]).max() < 1e-8 and np.abs(t2 - r1["t2"]).max() < 1e-8
