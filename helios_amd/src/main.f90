@@ -1,6 +1,6 @@
 ! helios_gpu: host program for the GPU r-ccsd path.
 !
-!   helios_gpu -f input.inp          r-ccsd on the GPU
+!   helios_gpu -f input.inp          r-ccsd (or r-ccsd(t)) on the GPU
 !   helios_gpu -f input.inp -scf     RHF only (no device needed)
 !
 ! Reads the reference's input blocks, builds the PPP model, runs the RHF and
@@ -22,7 +22,8 @@ program helios_gpu
     type(hmol)         :: mol
     character(len=512) :: arg, fname
     integer(kind=iglu) :: k, status, iterations, m
-    real(kind=rglu)    :: energy(5), escf, eref
+    real(kind=rglu)    :: energy(5), escf, eref, energyT(5), et(2)
+    logical            :: triples
     real(kind=rglu), allocatable :: V(:,:), E(:), Fmo(:,:)
     logical :: scfOnly
 
@@ -61,11 +62,13 @@ program helios_gpu
     end if
 
     do m = 1, max(ctl%nmethods, 1)
+        triples = .false.
         if (ctl%nmethods > 0) then
-            if (trim(ctl%methods(m)) /= 'r-ccsd') then
+            if (trim(ctl%methods(m)) /= 'r-ccsd' .and. trim(ctl%methods(m)) /= 'r-ccsd(t)') then
                 write (*, '(A,A,A)') ' method ', trim(ctl%methods(m)), ' is not provided by helios_gpu; skipped'
                 cycle
             end if
+            triples = trim(ctl%methods(m)) == 'r-ccsd(t)'
         end if
         call setCCParameters(ctl, mol)
         call initCC(ctl, mol)
@@ -77,6 +80,12 @@ program helios_gpu
             call energyCC(energy)
         end if
         write (*, '(/A,I0,A,I0)') ' r-ccsd status ', status, '  iterations ', iterations
+        if (triples .and. status == 1) then
+            call energyCCT(energyT, et)
+            write (*, '(A,F24.15,A)') ' CCSD energy ', energy(1), ' eV'
+            write (*, '(A,2F24.15,A)') ' (T) E[T], E[ST] ', et(1), et(2), ' eV'
+            energy = energyT
+        end if
         write (*, '(A,F24.15,A)') ' Resulting energy ', energy(1), ' eV'
         write (*, '(A,F24.15,A)') ' Reference energy ', energy(2), ' eV'
         write (*, '(A,F24.15,A)') ' Correlation energy ', energy(3) + energy(4), ' eV'

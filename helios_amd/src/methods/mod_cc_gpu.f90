@@ -19,7 +19,7 @@ module coupledClusterGPU
     real(kind=rglu)    :: refeEnergy
     real(kind=rglu), allocatable :: hV(:,:), E(:), F(:,:)
 
-    public :: setCCParameters, initCC, iterationCC, energyCC, finalizeCC, solveCC, getAmplitudes
+    public :: setCCParameters, initCC, iterationCC, energyCC, finalizeCC, solveCC, getAmplitudes, energyCCT
     public :: refeEnergy, N, Nocc
 
 contains
@@ -99,6 +99,16 @@ contains
         real(kind=rglu), intent(out) :: energy(5)
         call gpuCheck(hcc_energy(ctx, energy), 'hcc_energy', ctx)
     end subroutine energyCC
+
+    ! r-ccsd(t): the reference routes it to the spin-orbital CCSD and its (T)
+    ! loop (mod_cc.f90:109-118, 2028-2111); for the closed-shell RHF reference
+    ! the same energies follow from the converged closed-shell amplitudes.
+    ! energy: CCSD + c1 + c2 + c3, reference, singles, doubles, c1 (the
+    ! reference's layout and signs); et: E[T], E[ST]
+    subroutine energyCCT(energy, et)
+        real(kind=rglu), intent(out) :: energy(5), et(2)
+        call gpuCheck(hcc_batch_triples(ctx, energy, et), 'hcc_batch_triples', ctx)
+    end subroutine energyCCT
 
     ! whole solve on the device (iterator rules applied there)
     subroutine solveCC(ctl, energy, iterations, status)

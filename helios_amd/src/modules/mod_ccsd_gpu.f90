@@ -87,6 +87,14 @@ module ccsdGPU
             integer(c_int), intent(out) :: iterations, status
         end function hcc_solve
 
+        ! (T) of the converged amplitudes: energy(5) as energyCC('spin-r-ccsd(t)')
+        ! (mod_cc.f90:2028-2111), e_t = standard E[T], E[ST]
+        integer(c_int) function hcc_batch_triples(ctx, energy, e_t) bind(C, name='hcc_batch_triples')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: energy(5), e_t(2)
+        end function hcc_batch_triples
+
         subroutine hcc_destroy(ctx) bind(C, name='hcc_destroy')
             import :: c_ptr
             type(c_ptr), value :: ctx

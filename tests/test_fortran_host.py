@@ -64,3 +64,26 @@ def test_host_mixed_learned_on_gpu(tmp_path):
     print(out[-600:])
     assert "r-ccsd status 1" in out
     assert abs(energy_of(out) - ref["energy"][0]) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("driver", ["host", "device"])
+def test_host_rccsd_t_on_gpu(tmp_path, driver):
+    """method=r-ccsd(t) through the Fortran host: the resulting energy is the
+    CCSD energy plus c1 + c2 + c3 of the restated spin-orbital (T) loop."""
+    from oracle import triples
+    mol = M.octatetraene()
+    inp = tmp_path / f"{mol.name}.inp"
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran host not built")
+    inp.write_text(M.to_input(mol, method="r-ccsd(t)", extra=f"driver={driver}, diis-steps=20"))
+    r = subprocess.run([EXE, "-f", str(inp)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout
+    p = prepare(mol)
+    ref = ccsd.solve(p["C"], p["F"], p["G"], mol.nocc, p["eref"], eps=1e-10, diis_scale=True)
+    c = triples.triples(p["C"], p["F"], p["G"], mol.nocc, ref["t1"], ref["t2"])
+    print(out[-700:], "oracle", ref["energy"][0], c)
+    assert "r-ccsd status 1" in out
+    assert abs(energy_of(out, "CCSD energy") - ref["energy"][0]) < 1e-8
+    assert abs(energy_of(out) - (ref["energy"][0] + sum(c))) < 1e-8
