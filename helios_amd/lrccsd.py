@@ -210,40 +210,110 @@ class _Ops:
         self.ctx.lr_combine(np.array(head, dtype=np.int32), np.array(beta), np.array(ts, dtype=np.int32),
                             np.array(tc))
 
+    def combine_blocks(self, blocks):
+        """blocks: list of (mol, dst slots [m], beta, term slots [t], coefficients [m, t]):
+        dst_q = beta dst_q + sum_j coef[q, j] slot_j, built as arrays."""
+        heads, betas, tss, tcs = [], [], [], []
+        at = 0
+        for mol, dst, beta, slots, coef in blocks:
+            m, t = coef.shape
+            if m == 0:
+                continue
+            h = np.empty((m, 4), dtype=np.int32)
+            h[:, 0] = mol
+            h[:, 1] = dst
+            h[:, 2] = at + t * np.arange(m)
+            h[:, 3] = t
+            heads.append(h)
+            betas.append(np.full(m, beta))
+            tss.append(np.tile(np.asarray(slots, dtype=np.int32), m))
+            tcs.append(coef.ravel())
+            at += m * t
+        if heads:
+            self.ctx.lr_combine(np.concatenate(heads), np.concatenate(betas), np.concatenate(tss),
+                                np.concatenate(tcs))
+
+
+def _accept(M, n0, thresh, room):
+    """Gram-Schmidt in the coordinates of the candidates, from their Gram
+    matrix M: rows of T give the accepted orthonormal vectors as combinations
+    of the candidates, in order; a candidate whose part orthogonal to the
+    accepted ones is below ``thresh`` of its original norm n0 is dropped (each
+    projection applied twice)."""
+    T = []
+    for r in range(len(M)):
+        if len(T) >= room:
+            break
+        c = np.zeros(len(M))
+        c[r] = 1.0
+        for _ in range(2):
+            for t in T:
+                c = c - (t @ M @ c) * t
+        nrm = np.sqrt(max(float(c @ M @ c), 0.0))
+        if nrm > thresh * n0[r]:
+            T.append(c / nrm)
+    return np.array(T).reshape(-1, len(M))
+
 
 def _orthonormalize(ops, L, subs, cands, thresh):
     """Append the candidates (per molecule: list of (slot, norm before the
-    projections)) to the basis as V[k + pending], each without its components
-    along the basis and the candidates accepted before it (Gram-Schmidt,
-    twice), normalised; a candidate keeps less than ``thresh`` of its norm
-    only if the basis already holds it, and is dropped."""
-    width = max((len(c) for c in cands.values()), default=0)
-    for r in range(width):
-        act = [k for k, c in cands.items()
-               if r < len(c) and subs[k].k + subs[k].pending < min(subs[k].dim, L.cap)]
-        if not act:
-            continue
-        for _ in range(2):
-            jobs, spans = [], []
-            for k in act:
-                s, sl = subs[k], cands[k][r][0]
-                nb = s.k + s.pending
-                spans.append((len(jobs), nb))
-                jobs.extend((k, sl, L.V + j) for j in range(nb))
-            P = ops.dots(jobs)
-            cj = []
-            for k, (at, nb) in zip(act, spans):
-                if nb:
-                    cj.append((k, cands[k][r][0], 1.0, [(L.V + j, -P[at + j]) for j in range(nb)]))
-            ops.combine(cj)
-        nrm = np.sqrt(np.maximum(ops.dots([(k, cands[k][r][0], cands[k][r][0]) for k in act]), 0.0))
-        cj = []
-        for k, nr in zip(act, nrm):
+    projections)) to the basis as V[k + pending...]: their components along
+    the basis are removed by block Gram-Schmidt (twice), then they are
+    orthonormalised among themselves in order from their Gram matrix; a
+    candidate that keeps less than ``thresh`` of its norm is already held by
+    the basis and is dropped."""
+    act = [k for k, c in cands.items() if c and subs[k].k + subs[k].pending < min(subs[k].dim, L.cap)]
+    if not act:
+        return
+    for _ in range(2):
+        jobs, spans = [], []
+        for k in act:
             s = subs[k]
-            if nr > thresh * cands[k][r][1]:
-                cj.append((k, L.V + s.k + s.pending, 0.0, [(cands[k][r][0], 1.0 / nr)]))
-                s.pending += 1
-        ops.combine(cj)
+            nb = s.k + s.pending
+            cs = np.array([sl for sl, _ in cands[k]], dtype=np.int32)
+            if nb == 0:
+                continue
+            J = np.empty((len(cs), nb, 3), dtype=np.int32)
+            J[:, :, 0] = k
+            J[:, :, 1] = cs[:, None]
+            J[:, :, 2] = L.V + np.arange(nb)[None, :]
+            spans.append((k, sum(len(j) for j in jobs), cs, nb))
+            jobs.append(J.reshape(-1, 3))
+        if not jobs:
+            break
+        P = ops.dots(np.concatenate(jobs))
+        blocks = []
+        for k, at, cs, nb in spans:
+            coef = -P[at:at + len(cs) * nb].reshape(len(cs), nb)
+            for q, sl in enumerate(cs):
+                blocks.append((k, [sl], 1.0, L.V + np.arange(nb), coef[q:q + 1]))
+        ops.combine_blocks(blocks)
+    jobs, spans = [], []
+    for k in act:
+        cs = [sl for sl, _ in cands[k]]
+        m = len(cs)
+        r, c = np.triu_indices(m)
+        J = np.empty((len(r), 3), dtype=np.int32)
+        J[:, 0] = k
+        J[:, 1] = np.asarray(cs)[r]
+        J[:, 2] = np.asarray(cs)[c]
+        spans.append((k, sum(len(j) for j in jobs), r, c))
+        jobs.append(J)
+    g = ops.dots(np.concatenate(jobs))
+    blocks = []
+    for k, at, r, c in spans:
+        s = subs[k]
+        cs = [sl for sl, _ in cands[k]]
+        m = len(cs)
+        M = np.zeros((m, m))
+        M[r, c] = g[at:at + len(r)]
+        M[c, r] = g[at:at + len(r)]
+        room = min(s.dim, L.cap) - s.k - s.pending
+        T = _accept(M, np.array([n0 for _, n0 in cands[k]]), thresh, room)
+        if len(T):
+            blocks.append((k, L.V + s.k + s.pending + np.arange(len(T)), 0.0, cs, T))
+            s.pending += len(T)
+    ops.combine_blocks(blocks)
 
 
 def _collapse(ops, L, subs, which):
@@ -314,13 +384,20 @@ def _collapse(ops, L, subs, which):
 def _absorb(ops, L, subs, act):
     """Extend G by the rows and columns of the vectors whose sigmas just arrived."""
     jobs, spans = [], []
+    at = 0
     for k in act:
         s = subs[k]
         k0, k1 = s.k, s.k + s.pending
-        spans.append((len(jobs), k0, k1))
-        jobs.extend((k, L.V + a, L.S + b) for a in range(k1) for b in range(k0, k1))
-        jobs.extend((k, L.V + a, L.S + b) for a in range(k0, k1) for b in range(k0))
-    g = ops.dots(jobs)
+        a, b = np.meshgrid(np.arange(k1), np.arange(k0, k1), indexing="ij")      # new columns, all rows
+        c, d = np.meshgrid(np.arange(k0, k1), np.arange(k0), indexing="ij")      # new rows, old columns
+        J = np.empty((a.size + c.size, 3), dtype=np.int32)
+        J[:, 0] = k
+        J[:, 1] = L.V + np.concatenate([a.ravel(), c.ravel()])
+        J[:, 2] = L.S + np.concatenate([b.ravel(), d.ravel()])
+        jobs.append(J)
+        spans.append((at, k0, k1))
+        at += len(J)
+    g = ops.dots(np.concatenate(jobs))
     for k, (at, k0, k1) in zip(act, spans):
         s = subs[k]
         G = np.zeros((k1, k1))
@@ -423,17 +500,17 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
         _absorb(ops, L, subs, act)
         # Ritz vectors and their sigmas into the current buffer
         ritz = {}
-        cj = []
+        blocks = []
         for k in act:
             s = subs[k]
             entries, rows = _ritz(s)
             ritz[k] = entries
             s.rows = len(rows)
             xb = L.XB[s.cur]
-            for q, y in enumerate(rows):
-                cj.append((k, xb + q, 0.0, [(L.V + j, y[j]) for j in range(s.k)]))
-                cj.append((k, xb + L.R + q, 0.0, [(L.S + j, y[j]) for j in range(s.k)]))
-        ops.combine(cj)
+            Y = np.array(rows).reshape(-1, s.k)
+            blocks.append((k, xb + np.arange(len(Y)), 0.0, L.V + np.arange(s.k), Y))
+            blocks.append((k, xb + L.R + np.arange(len(Y)), 0.0, L.S + np.arange(s.k), Y))
+        ops.combine_blocks(blocks)
         # residuals J x - lam x into the candidate slots, and their max-norms
         cj, mj, where = [], [], {}
         for k in act:
