@@ -144,23 +144,27 @@ typedef double trip_d4 __attribute__((ext_vector_type(4)));
 constexpr int kTripMfmaThreads = 512;
 constexpr int kTripKD = (kTripLdsV + 3) / 4;   // K steps over d
 constexpr int kTripKL = kTripMaxOV / 4;        // K steps over l
-// LDS (doubles): W[v^3], B operands 4 x 32 x 32 (reused for the L slices of
-// the energy pass), orbital energies and t1 rows, reduction
+// LDS (doubles): W[v^3], B operands 2 x 32 x kTripBC (reused for the L slices
+// of the energy pass), orbital energies and t1 rows, reduction
 // W[a][b][c] at a S2 + b S1 + c with S1 = v + 1 and S2 = v S1 + 1 (both odd):
 // accesses along any one index are free of LDS bank conflicts
 constexpr int kTripLdsW = kTripLdsV * (kTripLdsV * (kTripLdsV + 1) + 1);
 constexpr int kTripLdsE = 4 * kTripLdsV;
+// The two permutations of a pair share the A operand, so their B operands sit
+// side by side as one operand of 2v columns (the first permutation's at
+// columns z, the second's at v + z): ceil(2v / 16) column tiles per product
+// instead of 2 ceil(v / 16), no padding columns for v = 24 (48 = 3 x 16).
+constexpr int kTripBC = 48;
+static_assert(kTripBC >= 2 * kTripLdsV && kTripBC % 16 == 0, "combined B columns");
 
 __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol* __restrict__ mols, int nm,
                                                                    long total, const int2* __restrict__ items,
                                                                    double* __restrict__ part) {
   extern __shared__ double lds[];
   double* W = lds;
-  double* Tb = W + kTripLdsW;                 // [d][z], 32 x 32, permutation 2h
-  double* Qb = Tb + 32 * 32;                  // [l][z], 32 x 32
-  double* Tb2 = Qb + 32 * 32;                 // the same for permutation 2h + 1
-  double* Qb2 = Tb2 + 32 * 32;
-  double* E = Qb2 + 32 * 32;                  // e_vir (v), t1 rows of i, j, k
+  double* Tb = W + kTripLdsW;                 // [d][c], 32 x kTripBC: t_rq^{z d} | t_qr^{z d}
+  double* Qb = Tb + 32 * kTripBC;             // [l][c], 32 x kTripBC: (ql|rz) | (rl|qz)
+  double* E = Qb + 32 * kTripBC;              // e_vir (v), t1 rows of i, j, k
   double* red = E + kTripLdsE;                // kTripMfmaThreads / 64
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
   constexpr int nwave = kTripMfmaThreads / 64;
@@ -173,42 +177,47 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
     if (v > kTripLdsV) continue;   // uniform over the workgroup: the global-W kernel takes it
     const int packed = info.y;
     const int occ[3] = {packed & 0xff, (packed >> 8) & 0xff, (packed >> 16) & 0xff};
-    const int vv = v * v, rt_n = (vv + 15) >> 4, ct_n = (v + 15) >> 4;
+    const int vv = v * v, rt_n = (vv + 15) >> 4, ct_n = (2 * v + 15) >> 4;
     const int S1 = v + 1, S2 = v * S1 + 1;
     for (int e = threadIdx.x; e < v * S2; e += kTripMfmaThreads) W[e] = 0.0;
     const int i = occ[0], j = occ[1], k = occ[2];
     // operands of the next phase, loaded into registers while the current
-    // phase runs: the B operands of pair h (2 elements of each 32 x 32 array
-    // per thread), then the L slices, virtual orbital energies and t1 rows
-    constexpr int kStg = 32 * 32 / kTripMfmaThreads;   // elements per 32 x 32 array and thread
-    static_assert(kStg * kTripMfmaThreads == 32 * 32, "staging covers each array exactly");
-    static_assert(3 * kStg * kTripMfmaThreads >= 3 * kTripLdsV * kTripLdsV, "staging covers the L slices");
+    // phase runs: the B operands of pair h (kStg elements of each 32 x kTripBC
+    // array per thread), then the L slices, virtual orbital energies and t1 rows
+    constexpr int kStg = 32 * kTripBC / kTripMfmaThreads;   // elements per B array and thread
+    constexpr int kStgL = (3 * kTripLdsV * kTripLdsV + kTripMfmaThreads - 1) / kTripMfmaThreads;
+    static_assert(kStg * kTripMfmaThreads == 32 * kTripBC, "staging covers each B array exactly");
+    static_assert(kStgL + 1 <= 2 * kStg, "the staging registers hold the energy-pass data");
+    static_assert(3 * kTripLdsV * kTripLdsV <= 2 * 32 * kTripBC, "the B region holds the L slices");
     static_assert(kTripMfmaThreads >= 4 * kTripLdsV, "one energy-pass vector element per thread");
-    double stg[4 * kStg];
+    double stg[2 * kStg];
     auto fetch_b = [&](int h) {
       const int q = occ[kPerm[2 * h][1]], r = occ[kPerm[2 * h][2]];
 #pragma unroll
       for (int u = 0; u < kStg; ++u) {
-        const int e = threadIdx.x + u * kTripMfmaThreads, row = e >> 5, z = e & 31;
-        const bool dz = row < v && z < v, lz = row < o && z < v;
-        stg[u] = dz ? M.t2[(((long)r * o + q) * v + z) * v + row] : 0.0;                // t_rq^{z d}, d = row
-        stg[kStg + u] = dz ? M.t2[(((long)q * o + r) * v + z) * v + row] : 0.0;         // t_qr^{z d}
-        stg[2 * kStg + u] = lz ? M.B[(((long)q * o + row) * o + r) * v + z] : 0.0;      // (ql|rz), l = row
-        stg[3 * kStg + u] = lz ? M.B[(((long)r * o + row) * o + q) * v + z] : 0.0;      // (rl|qz)
+        const int e = threadIdx.x + u * kTripMfmaThreads, row = e / kTripBC, c = e - row * kTripBC;
+        const bool first = c < v, second = !first && c < 2 * v;
+        const int z = first ? c : c - v;
+        const bool dz = row < v && (first || second), lz = row < o && (first || second);
+        // first: t_rq^{z d} and (ql|rz); second: t_qr^{z d} and (rl|qz); d, l = row
+        const long tt = first ? (((long)r * o + q) * v + z) * v + row : (((long)q * o + r) * v + z) * v + row;
+        const long bb = first ? (((long)q * o + row) * o + r) * v + z : (((long)r * o + row) * o + q) * v + z;
+        stg[u] = dz ? M.t2[tt] : 0.0;
+        stg[kStg + u] = lz ? M.B[bb] : 0.0;
       }
     };
-    // energy-pass data: element e of [Ljk | Lik | Lij] (3 v^2 <= 4 kStg x threads)
+    // energy-pass data: element e of [Ljk | Lik | Lij] (3 v^2 <= kStgL x threads)
     // and of [e_vir | t1_i | t1_j | t1_k] (4 v <= threads)
     auto fetch_e = [&]() {
 #pragma unroll
-      for (int u = 0; u < 3 * kStg; ++u) {
+      for (int u = 0; u < kStgL; ++u) {
         const int e = threadIdx.x + u * kTripMfmaThreads, w = e / vv, f = e - w * vv;
         const long pair = w == 0 ? (long)j * o + k : (w == 1 ? (long)i * o + k : (long)i * o + j);
         stg[u] = w < 3 ? M.L[pair * vv + f] : 0.0;
       }
       const int w = threadIdx.x / v, f = threadIdx.x - w * v;
       const int row = w == 1 ? i : (w == 2 ? j : k);
-      stg[3 * kStg] = w == 0 ? M.F[(long)(o + f) * n + o + f] : (w < 4 ? M.t1[(long)row * v + f] : 0.0);
+      stg[kStgL] = w == 0 ? M.F[(long)(o + f) * n + o + f] : (w < 4 ? M.t1[(long)row * v + f] : 0.0);
     };
     fetch_b(0);
     // permutations 2h and 2h + 1 share p (hence the A fragments) and swap q, r;
@@ -224,9 +233,7 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
       for (int u = 0; u < kStg; ++u) {
         const int e = threadIdx.x + u * kTripMfmaThreads;
         Tb[e] = stg[u];
-        Tb2[e] = stg[kStg + u];
-        Qb[e] = stg[2 * kStg + u];
-        Qb2[e] = stg[3 * kStg + u];
+        Qb[e] = stg[kStg + u];
       }
       __syncthreads();
       if (h < 2) fetch_b(h + 1);
@@ -234,7 +241,8 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
       const double* Ap = M.A + (long)p * vv * v;     // [(x,y)][d]
       const double* Up = M.t2 + (long)p * o * vv;    // [l][(x,y)]
       // A fragments of the next tile are prefetched; the t2 rows (contiguous
-      // along (x, y)) are loaded at the tile
+      // along (x, y)) are loaded at the tile (prefetching them too measured
+      // slower)
       double fa[kTripKD], fu[kTripKL], na[kTripKD];
       auto fetch = [&](int rt) {
         const int row = rt * 16 + l16;
@@ -252,11 +260,28 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
       for (int t = 0; t < tiles_per_wave; ++t) {
         const int rt = wave + t * nwave;
         const bool has = rt < rt_n;
-        trip_d4 acc[2], acc2[2];
+        // column tiles of the combined B operand: columns c < v belong to
+        // permutation 2h (z = c), v <= c < 2v to permutation 2h + 1 (z = c - v)
+        constexpr int kCt = kTripBC / 16;
+        trip_d4 acc[kCt];
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          acc[ct] = trip_d4{0.0, 0.0, 0.0, 0.0};
-          acc2[ct] = trip_d4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < kCt; ++ct) acc[ct] = trip_d4{0.0, 0.0, 0.0, 0.0};
+        // (x, y) of the lane's four output rows rr = 16 rt + kq + 4 e, from one
+        // division and carries (x < v exactly when rr < v^2)
+        int rx[4], ry[4];
+        {
+          const int base = rt * 16 + kq;
+          int x = base / v, y = base - x * v;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            rx[e] = x;
+            ry[e] = y;
+            y += 4;
+            while (y >= v) {
+              y -= v;
+              ++x;
+            }
+          }
         }
         if (has) {
 #pragma unroll
@@ -272,10 +297,9 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
             if (4 * k >= v) break;
             const int d = 4 * k + kq;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
+            for (int ct = 0; ct < kCt; ++ct) {
               if (ct >= ct_n) break;
-              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * 32 + 16 * ct + l16], acc[ct], 0, 0, 0);
-              acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb2[d * 32 + 16 * ct + l16], acc2[ct], 0, 0, 0);
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[k], Tb[d * kTripBC + 16 * ct + l16], acc[ct], 0, 0, 0);
             }
           }
 #pragma unroll
@@ -283,36 +307,31 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
             if (4 * k >= o) break;
             const int l = 4 * k + kq;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
+            for (int ct = 0; ct < kCt; ++ct) {
               if (ct >= ct_n) break;
-              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * 32 + 16 * ct + l16], acc[ct], 0, 0, 0);
-              acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb2[l * 32 + 16 * ct + l16], acc2[ct], 0, 0, 0);
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[k], Qb[l * kTripBC + 16 * ct + l16], acc[ct], 0, 0, 0);
             }
           }
-          // permutation 2h: rows (x, y), columns z at x sx + y sy + z sz
+          // permutation 2h: rows (x, y), columns z < v at x sx + y sy + z sz
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < kCt; ++ct) {
             const int z = 16 * ct + l16;
             if (ct >= ct_n || z >= v) continue;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int rr = rt * 16 + kq + 4 * e;
-              if (rr < vv) W[(rr / v) * sx + (rr % v) * sy + z * sz] += acc[ct][e];
-            }
+            for (int e = 0; e < 4; ++e)
+              if (rx[e] < v) W[rx[e] * sx + ry[e] * sy + z * sz] += acc[ct][e];
           }
         }
         __syncthreads();
-        // permutation 2h + 1: the same tiles at x sx + y sz + z sy
+        // permutation 2h + 1: columns v + z of the same tiles at x sx + y sz + z sy
         if (has) {
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
-            const int z = 16 * ct + l16;
-            if (ct >= ct_n || z >= v) continue;
+          for (int ct = 0; ct < kCt; ++ct) {
+            const int z = 16 * ct + l16 - v;
+            if (ct >= ct_n || z < 0 || z >= v) continue;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int rr = rt * 16 + kq + 4 * e;
-              if (rr < vv) W[(rr / v) * sx + (rr % v) * sz + z * sy] += acc2[ct][e];
-            }
+            for (int e = 0; e < 4; ++e)
+              if (rx[e] < v) W[rx[e] * sx + ry[e] * sz + z * sy] += acc[ct][e];
           }
         }
         __syncthreads();
@@ -329,16 +348,25 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
     double* t1j = t1i + v;
     double* t1k = t1j + v;
 #pragma unroll
-    for (int u = 0; u < 3 * kStg; ++u) {
+    for (int u = 0; u < kStgL; ++u) {
       const int e = threadIdx.x + u * kTripMfmaThreads;
       if (e < 3 * vv) Tb[e] = stg[u];
     }
-    if ((int)threadIdx.x < 4 * v) E[threadIdx.x] = stg[3 * kStg];
+    if ((int)threadIdx.x < 4 * v) E[threadIdx.x] = stg[kStgL];
     __syncthreads();
     const double eijk = M.F[(long)i * n + i] + M.F[(long)j * n + j] + M.F[(long)k * n + k];
     double eT = 0.0, eST = 0.0;
+    // element e = (a v + b) v + c; the thread's elements step by the block
+    // size, whose (a, b, c) step is added with carries instead of dividing
+    const int st_a = kTripMfmaThreads / vv, st_r = kTripMfmaThreads - st_a * vv, st_b = st_r / v,
+              st_c = st_r - st_b * v;
+    int a = (int)threadIdx.x / vv, b, c;
+    {
+      const int r = (int)threadIdx.x - a * vv;
+      b = r / v;
+      c = r - b * v;
+    }
     for (int e = threadIdx.x; e < vv * v; e += kTripMfmaThreads) {
-      const int a = e / vv, b = (e / v) % v, c = e % v;
       const double Wabc = W[a * S2 + b * S1 + c];
       const double r3 = 4.0 * Wabc + W[b * S2 + c * S1 + a] + W[c * S2 + a * S1 + b] -
                         2.0 * (W[a * S2 + c * S1 + b] + W[b * S2 + a * S1 + c] + W[c * S2 + b * S1 + a]);
@@ -346,6 +374,17 @@ __global__ __launch_bounds__(kTripMfmaThreads) void k_triples_mfma(const TripMol
       const double V = t1i[a] * Ljk[b * v + c] + t1j[b] * Lik[a * v + c] + t1k[c] * Lij[a * v + b];
       eT += Wabc * g;
       eST += V * g;
+      c += st_c;
+      b += st_b;
+      a += st_a;
+      if (c >= v) {
+        c -= v;
+        ++b;
+      }
+      if (b >= v) {
+        b -= v;
+        ++a;
+      }
     }
     // block sums (fixed order)
     for (int off = 32; off > 0; off >>= 1) {
@@ -386,7 +425,7 @@ __global__ void k_triples_final(int nm, const long* __restrict__ prefix, const d
 
 size_t triples_lds_bytes() { return sizeof(double) * (kTripThreads / 64 + 2 * kTripMaxV * kTripMaxV); }
 size_t triples_mfma_lds_bytes() {
-  return sizeof(double) * (kTripLdsW + 4 * 32 * 32 + kTripLdsE + 2 * (kTripMfmaThreads / 64));
+  return sizeof(double) * (kTripLdsW + 2 * 32 * kTripBC + kTripLdsE + 2 * (kTripMfmaThreads / 64));
 }
 
 hipError_t launch_triples(const TripMol* mols, int nm, const long* prefix, long total, const int2* items,
