@@ -198,3 +198,25 @@ def test_vector_space_operations():
     with pytest.raises(capi.HccError):
         ctx.lr_sigma_slots(np.array([0, -1, 0]), np.array([0, 0, 1]), 1e-5)   # in == out
     ctx.close()
+
+
+def test_device_davidson_equals_oracle_driven_at_n30():
+    """Beyond the sizes whose dense Jacobian can be formed: for a 30-site
+    batch member (25650 coordinates) the device Davidson (vectors in device
+    memory, sigma from the GPU residual) and the same algorithm driven by the
+    oracle (numpy vectors, the oracle's residual) reach the same three roots,
+    average excitation levels and reference weights."""
+    from tests.test_lrccsd_oracle import _OracleCtx
+    mol = M.batch_member(13)
+    p = prepare(mol)
+    o, v = mol.nocc, mol.n - mol.nocc
+    ctx = _solved_ctx([p])
+    om, res, its, conv, ael, r0 = lrccsd.excitation_energies(ctx, [(o, v)], nroots=3, gammas=[p["G"]])
+    ctx.close()
+    ref = ccsd.solve(p["C"], p["F"], p["G"], o, p["eref"], eps=1e-11, diis_scale=True)
+    om2, res2, its2, conv2, ael2, r02 = lrccsd.excitation_energies(
+        _OracleCtx([(p, o, ref["t1"], ref["t2"])]), [(o, v)], nroots=3, gammas=[p["G"]])
+    print("gpu", om[0], its[0], "oracle-driven", om2[0], its2[0], "AEL", ael[0], ael2[0], "r0", r0[0], r02[0])
+    assert conv[0] and conv2[0]
+    assert np.abs(om[0] - om2[0]).max() < 2e-6
+    assert np.abs(ael[0] - ael2[0]).max() < 1e-4 and np.abs(r0[0] - r02[0]).max() < 1e-4
