@@ -139,7 +139,8 @@ def lr_leg(mols, nroots=3):
         return run
 
     ctx.lr_sigma_slots = timed(ctx.lr_sigma_slots, "sigma")
-    for name in ("lr_dots", "lr_combine", "lr_precondition", "lr_put", "lr_get", "lr_space"):
+    for name in ("lr_dots", "lr_combine", "lr_gram", "lr_combine_block", "lr_precondition", "lr_put", "lr_get",
+                 "lr_space"):
         setattr(ctx, name, timed(getattr(ctx, name), "space"))
     t1 = time.perf_counter()
     om, res, its, conv, ael, r0 = lrccsd.excitation_energies(
@@ -261,6 +262,8 @@ def main():
     ap.add_argument("--max-outer", type=int, default=30)
     ap.add_argument("--no-profile", action="store_true", help="skip the profiled solves (kernel attribution)")
     ap.add_argument("--no-lr", action="store_true", help="skip the LR-CCSD leg (64-molecule sample)")
+    ap.add_argument("--lr-only", action="store_true",
+                    help="run only the LR-CCSD leg on the first 64 batch molecules and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (HBM counter calibration for profiler passes)")
     args = ap.parse_args()
@@ -271,6 +274,10 @@ def main():
         print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark runs on one MI355X; "
               "across GPUs the batch would shard by molecule", file=sys.stderr)
         sys.exit(2)
+
+    if args.lr_only:
+        print(json.dumps({"lr_ccsd_sample": lr_leg(prepare_batch(64, args.workers))}), flush=True)
+        return
 
     t0 = time.time()
     mols = prepare_batch(args.nmol, args.workers)

@@ -56,6 +56,8 @@ SIGNATURES = {
     "hcc_lr_get": (_i, [_vp, _i, _i, _P(_d)]),
     "hcc_lr_dots": (_i, [_vp, _i, _P(_i), _i, _P(_d)]),
     "hcc_lr_combine": (_i, [_vp, _i, _P(_i), _P(_d), _P(_i), _P(_d)]),
+    "hcc_lr_gram": (_i, [_vp, _i, _P(_i), _i, _P(_i), _P(_d)]),
+    "hcc_lr_combine_block": (_i, [_vp, _i, _P(_i), _P(_d), _i, _P(_i), _i, _P(_d)]),
     "hcc_lr_precondition": (_i, [_vp, _i, _P(_i), _P(_d), _d]),
     "hcc_lr_sigma_slots": (_i, [_vp, _d, _P(_i), _P(_i)]),
     "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
@@ -325,6 +327,29 @@ class Context:
         tc = np.ascontiguousarray(term_coef if len(term_coef) else [0.0], dtype=np.float64)
         self._check(self.lib.hcc_lr_combine(self.h, len(jobs), _ip(jobs), _dp(beta), _ip(ts), _dp(tc)),
                     "hcc_lr_combine")
+
+    def lr_gram(self, jobs, slots):
+        """jobs: (J, 5) ints (mol, na, nb, a, b) over the slot list ``slots``
+        -> the jobs' na x nb Gram blocks, row-major, concatenated."""
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 5)
+        slots = np.ascontiguousarray(slots if len(slots) else [0], dtype=np.int32)
+        out = np.zeros(int(np.sum(jobs[:, 1].astype(np.int64) * jobs[:, 2])) if len(jobs) else 0)
+        if len(jobs):
+            self._check(self.lib.hcc_lr_gram(self.h, len(jobs), _ip(jobs), len(slots), _ip(slots), _dp(out)),
+                        "hcc_lr_gram")
+        return out
+
+    def lr_combine_block(self, jobs, beta, slots, coef):
+        """jobs: (J, 6) ints (mol, m, t, d, s, c): slots[d + p] = beta slots[d + p]
+        + sum_q coef[c + p t + q] slots[s + q]."""
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 6)
+        if not len(jobs):
+            return
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        slots = np.ascontiguousarray(slots if len(slots) else [0], dtype=np.int32)
+        coef = np.ascontiguousarray(coef if len(coef) else [0.0], dtype=np.float64)
+        self._check(self.lib.hcc_lr_combine_block(self.h, len(jobs), _ip(jobs), _dp(beta), len(slots), _ip(slots),
+                                                  len(coef), _dp(coef)), "hcc_lr_combine_block")
 
     def lr_precondition(self, jobs, shift, floor):
         jobs = np.ascontiguousarray(jobs, dtype=np.int32).reshape(-1, 2)

@@ -1757,22 +1757,28 @@ int hcc_lr_dots(hcc_ctx* ctx, int njobs, const int* jobs, int mode, double* out)
     return HCC_E_ARG;
   if (int r = lr_space_ready(ctx)) return r;
   Batch& b = *ctx->batch;
+  std::vector<int> prefix(njobs + 1, 0);
   for (int j = 0; j < njobs; ++j) {
     const int mol = jobs[3 * j], sa = jobs[3 * j + 1], sb = jobs[3 * j + 2];
     if (!lr_slot_ok(b, mol, sa) || !(lr_slot_ok(b, mol, sb) || (mode == 1 && sb == -1))) {
       ctx->err = "dot job " + std::to_string(j) + ": molecule or slot out of range";
       return HCC_E_ARG;
     }
+    prefix[j + 1] = prefix[j] + (int)((b.lr_len[mol] + kLrDotChunk - 1) / kLrDotChunk);
   }
   if (njobs == 0) return HCC_OK;
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const size_t jb = lr_align(3 * sizeof(int) * (size_t)njobs);
-  char* st = lr_stage(b, jb + sizeof(double) * (size_t)njobs);
-  HCHECK(hipMemcpyAsync(st, jobs, 3 * sizeof(int) * (size_t)njobs, hipMemcpyHostToDevice, s));
-  double* dout = reinterpret_cast<double*>(st + jb);
-  lr_dots(b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st), mode, dout, s);
+  const int nchunks = prefix[njobs];
+  const size_t o_pre = 0, o_job = lr_align(sizeof(int) * (njobs + 1)), o_out = o_job + lr_align(3 * sizeof(int) * njobs),
+               o_part = o_out + lr_align(sizeof(double) * njobs), total = o_part + sizeof(double) * (nchunks + 1);
+  char* st = lr_stage(b, total);
+  HCHECK(hipMemcpyAsync(st + o_pre, prefix.data(), sizeof(int) * (njobs + 1), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_job, jobs, 3 * sizeof(int) * (size_t)njobs, hipMemcpyHostToDevice, s));
+  double* dout = reinterpret_cast<double*>(st + o_out);
+  lr_dots(b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st + o_pre),
+          reinterpret_cast<int*>(st + o_job), mode, reinterpret_cast<double*>(st + o_part), nchunks, dout, s);
   HCHECK(hipGetLastError());
   HCHECK(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)njobs, hipMemcpyDeviceToHost, s));
   HCHECK(hipStreamSynchronize(s));
@@ -1848,6 +1854,117 @@ int hcc_lr_precondition(hcc_ctx* ctx, int njobs, const int* jobs, const double* 
   HCHECK(hipMemcpyAsync(st + o_sh, shift, sizeof(double) * njobs, hipMemcpyHostToDevice, s));
   lr_precondition(b.dm, b.dlr_space, b.dlr_base, b.dlr_len, njobs, reinterpret_cast<int*>(st),
                   reinterpret_cast<int2*>(st + o_job), reinterpret_cast<double*>(st + o_sh), floor, prefix[njobs], s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+// slots[first .. first + count) lie in the list and name slots of `mol`
+static bool lr_slot_run_ok(const Batch& b, int mol, int nslot, const int* slots, int first, int count) {
+  if (first < 0 || count < 0 || (long)first + count > nslot) return false;
+  for (int q = first; q < first + count; ++q)
+    if (!lr_slot_ok(b, mol, slots[q])) return false;
+  return true;
+}
+
+int hcc_lr_gram(hcc_ctx* ctx, int njobs, const int* jobs, int nslot, const int* slots, double* out) {
+  if (ctx == nullptr || njobs < 0 || nslot < 0 || (njobs > 0 && (jobs == nullptr || out == nullptr)) ||
+      (nslot > 0 && slots == nullptr))
+    return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  std::vector<LrGramBlk> blk;
+  std::vector<int> prefix(1, 0);
+  long at = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const int mol = jobs[5 * j], na = jobs[5 * j + 1], nb = jobs[5 * j + 2], a = jobs[5 * j + 3], c = jobs[5 * j + 4];
+    if (mol < 0 || mol >= b.nm || !lr_slot_run_ok(b, mol, nslot, slots, a, na) ||
+        !lr_slot_run_ok(b, mol, nslot, slots, c, nb)) {
+      ctx->err = "gram job " + std::to_string(j) + ": molecule, slot or slot-list range invalid";
+      return HCC_E_ARG;
+    }
+    const int chunks = (int)((b.lr_len[mol] + kLrGramChunk - 1) / kLrGramChunk);
+    for (int p = 0; p < na; p += 16)
+      for (int q = 0; q < nb; q += 64) {
+        blk.push_back(LrGramBlk{mol, std::min(16, na - p), std::min(64, nb - q), a + p, c + q, (int)(at + (long)p * nb + q),
+                                nb, 0});
+        prefix.push_back(prefix.back() + chunks);
+      }
+    at += (long)na * nb;
+    if (at > (1L << 30)) {
+      ctx->err = "gram: output too large";
+      return HCC_E_ARG;
+    }
+  }
+  if (blk.empty()) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int nblk = (int)blk.size(), nchunks = prefix[nblk];
+  const size_t o_blk = lr_align(sizeof(int) * (nblk + 1)), o_sl = o_blk + lr_align(sizeof(LrGramBlk) * nblk),
+               o_out = o_sl + lr_align(sizeof(int) * (nslot + 1)), o_part = o_out + lr_align(sizeof(double) * at),
+               total = o_part + sizeof(double) * 1024 * (size_t)nchunks;
+  char* st = lr_stage(b, total);
+  HCHECK(hipMemcpyAsync(st, prefix.data(), sizeof(int) * (nblk + 1), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_blk, blk.data(), sizeof(LrGramBlk) * nblk, hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_sl, slots, sizeof(int) * nslot, hipMemcpyHostToDevice, s));
+  double* dout = reinterpret_cast<double*>(st + o_out);
+  lr_gram(b.dlr_space, b.dlr_base, b.dlr_len, nblk, reinterpret_cast<int*>(st), reinterpret_cast<LrGramBlk*>(st + o_blk),
+          reinterpret_cast<int*>(st + o_sl), reinterpret_cast<double*>(st + o_part), nchunks, dout, s);
+  HCHECK(hipGetLastError());
+  HCHECK(hipMemcpyAsync(out, dout, sizeof(double) * at, hipMemcpyDeviceToHost, s));
+  HCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+int hcc_lr_combine_block(hcc_ctx* ctx, int njobs, const int* jobs, const double* beta, int nslot, const int* slots,
+                         int ncoef, const double* coef) {
+  if (ctx == nullptr || njobs < 0 || nslot < 0 || ncoef < 0 || (njobs > 0 && (jobs == nullptr || beta == nullptr)) ||
+      (nslot > 0 && slots == nullptr) || (ncoef > 0 && coef == nullptr))
+    return HCC_E_ARG;
+  if (int r = lr_space_ready(ctx)) return r;
+  Batch& b = *ctx->batch;
+  std::vector<LrCombBlk> blk;
+  std::vector<int> prefix(1, 0);
+  for (int j = 0; j < njobs; ++j) {
+    const int mol = jobs[6 * j], m = jobs[6 * j + 1], t = jobs[6 * j + 2], d = jobs[6 * j + 3], sr = jobs[6 * j + 4],
+              c = jobs[6 * j + 5];
+    bool ok = mol >= 0 && mol < b.nm && lr_slot_run_ok(b, mol, nslot, slots, d, m) &&
+              lr_slot_run_ok(b, mol, nslot, slots, sr, t) && c >= 0 && (long)c + (long)m * t <= ncoef;
+    // distinct destinations, none among the terms (each element has one writer
+    // and no term is read after it was written)
+    for (int p = 0; ok && p < m; ++p) {
+      for (int q = p + 1; ok && q < m; ++q) ok = slots[d + p] != slots[d + q];
+      for (int q = 0; ok && q < t; ++q) ok = slots[d + p] != slots[sr + q];
+    }
+    if (!ok) {
+      ctx->err = "combine-block job " + std::to_string(j) +
+                 ": invalid molecule, slot, slot-list or coefficient range, repeated destination, or destination among "
+                 "the terms";
+      return HCC_E_ARG;
+    }
+    const int chunks = (int)((b.lr_len[mol] + kLrCombChunk - 1) / kLrCombChunk);
+    for (int p = 0; p < m; p += 16) {
+      LrCombBlk x{mol, std::min(16, m - p), t, d + p, sr, c + p * t, 0, 0, beta[j]};
+      blk.push_back(x);
+      prefix.push_back(prefix.back() + chunks);
+    }
+  }
+  if (blk.empty()) return HCC_OK;
+  API_BEGIN
+  HCHECK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int nblk = (int)blk.size();
+  const size_t o_blk = lr_align(sizeof(int) * (nblk + 1)), o_sl = o_blk + lr_align(sizeof(LrCombBlk) * nblk),
+               o_cf = o_sl + lr_align(sizeof(int) * (nslot + 1)), total = o_cf + sizeof(double) * (ncoef + 1);
+  char* st = lr_stage(b, total);
+  HCHECK(hipMemcpyAsync(st, prefix.data(), sizeof(int) * (nblk + 1), hipMemcpyHostToDevice, s));
+  HCHECK(hipMemcpyAsync(st + o_blk, blk.data(), sizeof(LrCombBlk) * nblk, hipMemcpyHostToDevice, s));
+  if (nslot > 0) HCHECK(hipMemcpyAsync(st + o_sl, slots, sizeof(int) * nslot, hipMemcpyHostToDevice, s));
+  if (ncoef > 0) HCHECK(hipMemcpyAsync(st + o_cf, coef, sizeof(double) * ncoef, hipMemcpyHostToDevice, s));
+  lr_combine_block(b.dlr_space, b.dlr_base, b.dlr_len, nblk, reinterpret_cast<int*>(st),
+                   reinterpret_cast<LrCombBlk*>(st + o_blk), reinterpret_cast<int*>(st + o_sl),
+                   reinterpret_cast<double*>(st + o_cf), prefix[nblk], s);
   HCHECK(hipGetLastError());
   HCHECK(hipStreamSynchronize(s));
   API_END

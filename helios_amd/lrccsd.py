@@ -30,13 +30,13 @@ are not provided.
 import numpy as np
 
 
-def cis_singlets(C, gamma, e, o):
+def cis_singlets(C, gamma, e, o, count=None):
     """Singlet CIS of the RHF reference: eigenvalues (ascending) and
     eigenvectors (columns over (i, a), i slowest) of
     A[(i a), (j b)] = (e_a - e_i) d_ij d_ab + 2 [ai|jb] - [ab|ji],
     [pq|rs] = sum_mn C_mp C_mq gamma_mn C_nr C_ns -- the matrix the reference
     diagonalises for its 'cis' guess (mod_lrccsd.f90:1253-1282 with
-    spat_hf_int, :1423-1445)."""
+    spat_hf_int, :1423-1445).  With ``count``, the lowest ``count`` pairs."""
     n = C.shape[0]
     v = n - o
     Co, Cv = C[:, :o], C[:, o:]
@@ -47,7 +47,8 @@ def cis_singlets(C, gamma, e, o):
     exch = (Y.T @ gamma @ Z).reshape(v, v, o, o).transpose(2, 0, 3, 1).reshape(o * v, o * v)   # [ab|ij]
     A = 2.0 * coul - exch
     A[np.diag_indices(o * v)] += (e[o:][None, :] - e[:o][:, None]).ravel()
-    return np.linalg.eigh(A)
+    w, U = np.linalg.eigh(A)
+    return (w, U) if count is None else (w[:count], U[:, :count])
 
 
 def average_excitation_level(r1, r2):
@@ -98,19 +99,19 @@ def reference_weight(g, f_ov, t1, omega, r1, r2):
 
 
 class _Layout:
-    """Slots of every molecule's device vectors: basis V and sigmas S (cap
-    each), two buffers of Ritz vectors with their sigmas (the current step's
-    and the last step's, R rows each), and the candidates for new basis
-    vectors (corrections, start vectors, seeds)."""
+    """Slots of every molecule's device vectors: two regions of basis V and
+    sigmas S (cap each; a restart writes the new basis into the other region),
+    the current Ritz vectors with their sigmas (R rows each), and the
+    candidates for new basis vectors (corrections, start vectors, seeds)."""
 
     def __init__(self, nroots, max_space):
         self.nwork = nroots + 4
         self.R = 2 * self.nwork            # a real Ritz vector takes one row, a complex pair two
         self.max_space = max(max_space, 6 * self.nwork)
         self.cap = self.max_space + self.R + 8
-        self.V, self.S = 0, self.cap
-        self.XB = [2 * self.cap, 2 * self.cap + 2 * self.R]   # X rows at XB[b], SX rows at XB[b] + R
-        self.C = 2 * self.cap + 4 * self.R
+        self.VS = [(0, self.cap), (2 * self.cap, 3 * self.cap)]
+        self.XB = 4 * self.cap             # X rows at XB, SX rows at XB + R
+        self.C = self.XB + 2 * self.R
         self.nC = self.R + self.nwork + 6
         self.nslots = self.C + self.nC
 
@@ -137,14 +138,22 @@ class _Sub:
         self.k = 0            # basis vectors with sigmas: V[0:k], S[0:k]
         self.pending = 0      # basis vectors V[k:k+pending] awaiting their sigmas
         self.G = np.zeros((0, 0))   # G[a, b] = <V_a, S_b>
-        self.cur = 0          # Ritz buffer written by the next step
-        self.prev_rows = 0    # rows of the last step's Ritz buffer (1 - cur)
+        self.reg = 0          # region of L.VS holding V and S
+        self.Ycur = np.zeros((0, 0))    # this step's Ritz vectors, basis coordinates (k x rows)
+        self.Yprev = np.zeros((0, 0))   # the last step's, in the current basis (zero-padded)
+        self.rows = 0
         self.seeds = None
         self.omega = np.full(nroots, np.nan)
         self.res = np.full(nroots, np.inf)
         self.root_slot = [None] * nroots
         self.done = False
         self.working = 0      # Ritz values in the working set at the last step
+
+    def V(self, L):
+        return L.VS[self.reg][0]
+
+    def S(self, L):
+        return L.VS[self.reg][1]
 
     def unit(self, e1, e2=-1):
         x = np.zeros(self.N)
@@ -160,8 +169,14 @@ class _Sub:
             return []
         m, n = np.triu_indices(n1)
         d = self._d2[m // v, n // v, m % v, n % v]
+        # the first ``count`` of a stable sort, without sorting all pairs
+        if count < len(d):
+            cand = np.flatnonzero(d <= np.partition(d, count - 1)[count - 1])
+            first = cand[np.argsort(d[cand], kind="stable")][:count]
+        else:
+            first = np.argsort(d, kind="stable")
         out = []
-        for q in np.argsort(d, kind="stable")[:count]:
+        for q in first:
             i, a, j, b = m[q] // v, m[q] % v, n[q] // v, n[q] % v
             out.append((n1 + ((i * o + j) * v + a) * v + b, n1 + ((j * o + i) * v + b) * v + a))
         return out
@@ -212,47 +227,63 @@ class _Ops:
 
     def combine_blocks(self, blocks):
         """blocks: list of (mol, dst slots [m], beta, term slots [t], coefficients [m, t]):
-        dst_q = beta dst_q + sum_j coef[q, j] slot_j, built as arrays."""
-        heads, betas, tss, tcs = [], [], [], []
-        at = 0
+        dst_q = beta dst_q + sum_j coef[q, j] slot_j, each term read once per 16 rows."""
+        heads, betas, sls, cfs = [], [], [], []
+        ns = nc = 0
         for mol, dst, beta, slots, coef in blocks:
             m, t = coef.shape
             if m == 0:
                 continue
-            h = np.empty((m, 4), dtype=np.int32)
-            h[:, 0] = mol
-            h[:, 1] = dst
-            h[:, 2] = at + t * np.arange(m)
-            h[:, 3] = t
-            heads.append(h)
-            betas.append(np.full(m, beta))
-            tss.append(np.tile(np.asarray(slots, dtype=np.int32), m))
-            tcs.append(coef.ravel())
-            at += m * t
+            heads.append((mol, m, t, ns, ns + m, nc))
+            betas.append(beta)
+            sls.append(np.asarray(dst, dtype=np.int32))
+            sls.append(np.asarray(slots, dtype=np.int32))
+            cfs.append(np.asarray(coef, dtype=np.float64).ravel())
+            ns += m + t
+            nc += m * t
         if heads:
-            self.ctx.lr_combine(np.concatenate(heads), np.concatenate(betas), np.concatenate(tss),
-                                np.concatenate(tcs))
+            self.ctx.lr_combine_block(np.array(heads, dtype=np.int32), np.array(betas), np.concatenate(sls),
+                                      np.concatenate(cfs))
+
+    def gram(self, blocks):
+        """blocks: list of (mol, slots a [na], slots b [nb]) -> list of the
+        na x nb matrices <a_p, b_q>, each vector read once per 16 x 64 piece."""
+        heads, sls = [], []
+        ns = 0
+        for mol, a, b in blocks:
+            heads.append((mol, len(a), len(b), ns, ns + len(a)))
+            sls.append(np.asarray(a, dtype=np.int32))
+            sls.append(np.asarray(b, dtype=np.int32))
+            ns += len(a) + len(b)
+        if not heads:
+            return []
+        g = self.ctx.lr_gram(np.array(heads, dtype=np.int32), np.concatenate(sls))
+        out, at = [], 0
+        for _, na, nb, _, _ in heads:
+            out.append(g[at:at + na * nb].reshape(na, nb))
+            at += na * nb
+        return out
 
 
 def _accept(M, n0, thresh, room):
     """Gram-Schmidt in the coordinates of the candidates, from their Gram
     matrix M: rows of T give the accepted orthonormal vectors as combinations
     of the candidates, in order; a candidate whose part orthogonal to the
-    accepted ones is below ``thresh`` of its original norm n0 is dropped (each
-    projection applied twice)."""
-    T = []
-    for r in range(len(M)):
+    accepted ones is below ``thresh`` of its original norm n0 is dropped (the
+    projection onto the accepted ones applied twice)."""
+    m = len(M)
+    T = np.zeros((0, m))
+    for r in range(m):
         if len(T) >= room:
             break
-        c = np.zeros(len(M))
+        c = np.zeros(m)
         c[r] = 1.0
         for _ in range(2):
-            for t in T:
-                c = c - (t @ M @ c) * t
+            c -= T.T @ (T @ (M @ c))
         nrm = np.sqrt(max(float(c @ M @ c), 0.0))
         if nrm > thresh * n0[r]:
-            T.append(c / nrm)
-    return np.array(T).reshape(-1, len(M))
+            T = np.vstack([T, c / nrm])
+    return T
 
 
 def _orthonormalize(ops, L, subs, cands, thresh):
@@ -266,155 +297,100 @@ def _orthonormalize(ops, L, subs, cands, thresh):
     if not act:
         return
     for _ in range(2):
-        jobs, spans = [], []
-        for k in act:
-            s = subs[k]
-            nb = s.k + s.pending
-            cs = np.array([sl for sl, _ in cands[k]], dtype=np.int32)
-            if nb == 0:
-                continue
-            J = np.empty((len(cs), nb, 3), dtype=np.int32)
-            J[:, :, 0] = k
-            J[:, :, 1] = cs[:, None]
-            J[:, :, 2] = L.V + np.arange(nb)[None, :]
-            spans.append((k, sum(len(j) for j in jobs), cs, nb))
-            jobs.append(J.reshape(-1, 3))
-        if not jobs:
+        gb = [(k, [sl for sl, _ in cands[k]], subs[k].V(L) + np.arange(subs[k].k + subs[k].pending))
+              for k in act if subs[k].k + subs[k].pending > 0]
+        if not gb:
             break
-        P = ops.dots(np.concatenate(jobs))
-        blocks = []
-        for k, at, cs, nb in spans:
-            coef = -P[at:at + len(cs) * nb].reshape(len(cs), nb)
-            for q, sl in enumerate(cs):
-                blocks.append((k, [sl], 1.0, L.V + np.arange(nb), coef[q:q + 1]))
-        ops.combine_blocks(blocks)
-    jobs, spans = [], []
-    for k in act:
-        cs = [sl for sl, _ in cands[k]]
-        m = len(cs)
-        r, c = np.triu_indices(m)
-        J = np.empty((len(r), 3), dtype=np.int32)
-        J[:, 0] = k
-        J[:, 1] = np.asarray(cs)[r]
-        J[:, 2] = np.asarray(cs)[c]
-        spans.append((k, sum(len(j) for j in jobs), r, c))
-        jobs.append(J)
-    g = ops.dots(np.concatenate(jobs))
+        P = ops.gram(gb)
+        ops.combine_blocks([(k, cs, 1.0, basis, -p) for (k, cs, basis), p in zip(gb, P)])
+    Ms = ops.gram([(k, [sl for sl, _ in cands[k]], [sl for sl, _ in cands[k]]) for k in act])
     blocks = []
-    for k, at, r, c in spans:
+    for k, M in zip(act, Ms):
         s = subs[k]
         cs = [sl for sl, _ in cands[k]]
-        m = len(cs)
-        M = np.zeros((m, m))
-        M[r, c] = g[at:at + len(r)]
-        M[c, r] = g[at:at + len(r)]
+        M = 0.5 * (M + M.T)
         room = min(s.dim, L.cap) - s.k - s.pending
         T = _accept(M, np.array([n0 for _, n0 in cands[k]]), thresh, room)
         if len(T):
-            blocks.append((k, L.V + s.k + s.pending + np.arange(len(T)), 0.0, cs, T))
+            blocks.append((k, s.V(L) + s.k + s.pending + np.arange(len(T)), 0.0, cs, T))
             s.pending += len(T)
     ops.combine_blocks(blocks)
 
 
 def _collapse(ops, L, subs, which):
     """Restart the subspaces of molecules ``which`` on their current and last
-    Ritz vectors (orthonormalised, sigmas following linearly); the last
-    step's vectors keep the direction of approach, which a restart onto the
-    current ones alone would lose."""
-    rows = {}
+    Ritz vectors; the last step's vectors keep the direction of approach,
+    which a restart onto the current ones alone would lose.  Both are
+    combinations of the orthonormal basis, so the restart is orthonormalised
+    in basis coordinates (Gram-Schmidt applied twice; a vector keeping less
+    than 1e-6 of its norm is dropped): V' = V Q and S' = S Q, written into the
+    other region, and G' = Q^T G Q."""
+    blocks = []
     for k in which:
         s = subs[k]
-        cur = [(L.XB[s.cur] + q, L.XB[s.cur] + L.R + q) for q in range(s.rows)]
-        old = [(L.XB[1 - s.cur] + q, L.XB[1 - s.cur] + L.R + q) for q in range(s.prev_rows)]
-        rows[k] = cur + old
-        s.k = 0
-    width = max((len(r) for r in rows.values()), default=0)
-    for r in range(width):
-        act = [k for k in which if r < len(rows[k])]
-        if not act:
-            continue
-        jobs, spans = [], []
-        for k in act:
-            x, _ = rows[k][r]
-            nb = subs[k].k
-            spans.append((len(jobs), nb))
-            jobs.append((k, x, x))
-            jobs.extend((k, x, L.V + j) for j in range(nb))
-        P = ops.dots(jobs)
-        n0 = {k: np.sqrt(max(P[at], 0.0)) for k, (at, _) in zip(act, spans)}
-        cj = []
-        for k, (at, nb) in zip(act, spans):
-            x, sx = rows[k][r]
-            a = subs[k].k
-            cj.append((k, L.V + a, 0.0, [(x, 1.0)] + [(L.V + j, -P[at + 1 + j]) for j in range(nb)]))
-            cj.append((k, L.S + a, 0.0, [(sx, 1.0)] + [(L.S + j, -P[at + 1 + j]) for j in range(nb)]))
-        ops.combine(cj)
-        jobs, spans = [], []
-        for k in act:
-            a = subs[k].k
-            spans.append((len(jobs), a))
-            jobs.extend((k, L.V + a, L.V + j) for j in range(a))
-        P = ops.dots(jobs)
-        cj = []
-        for k, (at, nb) in zip(act, spans):
-            a = subs[k].k
-            if nb:
-                cj.append((k, L.V + a, 1.0, [(L.V + j, -P[at + j]) for j in range(nb)]))
-                cj.append((k, L.S + a, 1.0, [(L.S + j, -P[at + j]) for j in range(nb)]))
-        ops.combine(cj)
-        nrm = np.sqrt(np.maximum(ops.dots([(k, L.V + subs[k].k, L.V + subs[k].k) for k in act]), 0.0))
-        cj = []
-        for k, nr in zip(act, nrm):
-            s = subs[k]
-            if nr > 1e-6 * n0[k]:
-                cj.append((k, L.V + s.k, 1.0 / nr, []))
-                cj.append((k, L.S + s.k, 1.0 / nr, []))
-                s.k += 1
-        ops.combine(cj)
-    jobs, spans = [], []
-    for k in which:
-        kk = subs[k].k
-        spans.append((len(jobs), kk))
-        jobs.extend((k, L.V + a, L.S + b) for a in range(kk) for b in range(kk))
-    g = ops.dots(jobs)
-    for k, (at, kk) in zip(which, spans):
-        subs[k].G = g[at:at + kk * kk].reshape(kk, kk)
+        kk = s.k
+        prev = np.zeros((kk, s.Yprev.shape[1]))
+        prev[:len(s.Yprev)] = s.Yprev
+        W = np.concatenate([s.Ycur, prev], axis=1)
+        Q = np.zeros((kk, 0))
+        for j in range(W.shape[1]):
+            c = W[:, j].copy()
+            n0 = np.linalg.norm(c)
+            for _ in range(2):
+                c -= Q @ (Q.T @ c)
+            nr = np.linalg.norm(c)
+            if nr > 1e-6 * n0 and Q.shape[1] < kk:
+                Q = np.concatenate([Q, (c / nr)[:, None]], axis=1)
+        v1, s1 = L.VS[1 - s.reg]
+        blocks.append((k, v1 + np.arange(Q.shape[1]), 0.0, s.V(L) + np.arange(kk), Q.T))
+        blocks.append((k, s1 + np.arange(Q.shape[1]), 0.0, s.S(L) + np.arange(kk), Q.T))
+        s.G = Q.T @ s.G @ Q
+        s.Ycur = Q.T @ s.Ycur
+        s.reg = 1 - s.reg
+        s.k = Q.shape[1]
+    ops.combine_blocks(blocks)
 
 
 def _absorb(ops, L, subs, act):
     """Extend G by the rows and columns of the vectors whose sigmas just arrived."""
-    jobs, spans = [], []
-    at = 0
+    gb = []
     for k in act:
         s = subs[k]
         k0, k1 = s.k, s.k + s.pending
-        a, b = np.meshgrid(np.arange(k1), np.arange(k0, k1), indexing="ij")      # new columns, all rows
-        c, d = np.meshgrid(np.arange(k0, k1), np.arange(k0), indexing="ij")      # new rows, old columns
-        J = np.empty((a.size + c.size, 3), dtype=np.int32)
-        J[:, 0] = k
-        J[:, 1] = L.V + np.concatenate([a.ravel(), c.ravel()])
-        J[:, 2] = L.S + np.concatenate([b.ravel(), d.ravel()])
-        jobs.append(J)
-        spans.append((at, k0, k1))
-        at += len(J)
-    g = ops.dots(np.concatenate(jobs))
-    for k, (at, k0, k1) in zip(act, spans):
+        gb.append((k, s.V(L) + np.arange(k1), s.S(L) + np.arange(k0, k1)))      # new columns, all rows
+        gb.append((k, s.V(L) + np.arange(k0, k1), s.S(L) + np.arange(k0)))      # new rows, old columns
+    g = ops.gram(gb)
+    for q, k in enumerate(act):
         s = subs[k]
+        k0, k1 = s.k, s.k + s.pending
         G = np.zeros((k1, k1))
         G[:k0, :k0] = s.G
-        p = k1 - k0
-        G[:, k0:] = g[at:at + k1 * p].reshape(k1, p)
-        G[k0:, :k0] = g[at + k1 * p:at + k1 * p + p * k0].reshape(p, k0)
+        G[:, k0:] = g[2 * q]
+        G[k0:, :k0] = g[2 * q + 1]
         s.G = G
         s.k, s.pending = k1, 0
 
 
-def _ritz(s):
-    """Rayleigh-Ritz of one subspace: the working Ritz values, lowest real
-    parts first (one entry per real value and per complex pair), and the
-    subspace coefficients of their vectors (a real one: one row; a complex
-    pair: its real and imaginary parts)."""
-    lam, Y = np.linalg.eig(s.G)   # right eigenvectors of V^T J V
+def _ritz_all(subs, act):
+    """_ritz of the molecules ``act``, the eigenproblems of equal dimension
+    solved as one stacked call."""
+    by_k = {}
+    for k in act:
+        by_k.setdefault(subs[k].k, []).append(k)
+    out = {}
+    for _, ks in by_k.items():
+        lam, Y = np.linalg.eig(np.stack([subs[k].G for k in ks]))
+        for q, k in enumerate(ks):
+            out[k] = _ritz(subs[k], lam[q], Y[q])
+    return out
+
+
+def _ritz(s, lam, Y):
+    """Rayleigh-Ritz of one subspace from the eigenpairs (lam, Y) of
+    G = V^T J V: the working Ritz values, lowest real parts first (one entry
+    per real value and per complex pair), and the subspace coefficients of
+    their vectors (a real one: one row; a complex pair: its real and
+    imaginary parts)."""
     keep = lam.imag >= -1e-12 * np.maximum(np.abs(lam.real), 1.0)
     lam, Y = lam[keep], Y[:, keep]
     order = np.argsort(lam.real, kind="stable")
@@ -465,7 +441,7 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
         count = min(s.n1, nroots + 6)
         starts = []
         if gammas is not None:
-            U = cis_singlets(C, np.asarray(gammas[k], dtype=np.float64), e, o)[1]
+            U = cis_singlets(C, np.asarray(gammas[k], dtype=np.float64), e, o, count)[1]
             for q in range(count):
                 x = np.zeros(s.N)
                 x[:s.n1] = U[:, q] / np.linalg.norm(U[:, q])
@@ -494,27 +470,29 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
             for k in act:
                 s = subs[k]
                 if c < s.pending:
-                    ins[k] = L.V + s.k + c
-                    outs[k] = L.S + s.k + c
+                    ins[k] = s.V(L) + s.k + c
+                    outs[k] = s.S(L) + s.k + c
             ctx.lr_sigma_slots(ins, outs, h)
         _absorb(ops, L, subs, act)
-        # Ritz vectors and their sigmas into the current buffer
+        # Ritz vectors and their sigmas into the Ritz buffer
         ritz = {}
         blocks = []
+        xb = L.XB
+        found = _ritz_all(subs, act)
         for k in act:
             s = subs[k]
-            entries, rows = _ritz(s)
+            entries, rows = found[k]
             ritz[k] = entries
             s.rows = len(rows)
-            xb = L.XB[s.cur]
             Y = np.array(rows).reshape(-1, s.k)
-            blocks.append((k, xb + np.arange(len(Y)), 0.0, L.V + np.arange(s.k), Y))
-            blocks.append((k, xb + L.R + np.arange(len(Y)), 0.0, L.S + np.arange(s.k), Y))
+            s.Ycur = Y.T
+            blocks.append((k, xb + np.arange(len(Y)), 0.0, s.V(L) + np.arange(s.k), Y))
+            blocks.append((k, xb + L.R + np.arange(len(Y)), 0.0, s.S(L) + np.arange(s.k), Y))
         ops.combine_blocks(blocks)
         # residuals J x - lam x into the candidate slots, and their max-norms
         cj, mj, where = [], [], {}
         for k in act:
-            s, xb = subs[k], L.XB[subs[k].cur]
+            s = subs[k]
             q = 0
             where[k] = []
             for lam, real, c in ritz[k]:
@@ -558,7 +536,7 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
             for r, (val, res, c) in enumerate(reals[:s.nroots]):
                 s.omega[r] = val
                 s.res[r] = res
-                s.root_slot[r] = L.XB[s.cur] + c
+                s.root_slot[r] = L.XB + c
             nr = min(s.nroots, s.dim)
             if len(reals) >= nr and all(res < eps for _, res, _ in reals[:nr]) and worst_pair < 1e-3:
                 s.done = True
@@ -581,9 +559,7 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
         if collapse:
             _collapse(ops, L, subs, collapse)
         for k in act:
-            s = subs[k]
-            s.prev_rows = s.rows
-            s.cur = 1 - s.cur
+            subs[k].Yprev = subs[k].Ycur
         _orthonormalize(ops, L, subs, cands, 1e-3)
         # fewer Ritz values than working roots: widen the subspace along the
         # next diagonal directions

@@ -226,6 +226,19 @@ int hcc_lr_dots(hcc_ctx* ctx, int njobs, const int* jobs, int mode, double* out)
  * terms (same molecule; no term may be dst; beta = 0 ignores dst's contents). */
 int hcc_lr_combine(hcc_ctx* ctx, int njobs, const int* jobs, const double* beta, const int* term_slot,
                    const double* term_coef);
+/* Gram blocks: njobs jobs of 5 ints {mol, na, nb, a, b}: the na x nb block
+ * G[p][q] = <slots[a + p], slots[b + q]> of molecule mol, written row-major
+ * into out, the jobs' blocks one after another.  slots: nslot slot numbers.
+ * Each vector of a job streams from memory once per 16 x 64 piece of its
+ * block (hcc_lr_dots reads both vectors of every pair). */
+int hcc_lr_gram(hcc_ctx* ctx, int njobs, const int* jobs, int nslot, const int* slots, double* out);
+/* Block combinations: njobs jobs of 6 ints {mol, m, t, d, s, c}: for p < m,
+ * slots[d + p] = beta[j] slots[d + p] + sum_{q < t} coef[c + p t + q] slots[s + q]
+ * (m distinct destinations, none among the terms; beta = 0 ignores their
+ * contents).  slots: nslot slot numbers, coef: ncoef values.  The terms
+ * stream from memory once per 16 destinations. */
+int hcc_lr_combine_block(hcc_ctx* ctx, int njobs, const int* jobs, const double* beta, int nslot, const int* slots,
+                         int ncoef, const double* coef);
 /* njobs jobs of 2 ints {mol, slot}: x_e /= (shift[j] - D_e) with the
  * orbital-energy difference D_e (e_a - e_i for singles, e_a + e_b - e_i - e_j
  * for doubles) and |shift - D_e| raised to `floor` where smaller. */

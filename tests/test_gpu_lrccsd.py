@@ -12,8 +12,9 @@ against the oracle (oracle/lrccsd.py, pinned by the two-electron full CI):
 * for two electrons they equal the singlet full-CI excitation energies;
 * a molecule's roots in the batch equal its roots alone;
 * the device vector space's operations (dot products, max-norms, linear
-  combinations, the orbital-energy preconditioner, sigma between slots)
-  equal their numpy counterparts, and invalid jobs are rejected.
+  combinations, Gram blocks, block combinations, the orbital-energy
+  preconditioner, sigma between slots) equal their numpy counterparts, and
+  invalid jobs are rejected.
 """
 import numpy as np
 import pytest
@@ -197,6 +198,82 @@ def test_vector_space_operations():
         ctx.lr_dots([(0, 6, 0)])
     with pytest.raises(capi.HccError):
         ctx.lr_sigma_slots(np.array([0, -1, 0]), np.array([0, 0, 1]), 1e-5)   # in == out
+    ctx.close()
+
+
+def test_block_gram_and_combinations():
+    """hcc_lr_gram and hcc_lr_combine_block against numpy on the same slots:
+    blocks wider than one 16 x 64 piece, a single pair, vector lengths that
+    end inside a 16-element run, a 64-element step and a 4096-element chunk
+    (N = 272, 2450, 50850), slot lists out of order, beta on and off, a
+    scaling with no terms; invalid jobs are rejected before any work."""
+    mols = [M.octatetraene(), M.random_fragment(14, seed=3), M.random_fragment(30, seed=5)]
+    preps = [prepare(m) for m in mols]
+    ctx = _solved_ctx(preps)
+    nsl = 96
+    ctx.lr_space(nsl)
+    rng = np.random.default_rng(11)
+    X = []
+    for k, m in enumerate(mols):
+        n = m.nocc * (m.n - m.nocc) * (1 + m.nocc * (m.n - m.nocc))
+        rows = rng.standard_normal((nsl, n))
+        for sl in range(nsl):
+            ctx.lr_put(k, sl, rows[sl])
+        X.append(rows)
+    perm = rng.permutation(nsl)
+    blocks = [(0, perm[:20], perm[20:90]), (1, perm[5:6], perm[7:8]), (2, perm[:20], perm[20:90]),
+              (2, perm[90:93], perm[90:93]), (1, perm[:17], perm[30:95])]
+    heads, sl = [], []
+    at = 0
+    for k, a, b in blocks:
+        heads.append((k, len(a), len(b), at, at + len(a)))
+        sl.extend(a)
+        sl.extend(b)
+        at += len(a) + len(b)
+    g = ctx.lr_gram(np.array(heads), np.array(sl))
+    at = 0
+    for k, a, b in blocks:
+        want = X[k][a] @ X[k][b].T
+        bound = np.abs(X[k][a]) @ np.abs(X[k][b]).T
+        got = g[at:at + want.size].reshape(want.shape)
+        assert np.all(np.abs(got - want) <= 1e-13 * bound)
+        at += want.size
+    # block combinations
+    jobs = [(0, perm[:18], 0.5, perm[18:88]), (2, perm[:18], 0.0, perm[18:88]), (1, perm[40:41], 2.0, []),
+            (2, perm[88:91], -1.0, perm[91:96])]
+    heads, sl, cf, beta = [], [], [], []
+    at = nc = 0
+    coefs = []
+    for k, d, bt, t in jobs:
+        C = rng.standard_normal((len(d), len(t)))
+        coefs.append(C)
+        heads.append((k, len(d), len(t), at, at + len(d), nc))
+        sl.extend(d)
+        sl.extend(t)
+        cf.extend(C.ravel())
+        beta.append(bt)
+        at += len(d) + len(t)
+        nc += C.size
+    ctx.lr_combine_block(np.array(heads), np.array(beta), np.array(sl), np.array(cf))
+    for (k, d, bt, t), C in zip(jobs, coefs):
+        want = bt * X[k][d] + (C @ X[k][t] if len(t) else 0.0)
+        bound = abs(bt) * np.abs(X[k][d]) + (np.abs(C) @ np.abs(X[k][t]) if len(t) else 0.0)
+        for q, s in enumerate(d):
+            got = ctx.lr_get(k, s, X[k].shape[1])
+            assert np.all(np.abs(got - want[q]) <= 1e-13 * bound[q])
+            X[k][s] = got
+    # untouched slots stay as they were
+    for k in range(3):
+        assert np.array_equal(ctx.lr_get(k, perm[95], X[k].shape[1]), X[k][perm[95]])
+    with pytest.raises(capi.HccError):   # destination among the terms
+        ctx.lr_combine_block(np.array([[0, 1, 1, 0, 1, 0]]), np.array([0.0]), np.array([3, 3]), np.array([1.0]))
+    with pytest.raises(capi.HccError):   # repeated destination
+        ctx.lr_combine_block(np.array([[0, 2, 1, 0, 2, 0]]), np.array([0.0]), np.array([3, 3, 4]),
+                             np.array([1.0, 1.0]))
+    with pytest.raises(capi.HccError):   # slot-list range past its end
+        ctx.lr_gram(np.array([[0, 2, 2, 0, 1]]), np.array([1, 2]))
+    with pytest.raises(capi.HccError):   # slot out of range
+        ctx.lr_gram(np.array([[0, 1, 1, 0, 1]]), np.array([1, nsl]))
     ctx.close()
 
 

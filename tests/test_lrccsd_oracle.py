@@ -99,6 +99,24 @@ class _OracleCtx:
                 acc = acc + term_coef[t] * self.space[m][term_slot[t]]
             self.space[m][d] = acc
 
+    def lr_gram(self, jobs, slots):
+        out = []
+        for m, na, nb, a, b in np.asarray(jobs, dtype=np.int64).reshape(-1, 5):
+            X = self.space[m][slots[a:a + na]]
+            Y = self.space[m][slots[b:b + nb]]
+            out.append((X @ Y.T).ravel())
+        return np.concatenate(out) if out else np.zeros(0)
+
+    def lr_combine_block(self, jobs, beta, slots, coef):
+        for (m, mm, t, d, s, c), bt in zip(np.asarray(jobs, dtype=np.int64).reshape(-1, 6), beta):
+            dst, src = list(slots[d:d + mm]), list(slots[s:s + t])
+            assert len(set(dst)) == mm and not set(dst) & set(src)
+            C = np.asarray(coef[c:c + mm * t]).reshape(mm, t)
+            acc = C @ self.space[m][src] if t else np.zeros((mm, self.space[m].shape[1]))
+            if bt != 0.0:
+                acc = acc + bt * self.space[m][dst]
+            self.space[m][dst] = acc
+
     def lr_precondition(self, jobs, shift, floor):
         for (m, sl), w in zip(np.asarray(jobs).reshape(-1, 2), shift):
             p, o = self.items[m][0], self.items[m][1]
@@ -163,6 +181,32 @@ def test_cis_singlets_are_spin_orbital_eigenpairs():
             x = r.ravel() / np.linalg.norm(r)
             assert np.abs(A @ x - w[k] * x).max() < 1e-10
         assert np.all(np.diff(w) >= 0)
+        # the lowest few alone (the Davidson's start vectors): same values, eigenvectors of A
+        ws, Us = lrccsd.cis_singlets(p["C"], p["G"], np.diag(p["F"]), o, 5)
+        assert ws.shape == (5,) and np.abs(ws - w[:5]).max() < 1e-10
+        for k in range(5):
+            r = np.zeros((2 * o, 2 * v))
+            r[0::2, 0::2] = r[1::2, 1::2] = Us[:, k].reshape(o, v)
+            x = r.ravel() / np.linalg.norm(r)
+            assert np.abs(A @ x - ws[k] * x).max() < 1e-10
+
+
+def test_double_seeds_follow_the_stable_diagonal_order():
+    """The lowest pair-symmetric unit doubles are those of a stable sort of
+    the diagonal, also where orbital energies are degenerate (ties)."""
+    from helios_amd.lrccsd import _Layout, _Sub
+    o, v = 3, 4
+    for e in [np.array([-3.0, -2.0, -2.0, 1.0, 1.0, 2.5, 4.0]), np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0])]:
+        s = _Sub(o, v, e, 3, _Layout(3, 48))
+        n1 = o * v
+        m, n = np.triu_indices(n1)
+        d = s._d2[m // v, n // v, m % v, n % v]
+        for count in (1, 2, 5, len(d), len(d) + 3):
+            want = []
+            for q in np.argsort(d, kind="stable")[:count]:
+                i, a, j, b = m[q] // v, m[q] % v, n[q] // v, n[q] % v
+                want.append((n1 + ((i * o + j) * v + a) * v + b, n1 + ((j * o + i) * v + b) * v + a))
+            assert s.double_seeds(count) == want
 
 
 def test_average_excitation_level():
