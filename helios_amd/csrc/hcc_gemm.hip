@@ -683,6 +683,39 @@ hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const 
   return hipGetLastError();
 }
 
+// Compacted table: dst[i] = src[idx[i]] with tile_begin = tbeg[i] (the live
+// problems of a table, their tiles renumbered consecutively)
+__global__ __launch_bounds__(256) void k_compact_probs(const GemmProb* __restrict__ src, const int* __restrict__ idx,
+                                                       const int* __restrict__ tbeg, int n, GemmProb* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  GemmProb p = src[idx[i]];
+  p.tile_begin = tbeg[i];
+  dst[i] = p;
+}
+
+// out[x] = i with prefix[i] <= x < prefix[i + 1] (prefix: n + 1 ascending entries)
+__global__ __launch_bounds__(256) void k_expand_prefix(const int* __restrict__ prefix, int n, int* __restrict__ out,
+                                                       int total) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  out[x] = lo;
+}
+
+hipError_t launch_compact_table(const GemmProb* src, const int* idx, const int* tbeg, const int* rpre, int n,
+                                int tiles, int red, GemmProb* dst, int* tile_prob, int* red_bprob, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact_probs, dim3((n + 255) / 256), dim3(256), 0, s, src, idx, tbeg, n, dst);
+  if (tiles > 0) hipLaunchKernelGGL(k_expand_prefix, dim3((tiles + 255) / 256), dim3(256), 0, s, tbeg, n, tile_prob, tiles);
+  if (red > 0) hipLaunchKernelGGL(k_expand_prefix, dim3((red + 255) / 256), dim3(256), 0, s, rpre, n, red_bprob, red);
+  return hipGetLastError();
+}
+
 template <int BM, int BN, int BK>
 static void launch_f64(const GemmProb* probs, const int* tp, int total_tiles, const int* active, hipStream_t s,
                        int grid, int flags) {

@@ -92,4 +92,11 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tile_prob, int tot
 hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const int* block_prob, int nblocks,
                                 const int* active, hipStream_t s);
 
+// Compacts a table to its live problems: dst[i] = src[idx[i]] with
+// tile_begin = tbeg[i], tile_prob over the renumbered tiles (tbeg: n + 1
+// entries, tiles = tbeg[n]) and red_bprob over the split-K reduction blocks
+// (rpre: n + 1 entries, red = rpre[n])
+hipError_t launch_compact_table(const GemmProb* src, const int* idx, const int* tbeg, const int* rpre, int n,
+                                int tiles, int red, GemmProb* dst, int* tile_prob, int* red_bprob, hipStream_t s);
+
 }  // namespace hcc

@@ -65,7 +65,29 @@ struct GemmTable {
   int tile = 64;               // output tile edge (64 or 32)
   int alt = -1;                // second part of the same contraction (other tile edge), index in Batch::alt
   int flags = 3;               // a_kfast | b_nfast << 1 | (dA != 0) << 2 | (dB != 0) << 3 | (K <= 64) << 4
+  // compaction to the molecules still running (compact_tables): per problem
+  // its molecule, tiles and split-K reduction blocks; the compacted copies of
+  // the problems, the tile and reduction-block maps, and the uploaded lists
+  std::vector<int> hmol, htiles, hred, hbuf;
+  GemmProb* cd = nullptr;
+  int *ctp = nullptr, *cbp = nullptr, *cbuf = nullptr;
+  int ctiles = -1;             // tiles of the compacted copy; -1: not compacted
+  int cred = 0;
 };
+
+// what a launch of a table uses: its compacted copy when there is one
+struct TableView {
+  const GemmProb* d;
+  const int* tp;
+  int tiles;
+  const int* rp;
+  const int* rb;
+  int red;
+};
+static TableView view(const GemmTable& t) {
+  if (t.ctiles < 0) return {t.d, t.tile_prob, t.tiles, t.red_prefix, t.red_bprob, t.red_blocks};
+  return {t.cd, t.ctp, t.ctiles, t.cbuf + 2 * t.nprob + 1, t.cbp, t.cred};
+}
 
 // Tile kinds of the FP64 GEMM: 64 (64x64, 4 waves), 32 (32x32, 1 wave) and
 // 128 (128x32, 4 waves stacked along M, for outputs at most 32 wide); of the
@@ -174,6 +196,9 @@ struct Batch {
   // device-resident vector space of the eigensolver (hcc_lr.h): lr_slots
   // vectors of lr_len[k] doubles per molecule from element lr_base[k]; own
   // allocations (not in `allocs`), replaced when the slot count changes
+  // running molecules the FP64 (it) and correction (dit) tables are compacted
+  // to; 0: not compacted
+  int it_live = 0, dit_live = 0;
   // molecules whose amplitude arenas hold meaningful amplitudes (a guess, a
   // solve or hcc_batch_set_amplitudes): HCC_GUESS_CURRENT needs all of them
   std::vector<char> amp_valid;
@@ -298,6 +323,11 @@ static GemmTable build_group(Batch& b, std::vector<GemmProb> probs, int ts) {
   t.nprob = (int)probs.size();
   t.tiles = tiles;
   t.tile = ts;
+  for (const GemmProb& p : probs) {
+    t.hmol.push_back(p.mol);
+    t.htiles.push_back(((p.M + tile_m(ts) - 1) / tile_m(ts)) * p.tiles_n * p.ksplit);
+    t.hred.push_back(p.ksplit > 1 ? (int)(((long)p.M * p.N + 255) / 256) : 0);
+  }
   if (!probs.empty()) {
     // the staging layout is a performance choice only (addressing is general):
     // take the largest problem's unit-stride flags for the whole table
@@ -915,11 +945,87 @@ static_assert(std::atomic<int>::is_always_lock_free, "hcc_request_stop must be a
 namespace hcc {
 
 static void launch_table(hcc_ctx* c, const GemmTable& t) {
-  HCHECK(launch_gemm_f64(t.d, t.tile_prob, t.tiles, c->batch->dact, c->stream, t.tile, t.flags));
-  if (t.red_blocks > 0)
-    HCHECK(launch_splitk_reduce(t.d, t.red_prefix, t.red_bprob, t.red_blocks, c->batch->dact, c->stream));
+  const TableView v = view(t);
+  HCHECK(launch_gemm_f64(v.d, v.tp, v.tiles, c->batch->dact, c->stream, t.tile, t.flags));
+  if (v.red > 0) HCHECK(launch_splitk_reduce(v.d, v.rp, v.rb, v.red, c->batch->dact, c->stream));
   if (t.alt >= 0) launch_table(c, c->batch->alt[t.alt]);
 }
+
+// One table compacted to the problems of molecules with act != 0: their tiles
+// renumbered consecutively, so a launch dispatches no workgroup of a finished
+// molecule (each would only read its molecule's flag and return).
+static void compact_table(Batch& b, GemmTable& t, const std::vector<int>& act, hipStream_t s) {
+  if (t.nprob == 0) return;
+  const int np = t.nprob;
+  if (t.cd == nullptr) {
+    t.cd = b.dalloc<GemmProb>(np);
+    t.ctp = b.dalloc<int>(std::max(1, t.tiles));
+    t.cbp = b.dalloc<int>(std::max(1, t.red_blocks));
+    t.cbuf = b.dalloc<int>(3 * (size_t)np + 2);
+  }
+  // the previous upload from hbuf has completed: a compaction follows a
+  // synchronisation of the stream
+  t.hbuf.assign(3 * (size_t)np + 2, 0);
+  int* idx = t.hbuf.data();
+  int* tbeg = idx + np;
+  int* rpre = tbeg + np + 1;
+  int n = 0, tiles = 0, red = 0;
+  for (int q = 0; q < np; ++q) {
+    if (act[t.hmol[q]] == 0) continue;
+    idx[n] = q;
+    tbeg[n] = tiles;
+    rpre[n] = red;
+    ++n;
+    tiles += t.htiles[q];
+    red += t.hred[q];
+  }
+  tbeg[n] = tiles;
+  rpre[n] = red;
+  HCHECK(hipMemcpyAsync(t.cbuf, t.hbuf.data(), t.hbuf.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HCHECK(launch_compact_table(t.d, t.cbuf, t.cbuf + np, t.cbuf + 2 * np + 1, n, tiles, red, t.cd, t.ctp, t.cbp, s));
+  t.ctiles = tiles;
+  t.cred = red;
+}
+
+// The FP64 residual's tables (dual = false) or the correction residual's
+// (dual = true), with their parts of other tile kinds, compacted to the
+// molecules with act != 0 once their count has fallen to kCompactFrac of the
+// count the tables were last built for.  A compaction must keep every
+// molecule the device will run before the next one: the running set of a solve
+// only shrinks, except that each inner phase of the mixed solver starts again
+// from all molecules still running (reset = true rebuilds from that set).
+// Returns whether the launches changed (captured graphs are then stale).
+constexpr double kCompactFrac = 0.85;
+static bool compact_tables(hcc_ctx* c, bool dual, const std::vector<int>& act, int alive, bool reset) {
+  Batch& b = *c->batch;
+  if (b.arena != nullptr || alive <= 0) return false;   // no-allocation contexts keep full tables
+  int& last = dual ? b.dit_live : b.it_live;
+  std::vector<GemmTable>& tabs = dual ? b.dit : b.it;
+  auto each = [&](const std::function<void(GemmTable&)>& f) {
+    for (GemmTable& t : tabs)
+      for (GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &b.alt[p->alt] : nullptr) f(*p);
+  };
+  if (alive > kCompactFrac * (reset || last == 0 ? b.nm : last)) {
+    if (!reset || last == 0) return false;
+    each([](GemmTable& t) { t.ctiles = -1; });   // the running set grew back: full tables
+    last = 0;
+    return true;
+  }
+  each([&](GemmTable& t) { compact_table(b, t, act, c->stream); });
+  last = alive;
+  return true;
+}
+
+// full tables again (end of a solve: other callers run any molecule)
+static void uncompact_tables(Batch& b) {
+  for (auto* v : {&b.it, &b.dit, &b.alt})
+    for (GemmTable& t : *v) t.ctiles = -1;
+  b.it_live = b.dit_live = 0;
+}
+struct UncompactGuard {
+  Batch& b;
+  ~UncompactGuard() { uncompact_tables(b); }
+};
 
 // GEMM kernel dispatches of a table (its parts of different tile kinds)
 static int table_parts(const hcc_ctx* c, const GemmTable& t) {
@@ -1057,10 +1163,9 @@ static void dgemm(hcc_ctx* c, const GemmTable& t, int id) {
   }
   prof_open(c, id);
   for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr) {
-    HCHECK(launch_gemm_dual(p->d, p->tile_prob, p->tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16,
-                            p->tile));
-    if (p->red_blocks > 0)
-      HCHECK(launch_splitk_reduce(p->d, p->red_prefix, p->red_bprob, p->red_blocks, c->batch->dact, c->stream));
+    const TableView v = view(*p);
+    HCHECK(launch_gemm_dual(v.d, v.tp, v.tiles, c->batch->dact, c->stream, p->flags, c->mixed_bf16, p->tile));
+    if (v.red > 0) HCHECK(launch_splitk_reduce(v.d, v.rp, v.rb, v.red, c->batch->dact, c->stream));
   }
   prof_close(c);
 }
@@ -2238,6 +2343,10 @@ struct GraphExec {
     if (e) (void)hipGraphExecDestroy(e);
   }
   void launch(hipStream_t s) { HCHECK(hipGraphLaunch(e, s)); }
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
 };
 
 static void capture(GraphExec& ge, hipStream_t s, const std::function<void()>& body) {
@@ -2277,10 +2386,13 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
   // reads its molecule's state on the device, so the launches never change
   GraphExec block;
   const bool graphs = ctx->graphs && !ctx->prof && maxiter >= check_every;
-  if (graphs)
+  auto capture_block = [&]() {
     capture(block, s, [&]() {
       for (int q = 0; q < check_every; ++q) run_iteration(ctx, prm, nullptr);
     });
+  };
+  UncompactGuard uncompact{b};
+  if (graphs) capture_block();
   while (it < maxiter) {
     if (graphs && it + check_every <= maxiter) {
       block.launch(s);
@@ -2294,6 +2406,10 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
     int alive = 0;
     for (int a : act) alive += a;
     if (alive == 0 || take_stop(ctx)) break;
+    if (compact_tables(ctx, false, act, alive, false) && graphs) {
+      block.reset();
+      capture_block();
+    }
   }
   prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
@@ -2356,7 +2472,12 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   std::vector<int> act(b.nm);
   ctx->stop_req.store(0);   // each iterator run starts unflagged (gIterator.f90:57)
   bool stopped = false;
-  auto alive = [&]() {
+  GraphExec inner_block;
+  UncompactGuard uncompact{b};
+  // outer = true: after the outer control (the running set, which the next
+  // inner phase and the next outer residual start from); false: after inner
+  // steps (the molecules still in their inner solve)
+  auto alive = [&](bool outer) {
     HCHECK(hipMemcpyAsync(act.data(), b.dact, b.nm * sizeof(int), hipMemcpyDeviceToHost, s));
     HCHECK(hipStreamSynchronize(s));
     if (ctx->prof) prof_collect(ctx);
@@ -2366,6 +2487,8 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       stopped = true;
       return 0;
     }
+    if (outer) compact_tables(ctx, false, act, n, false);
+    if (compact_tables(ctx, true, act, n, outer)) inner_block.reset();
     return n;
   };
   auto inner_iteration = [&]() {
@@ -2406,7 +2529,6 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
   // run_solve); the host checks between blocks whether any molecule is still
   // in its inner solve, so short blocks cut the iterations that run after the
   // last molecule has finished
-  GraphExec inner_block;
   const bool graphs = ctx->graphs && !ctx->prof;
   for (int outer = 0; outer < ctx->max_outer + 1; ++outer) {
     set_phase(b.dst, b.dact, b.nm, 0, s);
@@ -2415,7 +2537,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm0, b.part1, b.part2, s);
     mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     prof_close(ctx);
-    if (alive() == 0 || stopped) break;
+    if (alive(true) == 0 || stopped) break;
     prof_open(ctx, kProfCommit);
     set_phase(b.dst, b.dact, b.nm, 1, s);
     if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
@@ -2432,7 +2554,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       } else {
         for (int q = 0; q < kInnerBlock && inner < ctx->max_inner; ++q, ++inner) inner_iteration();
       }
-      if (alive() == 0) break;
+      if (alive(false) == 0) break;
     }
     if (stopped) break;
     prof_open(ctx, kProfCommit);
