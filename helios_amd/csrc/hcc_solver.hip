@@ -1027,10 +1027,12 @@ struct UncompactGuard {
   ~UncompactGuard() { uncompact_tables(b); }
 };
 
-// GEMM kernel dispatches of a table (its parts of different tile kinds)
+// GEMM kernel dispatches of a table: its parts of different tile kinds with
+// tiles to run (a part compacted to no running molecule is not dispatched)
 static int table_parts(const hcc_ctx* c, const GemmTable& t) {
-  int n = 1;
-  for (int a = t.alt; a >= 0; a = c->batch->alt[a].alt) ++n;
+  int n = 0;
+  for (const GemmTable* p = &t; p != nullptr; p = p->alt >= 0 ? &c->batch->alt[p->alt] : nullptr)
+    n += view(*p).tiles > 0;
   return n;
 }
 
