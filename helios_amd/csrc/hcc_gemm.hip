@@ -707,6 +707,11 @@ __global__ __launch_bounds__(256) void k_expand_prefix(const int* __restrict__ p
   out[x] = lo;
 }
 
+hipError_t launch_expand_prefix(const int* prefix, int n, int* out, int total, hipStream_t s) {
+  if (total > 0) hipLaunchKernelGGL(k_expand_prefix, dim3((total + 255) / 256), dim3(256), 0, s, prefix, n, out, total);
+  return hipGetLastError();
+}
+
 hipError_t launch_compact_table(const GemmProb* src, const int* idx, const int* tbeg, const int* rpre, int n,
                                 int tiles, int red, GemmProb* dst, int* tile_prob, int* red_bprob, hipStream_t s) {
   if (n <= 0) return hipSuccess;

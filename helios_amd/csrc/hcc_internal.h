@@ -92,6 +92,9 @@ hipError_t launch_gemm_dual(const GemmProb* probs, const int* tile_prob, int tot
 hipError_t launch_splitk_reduce(const GemmProb* probs, const int* prefix, const int* block_prob, int nblocks,
                                 const int* active, hipStream_t s);
 
+// out[x] = i with prefix[i] <= x < prefix[i + 1] for x < total (prefix: n + 1
+// ascending entries; zero-length ranges are never chosen for a valid x)
+hipError_t launch_expand_prefix(const int* prefix, int n, int* out, int total, hipStream_t s);
 // Compacts a table to its live problems: dst[i] = src[idx[i]] with
 // tile_begin = tbeg[i], tile_prob over the renumbered tiles (tbeg: n + 1
 // entries, tiles = tbeg[n]) and red_bprob over the split-K reduction blocks

@@ -199,6 +199,17 @@ struct Batch {
   // running molecules the FP64 (it) and correction (dit) tables are compacted
   // to; 0: not compacted
   int it_live = 0, dit_live = 0;
+  // the elementwise grids, compacted to the running molecules likewise
+  // (compact_raggeds): each grid, its full form, host block prefix, the
+  // compacted prefix and block-to-molecule map
+  struct RgComp {
+    Ragged* rg;
+    Ragged full;
+    std::vector<int> pre, hbuf;
+    int *cpre = nullptr, *cbm = nullptr;
+  };
+  std::vector<RgComp> rgc;
+  int rg_live = 0;
   // molecules whose amplitude arenas hold meaningful amplitudes (a guess, a
   // solve or hcc_batch_set_amplitudes): HCC_GUESS_CURRENT needs all of them
   std::vector<char> amp_valid;
@@ -646,6 +657,17 @@ static void allocate(Batch& b, bool keep_mp2) {
   b.part2 = b.dalloc<double>(b.rgT2.nblocks);
   b.partE = b.dalloc<double>(2 * (long)b.rgT2S.nblocks);
   b.partD = b.dalloc<double>((long)b.rgND.nblocks * m);
+  b.rgc.clear();
+  b.rg_live = 0;
+  for (Ragged* r : {&b.rgNN, &b.rgN3, &b.rgT2, &b.rgT2S, &b.rgOV, &b.rgXX, &b.rgND, &b.rgN, &b.rgP2, &b.rgPairL,
+                    &b.rgPairBlk, &b.rgOcc, &b.rgT2x, &b.rgT2Sx}) {
+    Batch::RgComp c;
+    c.rg = r;
+    c.full = *r;
+    c.pre.resize(b.nm + 1);
+    HCHECK(hipMemcpy(c.pre.data(), r->prefix, (b.nm + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    b.rgc.push_back(std::move(c));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,11 +1038,46 @@ static bool compact_tables(hcc_ctx* c, bool dual, const std::vector<int>& act, i
   return true;
 }
 
-// full tables again (end of a solve: other callers run any molecule)
+// The elementwise grids compacted to the molecules with act != 0 (the running
+// set, which only shrinks during a solve), by the rule of compact_tables.  A
+// running molecule keeps its blocks in order, renumbered from its compacted
+// prefix, so every kernel's (molecule, block within the molecule) is as
+// before; finished molecules get empty ranges.  Returns whether the launches
+// changed.
+static bool compact_raggeds(hcc_ctx* c, const std::vector<int>& act, int alive) {
+  Batch& b = *c->batch;
+  if (b.arena != nullptr || alive <= 0) return false;
+  if (alive > kCompactFrac * (b.rg_live > 0 ? b.rg_live : b.nm)) return false;
+  for (Batch::RgComp& r : b.rgc) {
+    if (r.cpre == nullptr) {
+      r.cpre = b.dalloc<int>(b.nm + 1);
+      r.cbm = b.dalloc<int>(std::max(1, r.full.nblocks));
+    }
+    // the previous upload from hbuf has completed (a compaction follows a
+    // synchronisation of the stream)
+    r.hbuf.assign(b.nm + 1, 0);
+    int blocks = 0;
+    for (int k = 0; k < b.nm; ++k) {
+      r.hbuf[k] = blocks;
+      if (act[k] != 0) blocks += r.pre[k + 1] - r.pre[k];
+    }
+    r.hbuf[b.nm] = blocks;
+    HCHECK(hipMemcpyAsync(r.cpre, r.hbuf.data(), (b.nm + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HCHECK(launch_expand_prefix(r.cpre, b.nm, r.cbm, blocks, c->stream));
+    *r.rg = Ragged{r.cpre, r.cbm, blocks};
+  }
+  b.rg_live = alive;
+  return true;
+}
+
+// full tables and grids again (end of a solve: the energies and other callers
+// run every molecule)
 static void uncompact_tables(Batch& b) {
   for (auto* v : {&b.it, &b.dit, &b.alt})
     for (GemmTable& t : *v) t.ctiles = -1;
   b.it_live = b.dit_live = 0;
+  for (Batch::RgComp& r : b.rgc) *r.rg = r.full;
+  b.rg_live = 0;
 }
 struct UncompactGuard {
   Batch& b;
@@ -2408,11 +2465,13 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
     int alive = 0;
     for (int a : act) alive += a;
     if (alive == 0 || take_stop(ctx)) break;
-    if (compact_tables(ctx, false, act, alive, false) && graphs) {
+    const bool tables = compact_tables(ctx, false, act, alive, false);
+    if ((compact_raggeds(ctx, act, alive) || tables) && graphs) {
       block.reset();
       capture_block();
     }
   }
+  uncompact_tables(b);
   prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
   prof_close(ctx);
@@ -2489,7 +2548,10 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
       stopped = true;
       return 0;
     }
-    if (outer) compact_tables(ctx, false, act, n, false);
+    if (outer) {
+      compact_tables(ctx, false, act, n, false);
+      if (compact_raggeds(ctx, act, n)) inner_block.reset();
+    }
     if (compact_tables(ctx, true, act, n, outer)) inner_block.reset();
     return n;
   };
@@ -2563,6 +2625,7 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     mixed_commit(b.dm, b.ddm, b.dst, b.rgT2Sx, b.rgP2, b.nm, prm.keep_diis && prm.diis, s);
     prof_close(ctx);
   }
+  uncompact_tables(b);
   set_phase(b.dst, b.dact, b.nm, 0, s);
   prof_open(ctx, kProfEnergy);
   energy(b.dm, b.dst, b.nm, b.rgT2S, b.partE, s);
