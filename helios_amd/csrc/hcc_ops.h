@@ -46,6 +46,11 @@ void guess(const Mol* mols, State* st, int* act, Ragged rgT2S, int nm, int kind,
 void reset_state(State* st, int* act, int nm, hipStream_t s);
 void zero_t1(const Mol* mols, Ragged rgOV, int nm, hipStream_t s);
 void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s);
+// the same for the blocks [off[k], off[k] + len[k]) of the molecules with a
+// non-empty range in the block prefix `pre` (nm + 1 entries, 4 * 256 elements
+// per block; blocks = pre[nm])
+void to_bf16_runs(const double* src, __bf16* dst, const long* off, const long* len, const int* pre, int nm, int blocks,
+                  hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 void stop_running(State* st, int* act, int nm, hipStream_t s);
 void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s);

@@ -1531,6 +1531,29 @@ __global__ __launch_bounds__(kEwThreads) void k_to_bf16(const double* __restrict
     dst[e] = (__bf16)(float)src[e];
 }
 
+// BF16 copies of the running molecules' blocks of an FP64 arena: workgroup b
+// belongs to molecule k with pre[k] <= b < pre[k + 1] (molecules that are not
+// running have empty ranges) and converts 4 * kEwThreads elements of the
+// block [off[k], off[k] + len[k]) of the arena
+__global__ __launch_bounds__(kEwThreads) void k_to_bf16_runs(const double* __restrict__ src, __bf16* __restrict__ dst,
+                                                            const long* __restrict__ off, const long* __restrict__ len,
+                                                            const int* __restrict__ pre, int nm) {
+  const int k = block_mol(pre, nm, blockIdx.x);
+  const long e0 = off[k] + (long)(blockIdx.x - pre[k]) * (4 * kEwThreads) + threadIdx.x;
+  const long e1 = off[k] + len[k];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const long e = e0 + q * kEwThreads;
+    if (e < e1) dst[e] = (__bf16)(float)src[e];
+  }
+}
+
+void to_bf16_runs(const double* src, __bf16* dst, const long* off, const long* len, const int* pre, int nm, int blocks,
+                  hipStream_t s) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_to_bf16_runs, dim3(blocks), dim3(kEwThreads), 0, s, src, dst, off, len, pre, nm);
+}
+
 void to_bf16(const double* src, __bf16* dst, long n, hipStream_t s) {
   if (n <= 0) return;
   const long blocks = std::min<long>((n + kEwThreads - 1) / kEwThreads, 65536);

@@ -166,6 +166,12 @@ struct Batch {
     __bf16* copy;      // BF16 copy, same element offsets
     bool variable;     // changes at every outer point
     bool used;         // an operand of some correction GEMM (refreshed only then)
+    // per molecule: its block of the arena (offset from base, length), on the
+    // host and the device, and the block prefix over the running molecules
+    std::vector<long> hoff, hlen;
+    long *doff = nullptr, *dlen = nullptr;
+    std::vector<int> hpre;
+    int* dpre = nullptr;
   };
   std::vector<Shadow> shadows;
   // measured tile/split choices per (contraction, n_occ, n_virt); null: cost heuristic
@@ -615,7 +621,17 @@ static void allocate(Batch& b, bool keep_mp2) {
       const Mol& last = b.hm[b.nm - 1];
       double* base = first.*(f.member);
       const long count = (last.*(f.member) - base) + f.size(last.n, last.o, last.v);
-      b.shadows.push_back({base, count, b.dalloc<__bf16>(count), variable, false});
+      Batch::Shadow sh{base, count, b.dalloc<__bf16>(count), variable, false};
+      for (int k = 0; k < b.nm; ++k) {
+        sh.hoff.push_back(b.hm[k].*(f.member) - base);
+        sh.hlen.push_back(f.size(b.n[k], b.o[k], b.v[k]));
+      }
+      sh.doff = b.dalloc<long>(b.nm);
+      sh.dlen = b.dalloc<long>(b.nm);
+      sh.dpre = b.dalloc<int>(b.nm + 1);
+      HCHECK(hipMemcpy(sh.doff, sh.hoff.data(), b.nm * sizeof(long), hipMemcpyHostToDevice));
+      HCHECK(hipMemcpy(sh.dlen, sh.hlen.data(), b.nm * sizeof(long), hipMemcpyHostToDevice));
+      b.shadows.push_back(std::move(sh));
     };
     for (const auto& f : variable_fields()) add_shadow(f, true);
     for (const auto& f : constant_fields()) add_shadow(f, false);
@@ -2496,9 +2512,31 @@ static int run_solve(hcc_ctx* ctx, int guess_kind, int maxiter, double eps, bool
 }
 
 // BF16 copies of the outer-point operands (variable arenas; constants too when asked)
-static void refresh_shadows(hcc_ctx* c, bool constants) {
-  for (const auto& sh : c->batch->shadows)
-    if (sh.used && (sh.variable || constants)) to_bf16(sh.base, sh.copy, sh.count, c->stream);
+// With `act` (the running molecules, `alive` of them) and fewer than
+// kCompactFrac of the molecules running, the variable copies are refreshed for
+// the running molecules' blocks only: the others' copies are not read again in
+// this solve.
+static void refresh_shadows(hcc_ctx* c, bool constants, const std::vector<int>* act = nullptr, int alive = 0) {
+  Batch& b = *c->batch;
+  const bool runs = act != nullptr && alive < kCompactFrac * b.nm;
+  for (auto& sh : b.shadows) {
+    if (!sh.used || !(sh.variable || constants)) continue;
+    if (!runs || !sh.variable) {
+      to_bf16(sh.base, sh.copy, sh.count, c->stream);
+      continue;
+    }
+    // the previous upload from hpre has completed (a refresh follows a
+    // synchronisation of the stream)
+    sh.hpre.assign(b.nm + 1, 0);
+    int blocks = 0;
+    for (int k = 0; k < b.nm; ++k) {
+      sh.hpre[k] = blocks;
+      if ((*act)[k] != 0) blocks += (int)((sh.hlen[k] + 4 * kEwThreads - 1) / (4 * kEwThreads));
+    }
+    sh.hpre[b.nm] = blocks;
+    HCHECK(hipMemcpyAsync(sh.dpre, sh.hpre.data(), (b.nm + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    to_bf16_runs(sh.base, sh.copy, sh.doff, sh.dlen, sh.dpre, b.nm, blocks, c->stream);
+  }
   HCHECK(hipGetLastError());
 }
 
@@ -2601,10 +2639,11 @@ static int run_solve_mixed(hcc_ctx* ctx, int guess_kind, int maxiter, double eps
     ew_update(b.dm, b.dst, b.rgOV, b.rgT2x, b.nm, prm0, b.part1, b.part2, s);
     mixed_outer_control(b.dm, b.dst, b.dact, b.nm, prm, ctx->max_outer, b.rgOV, b.part1, b.rgT2x, b.part2, s);
     prof_close(ctx);
-    if (alive(true) == 0 || stopped) break;
+    const int n_running = alive(true);
+    if (n_running == 0 || stopped) break;
     prof_open(ctx, kProfCommit);
     set_phase(b.dst, b.dact, b.nm, 1, s);
-    if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0);
+    if (ctx->mixed_bf16) refresh_shadows(ctx, outer == 0, &act, n_running);
     mixed_zero(b.dm, b.ddm, b.dst, b.rgT2Sx, b.nm, s);
     prof_close(ctx);
     first_step();
