@@ -184,8 +184,8 @@ def copy_calibration():
 
 def cpu_baseline(mols, iters_fp64, threads_all):
     """The reference CPU path (oracle/refloops.c: the reference's loop nests and
-    OpenMP pattern, O(o^4 v^4) per iteration) on the host cores, as
-    BASELINE.md plans it: on one batch member per even N from 10 to 24, the R
+    OpenMP pattern, O(o^4 v^4) per iteration) on the host cores: on one batch
+    member per even N from 10 to 24, the R
     build and one iteration are timed with 1 thread and with every host
     thread (and N = 10, 12 solved in full, whose time the parts must predict);
     every molecule of the batch is then costed as build + (its FP64 GPU
