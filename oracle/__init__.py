@@ -32,8 +32,9 @@ relative to the helios source tree):
 The reference itself (Fortran, Intel-compiler extensions, ~20 modules with
 generated includes) is not built here: it has no stand-alone source set for
 this path that AMD flang compiles without stand-in modules, so there is no
-``oracle/_ref``.  ``bench.py`` times ``ccsd.solve`` (numpy + BLAS) on the
-GPU box's host cores as the CPU baseline ("port").
+``oracle/_ref``.  ``bench.py`` times ``refloops`` (a C restatement of the
+reference's loop nests, ``oracle/refloops.c``) on the host cores as the CPU
+baseline ("port").
 
 Pinning status of each function is recorded in the module docstrings.
 """
