@@ -53,7 +53,9 @@ void to_bf16_runs(const double* src, __bf16* dst, const long* off, const long* l
                   hipStream_t s);
 void activate(State* st, int* act, int nm, hipStream_t s);
 void stop_running(State* st, int* act, int nm, hipStream_t s);
-void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s);
+// restore = false: saves the active flags and activates every molecule, or
+// with `sel` (device, nm pointers) those whose pointer is set; true: restores
+void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s, const double* const* sel = nullptr);
 void lr_perturb(const Mol* mols, Ragged rgT2, int nm, const double* t1base, const double* t1b, const double* t2base,
                 const double* t2b, const double* const* r1p, const double* const* r2p, double h, hipStream_t s);
 void lr_omega(const Mol* mols, Ragged rgT2, int nm, double* const* o1p, double* const* o2p, bool minus, double scale,

@@ -911,20 +911,22 @@ void activate(State* st, int* act, int nm, hipStream_t s) {
 }
 // post-solve passes over the whole batch: arm every molecule, keeping its
 // active flag in save[] (statuses untouched); then restore the flags
-__global__ void k_arm_all(State* __restrict__ st, int* __restrict__ act, int* __restrict__ save, int nm, int restore) {
+__global__ void k_arm_all(State* __restrict__ st, int* __restrict__ act, int* __restrict__ save, int nm, int restore,
+                          const double* const* __restrict__ sel) {
   const int mol = blockIdx.x * blockDim.x + threadIdx.x;
   if (mol >= nm) return;
   if (restore) {
     st[mol].active = save[mol];
     act[mol] = save[mol];
   } else {
+    const int on = sel == nullptr || sel[mol] != nullptr;
     save[mol] = st[mol].active;
-    st[mol].active = 1;
-    act[mol] = 1;
+    st[mol].active = on;
+    act[mol] = on;
   }
 }
-void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s) {
-  hipLaunchKernelGGL(k_arm_all, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, save, nm, restore ? 1 : 0);
+void arm_all(State* st, int* act, int* save, int nm, bool restore, hipStream_t s, const double* const* sel) {
+  hipLaunchKernelGGL(k_arm_all, dim3((nm + 63) / 64), dim3(64), 0, s, st, act, save, nm, restore ? 1 : 0, sel);
 }
 // Linear response: t = t0 + h r (t0 saved in the backup arenas at the same
 // offsets; r given per molecule as device pointers to its r1 [i][a] and r2

@@ -1791,7 +1791,9 @@ static int lr_ready(hcc_ctx* ctx) {
 
 // sigma = (Omega(t0 + h r) - Omega(t0 - h r)) / 2h for every molecule whose r1
 // pointer is set; ptrs (device) = r1 (nm), r2 (nm), sigma1 (nm), sigma2 (nm).
-// The amplitudes and the active flags are restored.
+// Only those molecules are active during the two residual passes (the others'
+// GEMM tiles and elementwise blocks return at once).  The amplitudes and the
+// active flags are restored.
 static void lr_sigma_run(hcc_ctx* ctx, double h, double* const* ptrs) {
   Batch& b = *ctx->batch;
   hipStream_t s = ctx->stream;
@@ -1801,7 +1803,7 @@ static void lr_sigma_run(hcc_ctx* ctx, double h, double* const* ptrs) {
   const double* const* r2p = ptrs + nm;
   HCHECK(hipMemcpyAsync(b.dlr_t1b, first.t1, b.lr_span1 * sizeof(double), hipMemcpyDeviceToDevice, s));
   HCHECK(hipMemcpyAsync(b.dlr_t2b, first.t2, b.lr_span2 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  arm_all(b.dst, b.dact, b.dtsave, nm, false, s);
+  arm_all(b.dst, b.dact, b.dtsave, nm, false, s, r1p);
   lr_perturb(b.dm, b.rgT2, nm, first.t1, b.dlr_t1b, first.t2, b.dlr_t2b, r1p, r2p, h, s);
   residual(ctx);
   lr_omega(b.dm, b.rgT2, nm, ptrs + 2 * nm, ptrs + 3 * nm, false, 1.0, s);
