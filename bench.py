@@ -32,12 +32,12 @@ PEAK_BF16 = 2.5e15
 PEAK_HBM = 8.0e12        # HBM3E bytes/s
 # HBM bytes per dispatch of the dominant kernels on the default 4096-molecule batch
 # with the default solver settings: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
-# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --workers 0 --no-cpu
+# 1 GiB copy), two rocprofv3 --pmc passes of "bench.py --workers 0 --no-cpu --no-lr
 # --skip-reference --steps 1 --warmup 0 --no-profile" summarised in
 # profiles/hbm_traffic_batch4096.md.  A counter pass cannot run inside this process, so
 # other workloads report null.
-GEMM_TRAFFIC_BATCH4096 = 3.010e9        # gemm_f64_kernel, FP64 solve
-DUAL_GEMM_TRAFFIC_BATCH4096 = 1.360e9   # gemm_dual_bf16, mixed solve
+GEMM_TRAFFIC_BATCH4096 = 3.285e9        # gemm_f64_kernel, FP64 solve
+DUAL_GEMM_TRAFFIC_BATCH4096 = 1.468e9   # gemm_dual_bf16, mixed solve
 
 
 def _prep(k):
