@@ -123,10 +123,13 @@ def lr_leg(mols, nroots=3):
     into the sigma products (two FP64 residual passes each), the other
     vector-space calls (dot products, combinations, preconditioner) and the
     host's small eigenproblems and bookkeeping."""
+    # each molecule is loaded nwork = nroots + 4 times (copies solved alike):
+    # one Jacobian-product call then applies J to that many of its vectors
+    copies = nroots + 4
     ctx = capi.Context(0, capi.HCC_FP64, 20)
-    ctx.batch_load(mols)
+    ctx.batch_load([m for m in mols for _ in range(copies)])
     ctx.batch_solve(capi.GUESS_MP2, 500, 1e-10)
-    f_it, _ = ctx.gemm_flops(len(mols))   # residual GEMM flops per molecule and pass
+    f_it, _ = ctx.gemm_flops(copies * len(mols))   # residual GEMM flops per loaded molecule and pass
     timers = {"sigma": [0, 0.0], "space": [0, 0.0]}
 
     def timed(fn, key):
@@ -139,21 +142,22 @@ def lr_leg(mols, nroots=3):
         return run
 
     sigma_flops = [0.0]   # residual GEMM flops of the molecules each sigma call runs (two passes)
-    sigma = ctx.lr_sigma_slots
+    sigma = ctx.lr_sigma_from
 
-    def sigma_counted(in_slot, out_slot, h):
+    def sigma_counted(src, in_slot, out_slot, h):
         sigma_flops[0] += 2.0 * float(f_it[np.asarray(in_slot) >= 0].sum())
-        return sigma(in_slot, out_slot, h)
+        return sigma(src, in_slot, out_slot, h)
 
-    ctx.lr_sigma_slots = timed(sigma_counted, "sigma")
+    ctx.lr_sigma_from = timed(sigma_counted, "sigma")
     for name in ("lr_dots", "lr_combine", "lr_gram", "lr_combine_block", "lr_precondition", "lr_put", "lr_get",
-                 "lr_space"):
+                 "lr_space_counts"):
         setattr(ctx, name, timed(getattr(ctx, name), "space"))
     t1 = time.perf_counter()
     om, res, its, conv, ael, r0 = lrccsd.excitation_energies(
         ctx, [(m["nocc"], m["n"] - m["nocc"]) for m in mols], nroots=nroots, gammas=[m["G"] for m in mols],
         log=lambda it, active: print(f"# LR-CCSD iteration {it}: {active} molecules active", file=sys.stderr,
-                                     flush=True))
+                                     flush=True),
+        replicas=[[copies * k + j for j in range(copies)] for k in range(len(mols))])
     wall = time.perf_counter() - t1
     ctx.close()
     (ns, ts), (nv, tv) = timers["sigma"], timers["space"]
@@ -162,7 +166,8 @@ def lr_leg(mols, nroots=3):
            "host_s": wall - ts - tv, "mean_iterations": float(its.mean()),
            "converged": int(conv.sum()), "mean_omega_eV": [float(x) for x in np.nanmean(om, axis=0)],
            "mean_excitation_level": [float(x) for x in np.nanmean(ael, axis=0)],
-           "mean_reference_weight_r0": [float(x) for x in np.nanmean(r0, axis=0)], "start": "cis"}
+           "mean_reference_weight_r0": [float(x) for x in np.nanmean(r0, axis=0)], "start": "cis",
+           "copies_per_molecule": copies}
     # a sigma call runs two residual passes over the molecules that have a vector in it
     sig_rate = sigma_flops[0] / max(ts, 1e-9)
     out["roofline_sigma"] = {"bound": "mfma", "achieved": sig_rate / 1e12, "peak": PEAK_FP64 / 1e12,

@@ -60,6 +60,8 @@ SIGNATURES = {
     "hcc_lr_combine_block": (_i, [_vp, _i, _P(_i), _P(_d), _i, _P(_i), _i, _P(_d)]),
     "hcc_lr_precondition": (_i, [_vp, _i, _P(_i), _P(_d), _d]),
     "hcc_lr_sigma_slots": (_i, [_vp, _d, _P(_i), _P(_i)]),
+    "hcc_lr_space_counts": (_i, [_vp, _P(_i)]),
+    "hcc_lr_sigma_from": (_i, [_vp, _d, _P(_i), _P(_i), _P(_i)]),
     "hcc_batch_load_scf": (_i, [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_d), _P(_d), _d, _i, _P(_d), _P(_i), _P(_i)]),
     "hcc_batch_results": (_i, [_vp, _P(_d), _P(_i), _P(_i)]),
     "hcc_last_timing": (_i, [_vp, _P(_d), _P(_d)]),
@@ -358,6 +360,16 @@ class Context:
         shift = np.ascontiguousarray(shift, dtype=np.float64)
         self._check(self.lib.hcc_lr_precondition(self.h, len(jobs), _ip(jobs), _dp(shift), float(floor)),
                     "hcc_lr_precondition")
+
+    def lr_space_counts(self, counts):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        self._check(self.lib.hcc_lr_space_counts(self.h, _ip(c)), "hcc_lr_space_counts")
+
+    def lr_sigma_from(self, src, in_slot, out_slot, h):
+        m = np.ascontiguousarray(src, dtype=np.int32)
+        a = np.ascontiguousarray(in_slot, dtype=np.int32)
+        b = np.ascontiguousarray(out_slot, dtype=np.int32)
+        self._check(self.lib.hcc_lr_sigma_from(self.h, float(h), _ip(m), _ip(a), _ip(b)), "hcc_lr_sigma_from")
 
     def lr_sigma_slots(self, in_slot, out_slot, h):
         a = np.ascontiguousarray(in_slot, dtype=np.int32)

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <atomic>
@@ -219,7 +220,8 @@ struct Batch {
   // molecules whose amplitude arenas hold meaningful amplitudes (a guess, a
   // solve or hcc_batch_set_amplitudes): HCC_GUESS_CURRENT needs all of them
   std::vector<char> amp_valid;
-  int lr_slots = 0;
+  int lr_slots = 0;            // largest slot count of a molecule; 0: no space
+  std::vector<int> lr_nslot;   // slots of each molecule
   double* dlr_space = nullptr;
   std::vector<long> lr_len, lr_base;
   long *dlr_len = nullptr, *dlr_base = nullptr;
@@ -1840,13 +1842,7 @@ int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r
 // ---------------------------------------------------------------------------
 // Device-resident vector space of the eigensolver (hcc_lr.h)
 // ---------------------------------------------------------------------------
-int hcc_lr_space(hcc_ctx* ctx, int nslots) {
-  if (ctx == nullptr) return HCC_E_ARG;
-  if (int r = lr_ready(ctx)) return r;
-  if (nslots < 1) {
-    ctx->err = "the vector space needs at least one slot";
-    return HCC_E_ARG;
-  }
+static int lr_space_impl(hcc_ctx* ctx, const std::vector<int>& counts) {
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   Batch& b = *ctx->batch;
@@ -1858,13 +1854,14 @@ int hcc_lr_space(hcc_ctx* ctx, int nslots) {
   for (int k = 0; k < nm; ++k) {
     b.lr_len[k] = (long)b.o[k] * b.v[k] + (long)b.o[k] * b.o[k] * b.v[k] * b.v[k];
     b.lr_base[k] = total;
-    total += (b.lr_len[k] * nslots + 31) / 32 * 32;   // 256-byte aligned molecule regions
+    total += (b.lr_len[k] * counts[k] + 31) / 32 * 32;   // 256-byte aligned molecule regions
   }
   if (b.dlr_space) HCHECK(hipFree(b.dlr_space));
   b.dlr_space = nullptr;
   b.lr_slots = 0;
-  HCHECK(hipMalloc(&b.dlr_space, (size_t)total * sizeof(double)));
-  HCHECK(hipMemsetAsync(b.dlr_space, 0, (size_t)total * sizeof(double), ctx->stream));
+  b.lr_nslot.clear();
+  HCHECK(hipMalloc(&b.dlr_space, (size_t)std::max<long>(total, 32) * sizeof(double)));
+  HCHECK(hipMemsetAsync(b.dlr_space, 0, (size_t)std::max<long>(total, 32) * sizeof(double), ctx->stream));
   if (b.dlr_len == nullptr) {
     b.dlr_len = b.dalloc<long>(nm);
     b.dlr_base = b.dalloc<long>(nm);
@@ -1872,8 +1869,30 @@ int hcc_lr_space(hcc_ctx* ctx, int nslots) {
   HCHECK(hipMemcpy(b.dlr_len, b.lr_len.data(), nm * sizeof(long), hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(b.dlr_base, b.lr_base.data(), nm * sizeof(long), hipMemcpyHostToDevice));
   HCHECK(hipStreamSynchronize(ctx->stream));
-  b.lr_slots = nslots;
+  b.lr_nslot = counts;
+  b.lr_slots = *std::max_element(counts.begin(), counts.end());
   API_END
+}
+
+int hcc_lr_space(hcc_ctx* ctx, int nslots) {
+  if (ctx == nullptr) return HCC_E_ARG;
+  if (int r = lr_ready(ctx)) return r;
+  if (nslots < 1) {
+    ctx->err = "the vector space needs at least one slot";
+    return HCC_E_ARG;
+  }
+  return lr_space_impl(ctx, std::vector<int>(ctx->batch->nm, nslots));
+}
+
+int hcc_lr_space_counts(hcc_ctx* ctx, const int* nslots) {
+  if (ctx == nullptr || nslots == nullptr) return HCC_E_ARG;
+  if (int r = lr_ready(ctx)) return r;
+  std::vector<int> counts(nslots, nslots + ctx->batch->nm);
+  if (*std::min_element(counts.begin(), counts.end()) < 0 || *std::max_element(counts.begin(), counts.end()) < 1) {
+    ctx->err = "slot counts must be non-negative, and some molecule needs a slot";
+    return HCC_E_ARG;
+  }
+  return lr_space_impl(ctx, counts);
 }
 
 static int lr_space_ready(hcc_ctx* ctx) {
@@ -1886,7 +1905,7 @@ static int lr_space_ready(hcc_ctx* ctx) {
 }
 
 static bool lr_slot_ok(const Batch& b, int mol, int slot) {
-  return mol >= 0 && mol < b.nm && slot >= 0 && slot < b.lr_slots;
+  return mol >= 0 && mol < b.nm && slot >= 0 && slot < b.lr_nslot[mol];
 }
 
 // staging region of at least `bytes` (grown by doubling; contents not kept)
@@ -2152,8 +2171,9 @@ int hcc_lr_combine_block(hcc_ctx* ctx, int njobs, const int* jobs, const double*
   API_END
 }
 
-int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* out_slot) {
-  if (ctx == nullptr || in_slot == nullptr || out_slot == nullptr) return HCC_E_ARG;
+// sigma of molecule k's trial vector, slot in_slot[k] of molecule src[k] (k
+// when src is null), into slot out_slot[k] of molecule src[k]
+static int lr_sigma_from(hcc_ctx* ctx, double h, const int* src, const int* in_slot, const int* out_slot) {
   if (int r = lr_space_ready(ctx)) return r;
   if (!(h > 0.0)) {
     ctx->err = "the difference step must be positive";
@@ -2162,26 +2182,47 @@ int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* ou
   Batch& b = *ctx->batch;
   const int nm = b.nm;
   std::vector<double*> ptrs(4 * (size_t)nm, nullptr);
+  std::set<std::pair<int, int>> written;
   for (int k = 0; k < nm; ++k) {
     if (in_slot[k] < 0) continue;
-    if (!lr_slot_ok(b, k, in_slot[k]) || !lr_slot_ok(b, k, out_slot[k]) || in_slot[k] == out_slot[k]) {
-      ctx->err = "sigma slots of molecule " + std::to_string(k) + " out of range or equal";
+    const int m = src != nullptr ? src[k] : k;
+    const bool same = m >= 0 && m < nm && b.n[m] == b.n[k] && b.o[m] == b.o[k] && b.v[m] == b.v[k];
+    if (!same || !lr_slot_ok(b, m, in_slot[k]) || !lr_slot_ok(b, m, out_slot[k]) || in_slot[k] == out_slot[k] ||
+        !written.insert({m, out_slot[k]}).second) {
+      ctx->err = "sigma of molecule " + std::to_string(k) +
+                 ": source molecule of other dimensions, slots out of range or equal, or a sigma slot written twice";
       return HCC_E_ARG;
     }
     const long n1 = (long)b.o[k] * b.v[k];
-    double* r = b.dlr_space + b.lr_base[k] + (long)in_slot[k] * b.lr_len[k];
-    double* q = b.dlr_space + b.lr_base[k] + (long)out_slot[k] * b.lr_len[k];
+    double* r = b.dlr_space + b.lr_base[m] + (long)in_slot[k] * b.lr_len[m];
+    double* q = b.dlr_space + b.lr_base[m] + (long)out_slot[k] * b.lr_len[m];
     ptrs[k] = r;
     ptrs[nm + k] = r + n1;
     ptrs[2 * nm + k] = q;
     ptrs[3 * nm + k] = q + n1;
   }
+  // an input slot that some molecule also writes would be read after its write
+  for (int k = 0; k < nm; ++k)
+    if (in_slot[k] >= 0 && written.count({src != nullptr ? src[k] : k, in_slot[k]})) {
+      ctx->err = "sigma of molecule " + std::to_string(k) + ": its input slot is another product's output";
+      return HCC_E_ARG;
+    }
   API_BEGIN
   HCHECK(hipSetDevice(ctx->device));
   HCHECK(hipMemcpyAsync(b.dlr_ps, ptrs.data(), ptrs.size() * sizeof(double*), hipMemcpyHostToDevice, ctx->stream));
   lr_sigma_run(ctx, h, b.dlr_ps);
   HCHECK(hipStreamSynchronize(ctx->stream));
   API_END
+}
+
+int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* out_slot) {
+  if (ctx == nullptr || in_slot == nullptr || out_slot == nullptr) return HCC_E_ARG;
+  return lr_sigma_from(ctx, h, nullptr, in_slot, out_slot);
+}
+
+int hcc_lr_sigma_from(hcc_ctx* ctx, double h, const int* src, const int* in_slot, const int* out_slot) {
+  if (ctx == nullptr || src == nullptr || in_slot == nullptr || out_slot == nullptr) return HCC_E_ARG;
+  return lr_sigma_from(ctx, h, src, in_slot, out_slot);
 }
 
 int hcc_batch_get_orbitals(hcc_ctx* ctx, int mol, double* C, double* F_mo, double* e_ref) {

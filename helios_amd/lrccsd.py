@@ -210,13 +210,17 @@ class _Sub:
 
 
 class _Ops:
-    """Batched calls on the device space, with job lists built per molecule."""
+    """Batched calls on the device space, with job lists built per molecule
+    (subspace k lives in the space of loaded molecule home[k])."""
 
-    def __init__(self, ctx):
+    def __init__(self, ctx, home):
         self.ctx = ctx
+        self.home = np.asarray(home, dtype=np.int32)
 
     def dots(self, jobs, mode=0):
-        return self.ctx.lr_dots(np.asarray(jobs, dtype=np.int32).reshape(-1, 3), mode)
+        jobs = np.array(jobs, dtype=np.int32).reshape(-1, 3)
+        jobs[:, 0] = self.home[jobs[:, 0]]
+        return self.ctx.lr_dots(jobs, mode)
 
     def combine(self, jobs):
         """jobs: list of (mol, dst, beta, [(slot, coef), ...])."""
@@ -224,7 +228,7 @@ class _Ops:
             return
         head, beta, ts, tc = [], [], [], []
         for mol, dst, b, terms in jobs:
-            head.append((mol, dst, len(ts), len(terms)))
+            head.append((self.home[mol], dst, len(ts), len(terms)))
             beta.append(b)
             for sl, c in terms:
                 ts.append(sl)
@@ -241,7 +245,7 @@ class _Ops:
             m, t = coef.shape
             if m == 0:
                 continue
-            heads.append((mol, m, t, ns, ns + m, nc))
+            heads.append((self.home[mol], m, t, ns, ns + m, nc))
             betas.append(beta)
             sls.append(np.asarray(dst, dtype=np.int32))
             sls.append(np.asarray(slots, dtype=np.int32))
@@ -258,7 +262,7 @@ class _Ops:
         heads, sls = [], []
         ns = 0
         for mol, a, b in blocks:
-            heads.append((mol, len(a), len(b), ns, ns + len(a)))
+            heads.append((self.home[mol], len(a), len(b), ns, ns + len(a)))
             sls.append(np.asarray(a, dtype=np.int32))
             sls.append(np.asarray(b, dtype=np.int32))
             ns += len(a) + len(b)
@@ -415,7 +419,7 @@ def _ritz(s, lam, Y):
 
 
 def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_space=48, log=None,
-                        gammas=None):
+                        gammas=None, replicas=None):
     """Lowest ``nroots`` singlet LR-CCSD excitation energies (eV) of every
     loaded molecule, after a converged solve of ``ctx``'s batch.
 
@@ -427,21 +431,34 @@ def excitation_energies(ctx, dims, nroots=1, eps=1e-6, maxiter=100, h=1e-5, max_
     iterations [nmol], converged [nmol], average excitation levels
     [nmol, nroots], reference weights r0 [nmol, nroots] -- NaN without
     ``gammas``).  ``log(iteration, active)`` is called after every Davidson
-    iteration when given."""
+    iteration when given.
+
+    replicas: for a batch that holds several copies of each molecule (loaded
+    from the same inputs and solved alike, so with the same amplitudes), the
+    batch indices of each molecule's copies, one list per entry of ``dims``;
+    the first holds the molecule's subspace, and one Jacobian product call
+    applies J to as many of its vectors as it has copies (the products are
+    those of the single molecule, bit for bit)."""
     # the host's matrices are small (projected problems of a few dozen rows,
     # CIS matrices of a few hundred): several BLAS threads only compete
     with threadpool_limits(1):
-        return _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gammas)
+        return _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gammas, replicas)
 
 
-def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gammas):
+def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gammas, replicas):
     L = _Layout(nroots, max_space)
-    ctx.lr_space(L.nslots)
-    ops = _Ops(ctx)
+    home = [r[0] for r in replicas] if replicas is not None else list(range(len(dims)))
+    if replicas is not None:
+        counts = np.zeros(ctx.nmol, dtype=np.int32)
+        counts[home] = L.nslots
+        ctx.lr_space_counts(counts)
+    else:
+        ctx.lr_space(L.nslots)
+    ops = _Ops(ctx, home)
     subs = []
     cands = {}
     for k, (o, v) in enumerate(dims):
-        C, Fmo, _ = ctx.batch_orbitals(k, o + v)
+        C, Fmo, _ = ctx.batch_orbitals(home[k], o + v)
         e = np.diag(Fmo).copy()
         s = _Sub(o, v, e, nroots, L)
         subs.append(s)
@@ -466,7 +483,7 @@ def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gam
         starts.extend(s.unit(e1, e2) for e1, e2 in s.double_seeds(2))
         cands[k] = []
         for q, x in enumerate(starts):
-            ctx.lr_put(k, L.C + q, x)
+            ctx.lr_put(home[k], L.C + q, x)
             cands[k].append((L.C + q, 1.0))
     _orthonormalize(ops, L, subs, cands, 1e-3)
     its = np.zeros(len(subs), dtype=int)
@@ -477,16 +494,33 @@ def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gam
                 s.done = True
         if not act:
             break
-        # sigmas of the pending vectors, one batched product per column
-        for c in range(max(subs[k].pending for k in act)):
-            ins = np.full(len(subs), -1, dtype=np.int32)
-            outs = np.zeros(len(subs), dtype=np.int32)
-            for k in act:
-                s = subs[k]
-                if c < s.pending:
-                    ins[k] = s.V(L) + s.k + c
-                    outs[k] = s.S(L) + s.k + c
-            ctx.lr_sigma_slots(ins, outs, h)
+        # sigmas of the pending vectors: one batched product per column, or
+        # with replicas, one per len(replicas[k]) columns
+        if replicas is None:
+            for c in range(max(subs[k].pending for k in act)):
+                ins = np.full(len(subs), -1, dtype=np.int32)
+                outs = np.zeros(len(subs), dtype=np.int32)
+                for k in act:
+                    s = subs[k]
+                    if c < s.pending:
+                        ins[k] = s.V(L) + s.k + c
+                        outs[k] = s.S(L) + s.k + c
+                ctx.lr_sigma_slots(ins, outs, h)
+        else:
+            rounds = max(-(-subs[k].pending // len(replicas[k])) for k in act)
+            for r in range(rounds):
+                src = np.zeros(ctx.nmol, dtype=np.int32)
+                ins = np.full(ctx.nmol, -1, dtype=np.int32)
+                outs = np.zeros(ctx.nmol, dtype=np.int32)
+                for k in act:
+                    s = subs[k]
+                    for j, b in enumerate(replicas[k]):
+                        c = r * len(replicas[k]) + j
+                        if c < s.pending:
+                            src[b] = home[k]
+                            ins[b] = s.V(L) + s.k + c
+                            outs[b] = s.S(L) + s.k + c
+                ctx.lr_sigma_from(src, ins, outs, h)
         _absorb(ops, L, subs, act)
         # Ritz vectors and their sigmas into the Ritz buffer
         ritz = {}
@@ -561,7 +595,7 @@ def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gam
             corr[k] = keep
             if s.k + len(keep) > s.max_space:
                 collapse.append(k)
-        ops_pre = [(k, sl) for k, sl in pj if k in corr]
+        ops_pre = [(home[k], sl) for k, sl in pj if k in corr]
         ctx.lr_precondition(np.array(ops_pre, dtype=np.int32).reshape(-1, 2),
                             np.array([sh for (k, _), sh in zip(pj, shifts) if k in corr]), 1e-2)
         live = list(corr)
@@ -588,7 +622,7 @@ def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gam
                     if got:
                         seeds[k] = []
                         for q, (e1, e2) in enumerate(got):
-                            ctx.lr_put(k, L.C + q, s.unit(e1, e2))
+                            ctx.lr_put(home[k], L.C + q, s.unit(e1, e2))
                             seeds[k].append((L.C + q, 1.0))
             if not seeds:
                 break
@@ -603,13 +637,13 @@ def _excitation_energies(ctx, dims, nroots, eps, maxiter, h, max_space, log, gam
     for k, s in enumerate(subs):
         g = f_ov = t1 = None
         if gammas is not None:
-            C, Fmo, _ = ctx.batch_orbitals(k, s.o + s.v)
+            C, Fmo, _ = ctx.batch_orbitals(home[k], s.o + s.v)
             g = ovov_integrals(C, np.asarray(gammas[k], dtype=np.float64), s.o)
             f_ov = Fmo[:s.o, s.o:]
-            t1, _ = ctx.batch_amplitudes(k, s.o, s.v, doubles=False)
+            t1, _ = ctx.batch_amplitudes(home[k], s.o, s.v, doubles=False)
         for r, sl in enumerate(s.root_slot):
             if sl is not None:
-                x = ctx.lr_get(k, sl, s.N)
+                x = ctx.lr_get(home[k], sl, s.N)
                 x1, x2 = x[:s.n1].reshape(s.o, s.v), x[s.n1:].reshape(s.o, s.o, s.v, s.v)
                 ael[k, r] = average_excitation_level(x1, x2)
                 if g is not None:

@@ -215,6 +215,10 @@ int hcc_batch_lr_sigma(hcc_ctx* ctx, double h, const double* r1, const double* r
  * pair-symmetric doubles [i][j][a][b]), zeroed.  A second call replaces the
  * space.  Needs solved amplitudes (as hcc_batch_lr_sigma). */
 int hcc_lr_space(hcc_ctx* ctx, int nslots);
+/* As hcc_lr_space with nslots[k] slots for molecule k (0: none), for batches
+ * that also hold replicas of a molecule -- the same inputs and amplitudes --
+ * that only compute its Jacobian products (hcc_lr_sigma_from). */
+int hcc_lr_space_counts(hcc_ctx* ctx, const int* nslots);
 /* Copy one vector (o v + o^2 v^2 doubles) into / out of slot `slot` of molecule `mol`. */
 int hcc_lr_put(hcc_ctx* ctx, int mol, int slot, const double* x);
 int hcc_lr_get(hcc_ctx* ctx, int mol, int slot, double* x);
@@ -246,6 +250,13 @@ int hcc_lr_precondition(hcc_ctx* ctx, int njobs, const int* jobs, const double* 
 /* sigma = J r as hcc_batch_lr_sigma, from slot in_slot[k] into slot
  * out_slot[k] of every molecule k (in_slot[k] < 0: molecule skipped). */
 int hcc_lr_sigma_slots(hcc_ctx* ctx, double h, const int* in_slot, const int* out_slot);
+/* As hcc_lr_sigma_slots, with molecule k's trial vector read from slot
+ * in_slot[k] of molecule src[k] and its sigma written to slot out_slot[k] of
+ * molecule src[k]: molecule k must have src[k]'s dimensions, and for sigma to
+ * be src[k]'s Jacobian product, its amplitudes (a replica loaded with the same
+ * inputs and solved alike).  One call then applies J to several vectors of
+ * one molecule.  No slot may be written twice or be both read and written. */
+int hcc_lr_sigma_from(hcc_ctx* ctx, double h, const int* src, const int* in_slot, const int* out_slot);
 
 /* MO coefficients C (n x n column-major, = hV), MO Fock matrix and reference
  * energy of loaded molecule `mol` (either array may be null): the values the

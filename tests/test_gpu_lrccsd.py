@@ -277,6 +277,46 @@ def test_block_gram_and_combinations():
     ctx.close()
 
 
+def test_replicas_equal_single_copies():
+    """Three copies of each molecule, loaded from the same inputs and solved
+    together: with the copies applying J to three vectors per call, the roots,
+    residuals, iteration counts, excitation levels and r0 equal those of the
+    molecules alone bit for bit (the copies' amplitudes and products are the
+    molecule's own); invalid sources are rejected."""
+    mols = [M.polyene(6), M.random_fragment(10, seed=4), M.random_fragment(12, seed=2)]
+    preps = [prepare(m) for m in mols]
+    dims = [(m.nocc, m.n - m.nocc) for m in mols]
+    gam = [p["G"] for p in preps]
+    ctx = _solved_ctx(preps)
+    ref = lrccsd.excitation_energies(ctx, dims, nroots=3, gammas=gam)
+    ctx.close()
+    R = 3
+    rep = _solved_ctx([p for p in preps for _ in range(R)])
+    got = lrccsd.excitation_energies(rep, dims, nroots=3, gammas=gam,
+                                     replicas=[[R * k + j for j in range(R)] for k in range(len(mols))])
+    print("iterations", ref[2], got[2])
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+    nm = R * len(mols)
+    src = np.zeros(nm, dtype=np.int32)
+    ins = np.full(nm, -1, dtype=np.int32)
+    outs = np.zeros(nm, dtype=np.int32)
+    ins[R] = 0
+    outs[R] = 1
+    with pytest.raises(capi.HccError):   # a copy of the second molecule takes the third's space as source
+        src[R] = R * 2
+        rep.lr_sigma_from(src, ins, outs, 1e-5)
+    with pytest.raises(capi.HccError):   # two copies write the same slot
+        src[:] = 0
+        ins[:2] = [0, 2]
+        outs[:2] = [1, 1]
+        ins[R] = -1
+        rep.lr_sigma_from(src, ins, outs, 1e-5)
+    with pytest.raises(capi.HccError):   # a copy without a space
+        rep.lr_put(1, 0, np.zeros(dims[0][0] * dims[0][1] * (1 + dims[0][0] * dims[0][1])))
+    rep.close()
+
+
 def test_device_davidson_equals_oracle_driven_at_n30():
     """Beyond the sizes whose dense Jacobian can be formed: for a 30-site
     batch member (25650 coordinates) the device Davidson (vectors in device
